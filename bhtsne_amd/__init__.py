@@ -1,0 +1,367 @@
+"""bhtsne_amd -- the bhtsne hot path (sparse affinities + per-epoch gradient) on MI355X.
+
+``TSNE`` mirrors the reference builder ``tSNE<'data, f32, &[f32], D>`` (src/lib.rs:149-1268):
+the same setters and defaults, ``barnes_hut`` / ``fit_sne`` and their ``*_with_neighbors``
+variants, ``embedding()``, ``kl_divergence()``, ``affinities()`` / ``with_affinities()``, and the
+same rejection messages (raised as ``InvalidInput``).  Every numeric step runs in the HIP library
+(libbhtsne_hip.so, C ABI in include/bhtsne_hip.h); the metric is Euclidean.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Callable, NamedTuple, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import BhtError, InvalidInput, check, ptr
+
+__all__ = ["Context", "Neighbor", "SparseAffinities", "TSNE", "BhtError", "InvalidInput",
+           "knn", "search_beta", "symmetrize", "default_context"]
+
+
+class Context:
+    """One device context (twiddle tables, HBM budget accounting)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        L = _lib.load()
+        h = C.c_void_p()
+        check(L.bht_ctx_create(int(device), int(hbm_budget), C.byref(h)))
+        self._h = h
+        self._L = L
+
+    @property
+    def handle(self):
+        return self._h
+
+    def bytes_in_use(self) -> int:
+        return int(self._L.bht_ctx_bytes_in_use(self._h))
+
+    def close(self):
+        if self._h:
+            self._L.bht_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = None
+
+
+def default_context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+class Neighbor(NamedTuple):
+    """A sample's nearest neighbour: index and distance (src/lib.rs:107-111)."""
+    index: int
+    distance: float
+
+
+@dataclass
+class SparseAffinities:
+    """CSR affinities (src/lib.rs:116-144): usize rows, u32 columns, f32 values, perplexity."""
+    rows: np.ndarray
+    columns: np.ndarray
+    values: np.ndarray
+    perplexity: float
+
+
+def _neighbors_arrays(neighbors):
+    """Flatten &[Vec<Neighbor<f32>>]-like input into (index u64, distance f32, row_lens u64)."""
+    if isinstance(neighbors, tuple) and len(neighbors) == 2:
+        idx = np.ascontiguousarray(neighbors[0], np.uint64)
+        dist = np.ascontiguousarray(neighbors[1], np.float32)
+        n, k = idx.shape
+        return idx.ravel(), dist.ravel(), np.full(n, k, np.uint64)
+    lens = np.array([len(r) for r in neighbors], np.uint64)
+    idx = np.array([nb[0] for r in neighbors for nb in r], np.uint64)
+    dist = np.array([nb[1] for r in neighbors for nb in r], np.float32)
+    return idx, dist, lens
+
+
+class TSNE:
+    """tSNE builder (src/lib.rs:241-435) over the MI355X engine; Euclidean metric only."""
+
+    def __init__(self, data, d_out: int = 2, ctx: Optional[Context] = None):
+        self._ctx = ctx or default_context()
+        self._L = _lib.load()
+        x = np.ascontiguousarray(data, np.float32)
+        if x.ndim == 1:
+            x = x.reshape(-1, 1)
+        self._x = x
+        self._n = x.shape[0]
+        self._d = int(d_out)
+        p = _lib.Params()
+        self._L.bht_params_default(C.byref(p))
+        p.d_out = self._d
+        h = C.c_void_p()
+        check(self._L.bht_job_create(self._ctx.handle, self._n, C.byref(p), C.byref(h)))
+        self._h = h
+        if x.shape[1] > 0 and self._n > 0:
+            check(self._L.bht_job_set_data_f32(self._h, ptr(x), self._n, x.shape[1]))
+        self._callback = None
+        self._fitted = False
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._L.bht_job_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # ---- parameters (src/lib.rs:277-371) ----------------------------------------------------
+    def _params(self):
+        p = _lib.Params()
+        check(self._L.bht_job_get_params(self._h, C.byref(p)))
+        return p
+
+    def _set(self, **kv):
+        p = self._params()
+        for k, v in kv.items():
+            setattr(p, k, v)
+        check(self._L.bht_job_set_params(self._h, C.byref(p)))
+        return self
+
+    def learning_rate(self, v: float):
+        return self._set(learning_rate=float(v))
+
+    def epochs(self, v: int):
+        return self._set(epochs=int(v))
+
+    def momentum(self, v: float):
+        return self._set(momentum=float(v))
+
+    def final_momentum(self, v: float):
+        return self._set(final_momentum=float(v))
+
+    def momentum_switch_epoch(self, v: int):
+        return self._set(momentum_switch_epoch=int(v))
+
+    def stop_lying_epoch(self, v: int):
+        return self._set(stop_lying_epoch=int(v))
+
+    def early_exaggeration(self, v: float):
+        return self._set(early_exaggeration=float(v))
+
+    def perplexity(self, v: float):
+        return self._set(perplexity=float(v))
+
+    def min_boxes(self, v: int):
+        """FIt-SNE grid lower bound per axis (10 reproduces the reference)."""
+        return self._set(min_boxes=int(v))
+
+    def max_boxes(self, v: int):
+        return self._set(max_boxes=int(v))
+
+    def bandwidth_mode(self, v: int):
+        return self._set(bandwidth_mode=int(v))
+
+    def epoch_callback(self, cb: Callable[[int, np.ndarray], None]):
+        self._callback = cb
+        return self
+
+    def initial_embedding(self, y):
+        y = np.ascontiguousarray(y, np.float32).ravel()
+        check(self._L.bht_job_set_initial_embedding(self._h, ptr(y), y.size))
+        return self
+
+    def params(self) -> dict:
+        p = self._params()
+        return {name: getattr(p, name) for name, _ in _lib.Params._fields_}
+
+    # ---- fits --------------------------------------------------------------------------------
+    def _cb(self):
+        if self._callback is None:
+            return _lib.NULL_CB
+        user = self._callback
+
+        def tramp(epoch, data, length, _u):
+            user(int(epoch), np.ctypeslib.as_array(data, (int(length),)).copy())
+        self._tramp = _lib.EPOCH_CB(tramp)
+        return self._tramp
+
+    @staticmethod
+    def _euclid(metric):
+        if metric not in (None, "euclidean"):
+            raise NotImplementedError("the MI355X engine implements the Euclidean metric only")
+
+    def barnes_hut(self, theta: float, metric="euclidean"):
+        self._euclid(metric)
+        check(self._L.bht_job_barnes_hut(self._h, float(theta), self._cb(), None))
+        self._fitted = True
+        return self
+
+    def barnes_hut_with_neighbors(self, theta: float, neighbors):
+        idx, dist, lens = _neighbors_arrays(neighbors)
+        check(self._L.bht_job_barnes_hut_with_neighbors(self._h, float(theta), ptr(idx),
+                                                        ptr(dist), ptr(lens), lens.size,
+                                                        self._cb(), None))
+        self._fitted = True
+        return self
+
+    def fit_sne(self, metric="euclidean"):
+        self._euclid(metric)
+        check(self._L.bht_job_fit_sne(self._h, self._cb(), None))
+        self._fitted = True
+        return self
+
+    def fit_sne_with_neighbors(self, neighbors):
+        idx, dist, lens = _neighbors_arrays(neighbors)
+        check(self._L.bht_job_fit_sne_with_neighbors(self._h, ptr(idx), ptr(dist), ptr(lens),
+                                                     lens.size, self._cb(), None))
+        self._fitted = True
+        return self
+
+    # ---- outputs -----------------------------------------------------------------------------
+    def embedding(self) -> np.ndarray:
+        out = np.zeros(self._n * self._d, np.float32)
+        check(self._L.bht_job_get_embedding(self._h, ptr(out), out.size))
+        return out
+
+    def kl_divergence_pair(self):
+        """(reference-compatible KL, true KL) or None before any fit."""
+        r = C.c_double()
+        t = C.c_double()
+        st = self._L.bht_job_kl_divergence(self._h, C.byref(r), C.byref(t))
+        if st == _lib.BHT_ERR_STATE and not self._fitted:
+            return None
+        check(st)
+        return r.value, t.value
+
+    def kl_divergence(self):
+        pair = self.kl_divergence_pair()
+        return None if pair is None else pair[0]
+
+    def affinities(self) -> Optional[SparseAffinities]:
+        nnz = C.c_uint64(0)
+        perp = C.c_float(0)
+        st = self._L.bht_job_get_affinities(self._h, None, None, None, C.byref(nnz),
+                                            C.byref(perp))
+        if st == _lib.BHT_ERR_STATE:
+            return None
+        check(st)
+        rows = np.zeros(self._n + 1, np.uint64)
+        cols = np.zeros(nnz.value, np.uint32)
+        vals = np.zeros(nnz.value, np.float32)
+        check(self._L.bht_job_get_affinities(self._h, ptr(rows), ptr(cols), ptr(vals),
+                                             C.byref(nnz), C.byref(perp)))
+        return SparseAffinities(rows, cols, vals, perp.value)
+
+    def with_affinities(self, aff: SparseAffinities):
+        rows = np.ascontiguousarray(aff.rows, np.uint64)
+        cols = np.ascontiguousarray(aff.columns, np.uint32)
+        vals = np.ascontiguousarray(aff.values, np.float32)
+        check(self._L.bht_job_set_affinities(self._h, ptr(rows), ptr(cols), ptr(vals), rows.size,
+                                             cols.size, float(aff.perplexity)))
+        return self
+
+    # ---- stepping interface (bench, parity checks) --------------------------------------------
+    def prepare(self, method: str = "fitsne", theta: float = 0.5):
+        m = _lib.METHOD_FITSNE if method == "fitsne" else _lib.METHOD_BARNES_HUT
+        check(self._L.bht_job_prepare(self._h, m, float(theta)))
+        return self
+
+    def step(self, n_epochs: int):
+        check(self._L.bht_job_step(self._h, int(n_epochs), self._cb(), None))
+        return self
+
+    def finish(self):
+        check(self._L.bht_job_finish(self._h))
+        self._fitted = True
+        return self
+
+    def synchronize(self):
+        check(self._L.bht_job_synchronize(self._h))
+
+    def forces(self):
+        len_ = self._n * self._d
+        pos = np.zeros(len_, np.float32)
+        neg = np.zeros(len_, np.float32)
+        inv = C.c_float(0)
+        check(self._L.bht_job_forces(self._h, ptr(pos), ptr(neg), C.byref(inv)))
+        return pos.reshape(self._n, self._d), neg.reshape(self._n, self._d), inv.value
+
+    def get_state(self):
+        len_ = self._n * self._d
+        y, uy, g = (np.zeros(len_, np.float32) for _ in range(3))
+        check(self._L.bht_job_get_state(self._h, ptr(y), ptr(uy), ptr(g)))
+        return y.reshape(self._n, self._d), uy.reshape(self._n, self._d), g.reshape(self._n,
+                                                                                    self._d)
+
+    def set_state(self, y, uy, gains):
+        y, uy, gains = (np.ascontiguousarray(a, np.float32).ravel() for a in (y, uy, gains))
+        check(self._L.bht_job_set_state(self._h, ptr(y), ptr(uy), ptr(gains)))
+        return self
+
+    def set_timing(self, enable: bool):
+        check(self._L.bht_job_set_timing(self._h, 1 if enable else 0))
+        return self
+
+    def timing(self):
+        """(repulsion ms, attraction+update ms, epochs) measured with HIP events on the job's
+        stream while timing was enabled."""
+        out = (C.c_double * 3)()
+        check(self._L.bht_job_get_timing(self._h, out))
+        return float(out[0]), float(out[1]), int(out[2])
+
+    def grid_info(self):
+        b, L_ = C.c_uint32(), C.c_uint32()
+        lo, hi = C.c_float(), C.c_float()
+        check(self._L.bht_job_grid_info(self._h, C.byref(b), C.byref(L_), C.byref(lo),
+                                        C.byref(hi)))
+        return dict(boxes=b.value, fft_len=L_.value, min=lo.value, max=hi.value)
+
+
+# ---- operators (the reference's internal steps) -------------------------------------------------
+
+def knn(x, k: int, ctx: Optional[Context] = None):
+    """Exact kNN, self excluded, ascending (distance, index) (src/tsne/vptree.rs:274-319)."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    x = np.ascontiguousarray(x, np.float32)
+    n, d = x.shape
+    idx = np.zeros((n, k), np.uint32)
+    dist = np.zeros((n, k), np.float32)
+    check(L.bht_knn_f32(ctx.handle, ptr(x), n, d, k, ptr(idx), ptr(dist)))
+    return idx, dist
+
+
+def search_beta(dist, perplexity: float, mode: int = _lib.BANDWIDTH_FP64,
+                ctx: Optional[Context] = None):
+    """Per-row bandwidth search (src/tsne/mod.rs:201-275)."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    dist = np.ascontiguousarray(dist, np.float32)
+    n, k = dist.shape
+    p = np.zeros_like(dist)
+    check(L.bht_search_beta_f32(ctx.handle, ptr(dist), n, k, float(perplexity), int(mode),
+                                ptr(p)))
+    return p
+
+
+def symmetrize(cols, vals, ctx: Optional[Context] = None):
+    """Symmetrisation into CSR in the reference's row order (src/tsne/mod.rs:306-406)."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    cols = np.ascontiguousarray(cols, np.uint32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    n, k = cols.shape
+    nnz = C.c_uint64(0)
+    check(L.bht_symmetrize(ctx.handle, ptr(cols), ptr(vals), n, k, None, None, None,
+                           C.byref(nnz)))
+    rows = np.zeros(n + 1, np.uint64)
+    oc = np.zeros(nnz.value, np.uint32)
+    ov = np.zeros(nnz.value, np.float32)
+    check(L.bht_symmetrize(ctx.handle, ptr(cols), ptr(vals), n, k, ptr(rows), ptr(oc), ptr(ov),
+                           C.byref(nnz)))
+    return rows, oc, ov

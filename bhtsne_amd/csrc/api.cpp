@@ -1,0 +1,1014 @@
+// C ABI of the engine (include/bhtsne_hip.h): context, jobs and the fit driver.
+//
+// The driver mirrors the reference builder's control flow (src/lib.rs:652-1268): parameter
+// validation with the same messages, the affinity cache rules, P normalisation / exaggeration /
+// stop-lying, the momentum switch and the epoch loop; the numeric work runs in the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/bhtsne_hip.h"
+#include "affinity.hpp"
+#include "internal.hpp"
+
+using namespace bht;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+bht_status fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+bht_status guarded(F &&f) {
+    try {
+        f();
+        return BHT_OK;
+    } catch (const Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(BHT_ERR_OOM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(BHT_ERR_HIP, e.what());
+    }
+}
+
+int64_t next_smooth(int64_t n) {  // interpolation.rs:364-378
+    int64_t c = n < 1 ? 1 : n;
+    for (;;) {
+        int64_t v = c;
+        for (int p : {2, 3, 5, 7})
+            while (v % p == 0) v /= p;
+        if (v == 1) return c;
+        c++;
+    }
+}
+
+}  // namespace
+
+// ---- context ---------------------------------------------------------------------------------
+
+struct bht_ctx {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t in_use = 0;
+    TwiddleSet tw;
+    hipStream_t stream = nullptr;
+
+    void *alloc(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (budget && in_use + bytes > budget)
+            throw Error(ERR_OOM, "HBM budget exceeded: " + std::to_string(in_use + bytes) +
+                                     " bytes requested of " + std::to_string(budget));
+        void *p = nullptr;
+        BHT_HIP(hipSetDevice(device));
+        BHT_HIP(hipMalloc(&p, bytes));
+        in_use += bytes;
+        return p;
+    }
+    void release(void *p, size_t bytes) {
+        if (!p) return;
+        hipFree(p);
+        in_use -= bytes ? bytes : 16;
+    }
+};
+
+namespace {
+
+// Device buffer owned by a context.
+struct DevBuf {
+    bht_ctx *ctx = nullptr;
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(bht_ctx *c) : ctx(c) {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p && ctx) ctx->release(p, bytes);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(bht_ctx *c, size_t b) {
+        ctx = c;
+        if (p && bytes >= b) return;
+        reset();
+        p = ctx->alloc(b);
+        bytes = b ? b : 16;
+    }
+    template <typename T>
+    T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+void build_twiddles(bht_ctx *ctx) {
+    std::map<int, int> off_of_len;
+    std::vector<float2> table;
+    std::vector<int> len_of(MAX_BOXES + 1, 0), off_of(MAX_BOXES + 1, 0);
+    for (int b = 1; b <= MAX_BOXES; b++) {
+        int L = (int)next_smooth(6 * (int64_t)b - 1);
+        auto it = off_of_len.find(L);
+        if (it == off_of_len.end()) {
+            int off = (int)table.size();
+            off_of_len[L] = off;
+            for (int j = 0; j < L; j++) {
+                // exp(-2 pi i j / L) from an exactly reduced angle in double
+                long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)j /
+                                (long double)L;
+                table.push_back(make_float2((float)cosl(a), (float)sinl(a)));
+            }
+            it = off_of_len.find(L);
+        }
+        len_of[b] = L;
+        off_of[b] = it->second;
+    }
+    ctx->tw.total = table.size();
+    ctx->tw.table = (float2 *)ctx->alloc(table.size() * sizeof(float2));
+    ctx->tw.len_of_boxes = (int *)ctx->alloc(len_of.size() * sizeof(int));
+    ctx->tw.off_of_boxes = (int *)ctx->alloc(off_of.size() * sizeof(int));
+    BHT_HIP(hipMemcpy(ctx->tw.table, table.data(), table.size() * sizeof(float2),
+                      hipMemcpyHostToDevice));
+    BHT_HIP(hipMemcpy(ctx->tw.len_of_boxes, len_of.data(), len_of.size() * sizeof(int),
+                      hipMemcpyHostToDevice));
+    BHT_HIP(hipMemcpy(ctx->tw.off_of_boxes, off_of.data(), off_of.size() * sizeof(int),
+                      hipMemcpyHostToDevice));
+}
+
+const char *MSG_PERPLEXITY =
+    "error: the provided perplexity is too large for the number of data points.\n";
+const char *MSG_THETA =
+    "error: theta value must be greater than 0.0.\n"
+    "            A value of 0.0 corresponds to using the exact version of the algorithm.";
+
+}  // namespace
+
+// ---- job -------------------------------------------------------------------------------------
+
+enum FitKind { FIT_NONE = -1, FIT_BH = 0, FIT_FITSNE = 1 };
+
+struct bht_job {
+    bht_ctx *ctx;
+    uint64_t n;
+    bht_params params;
+    hipStream_t stream = nullptr;
+
+    // sample vectors (Euclidean)
+    DevBuf x;
+    uint32_t d_in = 0;
+    bool have_data = false;
+
+    // affinities: CSR with u32 offsets on the device
+    DevBuf rows, cols, vals;
+    uint64_t nnz = 0;
+    bool have_p = false;
+    bool have_cached_perplexity = false;
+    float cached_perplexity = 0.0f;
+    bool stop_lying_fired = false;
+
+    // optimiser state
+    DevBuf y[2], uy, gains, frep, pos;
+    int cur = 0;
+    bool have_init = false;
+    std::vector<float> init;
+    uint64_t seed_counter = 0;
+
+    // epoch workspace
+    DevBuf keys, svals, keys_alt, svals_alt, grid, spec, rker, kspec, part, dpart, fpart,
+        sort_temp, scal;
+    int nodes_cap = 0, len_cap = 0;
+    FitsneDev fd{};
+
+    // run state
+    int method = -1;
+    float theta = 0.5f;
+    bool prepared = false;
+    uint64_t epoch = 0;          // next epoch index of the prepared run
+    float run_momentum = 0.5f;   // momentum at the start of the prepared run
+    int fit_kind = FIT_NONE;
+    float fit_theta = 0.5f;
+
+    // per-phase timing with events on this job's stream (3 events per timed epoch)
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    size_t events_used = 0;
+
+    explicit bht_job(bht_ctx *c, uint64_t n_, const bht_params &p) : ctx(c), n(n_), params(p) {
+        BHT_HIP(hipSetDevice(ctx->device));
+        BHT_HIP(hipStreamCreate(&stream));
+    }
+    ~bht_job() {
+        for (auto e : events) hipEventDestroy(e);
+        if (stream) hipStreamDestroy(stream);
+    }
+    void mark() {
+        if (!timing) return;
+        if (events_used == events.size()) {
+            hipEvent_t e;
+            BHT_HIP(hipEventCreate(&e));
+            events.push_back(e);
+        }
+        BHT_HIP(hipEventRecord(events[events_used++], stream));
+    }
+    uint32_t dim() const { return params.d_out; }
+    float *ycur() const { return y[cur].as<float>(); }
+};
+
+namespace {
+
+void check_perplexity(const bht_job *j) {  // src/tsne/mod.rs:55-59
+    float perp = j->params.perplexity;
+    uint64_t p = perp > 0.0f ? (uint64_t)perp : 0;
+    if (j->n == 0 || j->n - 1 < 3 * p) throw Error(ERR_INVALID, MSG_PERPLEXITY);
+}
+
+uint32_t n_neighbors_of(const bht_job *j) {  // lib.rs:721, 852
+    float k = 3.0f * j->params.perplexity;
+    return k > 0.0f ? (uint32_t)k : 0u;
+}
+
+void ensure_state_buffers(bht_job *j) {
+    size_t b = (size_t)j->n * j->dim() * sizeof(float);
+    j->y[0].ensure(j->ctx, b);
+    j->y[1].ensure(j->ctx, b);
+    j->uy.ensure(j->ctx, b);
+    j->gains.ensure(j->ctx, b);
+    j->frep.ensure(j->ctx, b);
+    j->pos.ensure(j->ctx, b);
+    j->scal.ensure(j->ctx, sizeof(EpochScalars));
+    j->dpart.ensure(j->ctx, 4096 * sizeof(double));
+    j->fpart.ensure(j->ctx, 4096 * sizeof(float));
+}
+
+void ensure_fitsne_workspace(bht_job *j) {
+    ensure_state_buffers(j);
+    bht_ctx *c = j->ctx;
+    int maxb = (int)std::min<uint32_t>(std::max<uint32_t>(j->params.max_boxes, 1), MAX_BOXES);
+    int minb = (int)std::max<uint32_t>(j->params.min_boxes, 1);
+    if (minb > maxb) maxb = minb > MAX_BOXES ? MAX_BOXES : minb;
+    j->nodes_cap = 3 * maxb;
+    j->len_cap = (int)next_smooth(6 * (int64_t)maxb - 1);
+    const uint64_t n = j->n;
+    j->keys.ensure(c, n * 4);
+    j->svals.ensure(c, n * 4);
+    j->keys_alt.ensure(c, n * 4);
+    j->svals_alt.ensure(c, n * 4);
+    j->sort_temp.ensure(c, radix_sort_temp_bytes(n));
+    const size_t nodes_total = j->dim() == 2 ? (size_t)j->nodes_cap * j->nodes_cap : j->nodes_cap;
+    j->grid.ensure(c, nodes_total * sizeof(float4));
+    if (j->dim() == 2) j->spec.ensure(c, 2 * (size_t)j->nodes_cap * j->len_cap * sizeof(float2));
+    const size_t h = j->len_cap / 2 + 1;
+    j->rker.ensure(c, h * h * sizeof(float));
+    j->kspec.ensure(c, h * h * sizeof(float));
+    const size_t nchunks = (n + 63) / 64;
+    j->part.ensure(c, 2 * nchunks * 36 * sizeof(float));
+    FitsneDev &d = j->fd;
+    d.n = (int)n;
+    d.dim = (int)j->dim();
+    d.min_boxes = minb > MAX_BOXES ? MAX_BOXES : minb;
+    d.nodes_cap = j->nodes_cap;
+    d.len_cap = j->len_cap;
+    d.keys = j->keys.as<uint32_t>();
+    d.vals = j->svals.as<uint32_t>();
+    d.keys_alt = j->keys_alt.as<uint32_t>();
+    d.vals_alt = j->svals_alt.as<uint32_t>();
+    d.grid = j->grid.as<float4>();
+    d.spec = j->spec.as<float2>();
+    d.rker = j->rker.as<float>();
+    d.kspec = j->kspec.as<float>();
+    d.part = j->part.as<float>();
+    d.dpart = j->dpart.as<double>();
+    d.fpart = j->fpart.as<float>();
+    d.sort_temp = j->sort_temp.p;
+    d.scal = j->scal.as<EpochScalars>();
+    d.tw = &c->tw;
+}
+
+// Builds P (CSR) from per-row neighbour indices / distances already on the device.
+void affinities_from_knn(bht_job *j, uint32_t *d_idx, float *d_dist, uint32_t k) {
+    bht_ctx *c = j->ctx;
+    const int64_t n = (int64_t)j->n;
+    const uint64_t ne = (uint64_t)n * k;
+    if (ne >= (1ull << 32)) throw Error(ERR_INVALID, "n * k must be below 2^32");
+    DevBuf p(c);
+    p.ensure(c, ne * 4);
+    search_beta(d_dist, p.as<float>(), n, (int)k, j->params.perplexity,
+                (int)j->params.bandwidth_mode, j->stream);
+    DevBuf pair(c), nm(c), nmpos(c), inc(c), rowc(c), inoff(c), rws(c), dest(c), edge(c),
+        dest2(c), edge2(c), tot(c), stemp(c), rtemp(c);
+    pair.ensure(c, ne * 4);
+    nm.ensure(c, ne * 4);
+    nmpos.ensure(c, ne * 4);
+    inc.ensure(c, n * 4);
+    rowc.ensure(c, n * 4);
+    inoff.ensure(c, n * 4);
+    dest.ensure(c, ne * 4);
+    edge.ensure(c, ne * 4);
+    dest2.ensure(c, ne * 4);
+    edge2.ensure(c, ne * 4);
+    tot.ensure(c, 16);
+    stemp.ensure(c, scan_temp_bytes(ne));
+    rtemp.ensure(c, radix_sort_temp_bytes(ne));
+    j->rows.ensure(c, (n + 1) * 4);
+    SymWork wk;
+    wk.pair = pair.as<float>();
+    wk.nonmutual = nm.as<uint32_t>();
+    wk.nm_pos = nmpos.as<uint32_t>();
+    wk.in_count = inc.as<uint32_t>();
+    wk.row_count = rowc.as<uint32_t>();
+    wk.in_off = inoff.as<uint32_t>();
+    wk.rows = j->rows.as<uint32_t>();
+    wk.dest = dest.as<uint32_t>();
+    wk.edge = edge.as<uint32_t>();
+    wk.dest_alt = dest2.as<uint32_t>();
+    wk.edge_alt = edge2.as<uint32_t>();
+    wk.total = tot.as<uint32_t>();
+    wk.scan_temp = stemp.p;
+    wk.sort_temp = rtemp.p;
+    wk.alloc_out = [&](uint64_t nnz) {
+        j->cols.ensure(c, nnz * 4);
+        j->vals.ensure(c, nnz * 4);
+        wk.out_cols = j->cols.as<uint32_t>();
+        wk.out_vals = j->vals.as<float>();
+    };
+    uint64_t nnz = symmetrize_device(d_idx, p.as<float>(), n, (int)k, wk, j->stream);
+    BHT_HIP(hipStreamSynchronize(j->stream));
+    j->nnz = nnz;
+    j->have_p = true;
+    j->cached_perplexity = j->params.perplexity;  // lib.rs:1123-1124
+    j->have_cached_perplexity = true;
+}
+
+void build_affinities_from_data(bht_job *j) {
+    if (!j->have_data) throw Error(ERR_STATE, "no sample vectors: call bht_job_set_data_f32");
+    uint32_t k = n_neighbors_of(j);
+    bht_ctx *c = j->ctx;
+    DevBuf idx(c), dist(c);
+    idx.ensure(c, (size_t)j->n * k * 4);
+    dist.ensure(c, (size_t)j->n * k * 4);
+    knn_exact(j->x.as<float>(), (int64_t)j->n, (int)j->d_in, (int)k, idx.as<uint32_t>(),
+              dist.as<float>(), j->stream);
+    affinities_from_knn(j, idx.as<uint32_t>(), dist.as<float>(), k);
+}
+
+// finalize_p_and_seed / prepare_cached (lib.rs:663-685, 1227-1236) + buffer prep.
+void prepare_run(bht_job *j, int method, float theta, bool cached) {
+    if (cached) j->stop_lying_fired = false;
+    if (method == FIT_FITSNE) ensure_fitsne_workspace(j);
+    else ensure_state_buffers(j);
+    const uint64_t len = j->n * j->dim();
+    hipStream_t s = j->stream;
+    // normalise P with the exaggeration folded in (mod.rs:284-293)
+    EpochScalars *sc = j->scal.as<EpochScalars>();
+    BHT_HIP(hipMemsetAsync(sc, 0, sizeof(EpochScalars), s));
+    sum_values_f64(j->vals.as<float>(), j->nnz, sc, j->dpart.as<double>(), s);
+    double psum = 0.0;
+    BHT_HIP(hipMemcpyAsync(&psum, &sc->psum, sizeof(double), hipMemcpyDeviceToHost, s));
+    BHT_HIP(hipStreamSynchronize(s));
+    const float exag = j->params.early_exaggeration;
+    const float scale = exag / ((float)psum + FLT_EPSILON);
+    scale_values(j->vals.as<float>(), j->nnz, scale, s);
+    if (j->params.stop_lying_epoch == 0) scale_values(j->vals.as<float>(), j->nnz, 1.0f / exag, s);
+    // seed
+    std::vector<float> host(len);
+    if (j->have_init) {
+        host = j->init;
+        j->init.clear();
+        j->have_init = false;
+    } else {
+        // random_init (mod.rs:142-147) draws from an unseeded N(0, 1e-4); here the stream is
+        // seeded per job so that runs repeat.
+        std::mt19937_64 rng(0x6268747366ull + 7919ull * (j->seed_counter++));
+        std::normal_distribution<double> nd(0.0, 1e-4);
+        for (auto &v : host) v = (float)nd(rng);
+    }
+    j->cur = 0;
+    BHT_HIP(hipMemcpyAsync(j->ycur(), host.data(), len * 4, hipMemcpyHostToDevice, s));
+    BHT_HIP(hipMemsetAsync(j->uy.p, 0, len * 4, s));
+    std::vector<float> ones(len, 1.0f);
+    BHT_HIP(hipMemcpyAsync(j->gains.p, ones.data(), len * 4, hipMemcpyHostToDevice, s));
+    BHT_HIP(hipMemsetAsync(sc, 0, sizeof(EpochScalars), s));
+    BHT_HIP(hipStreamSynchronize(s));
+    j->method = method;
+    j->theta = theta;
+    j->epoch = 0;
+    j->run_momentum = j->params.momentum;
+    j->prepared = true;
+}
+
+float resolve_learning_rate(const bht_job *j) {  // lib.rs:652-659
+    if (j->params.learning_rate > 0.0f) return j->params.learning_rate;
+    float a = (float)j->n / j->params.early_exaggeration / 4.0f;
+    return std::max(a, 50.0f);
+}
+
+void repulsion(bht_job *j) {
+    FitsneDev &d = j->fd;
+    d.y = j->ycur();
+    d.frep = j->frep.as<float>();
+    if (j->method == FIT_FITSNE) {
+        fitsne_center_plan(d, j->stream);
+        fitsne_repulsion(d, j->stream);
+    } else {
+        throw Error(ERR_INVALID, "Barnes-Hut repulsion is not available in this build");
+    }
+}
+
+void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
+    if (!j->prepared) throw Error(ERR_STATE, "bht_job_prepare must precede bht_job_step");
+    const uint64_t len = j->n * j->dim();
+    std::vector<float> snap(cb ? len : 0);
+    UpdateArgs a{};
+    a.n = (int)j->n;
+    a.dim = (int)j->dim();
+    a.rows = j->rows.as<uint32_t>();
+    a.cols = j->cols.as<uint32_t>();
+    a.vals = j->vals.as<float>();
+    a.uy = j->uy.as<float>();
+    a.gains = j->gains.as<float>();
+    a.frep = j->frep.as<float>();
+    a.learning_rate = resolve_learning_rate(j);
+    a.momentum = j->run_momentum;
+    a.final_momentum = j->params.final_momentum;
+    a.inv_exag = 1.0f / j->params.early_exaggeration;
+    a.momentum_switch_epoch =
+        (int)std::min<uint64_t>(j->params.momentum_switch_epoch, (uint64_t)INT32_MAX - 2);
+    a.stop_lying_epoch =
+        (int)std::min<uint64_t>(j->params.stop_lying_epoch, (uint64_t)INT32_MAX - 2);
+    a.scal = j->scal.as<EpochScalars>();
+    a.dpart = j->dpart.as<double>();
+    for (uint64_t e = 0; e < count; e++) {
+        j->mark();
+        repulsion(j);
+        j->mark();
+        a.y = j->ycur();
+        a.y_next = j->y[1 - j->cur].as<float>();
+        attract_update(a, j->stream);
+        j->mark();
+        j->cur = 1 - j->cur;
+        const uint64_t epoch = j->epoch++;
+        if (epoch == j->params.stop_lying_epoch && epoch != 0) j->stop_lying_fired = true;
+        if (epoch == j->params.momentum_switch_epoch) j->params.momentum = j->params.final_momentum;
+        if (cb) {
+            center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
+                        j->dpart.as<double>(), j->stream);
+            BHT_HIP(hipMemcpyAsync(snap.data(), j->ycur(), len * 4, hipMemcpyDeviceToHost,
+                                   j->stream));
+            BHT_HIP(hipStreamSynchronize(j->stream));
+            cb(epoch, snap.data(), len, user);
+        }
+    }
+}
+
+void finish_run(bht_job *j) {
+    if (!j->prepared) throw Error(ERR_STATE, "no prepared run");
+    center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
+                j->dpart.as<double>(), j->stream);
+    // stop-lying of the last epoch has not been applied lazily by a following epoch
+    const uint64_t sl = j->params.stop_lying_epoch;
+    if (sl != 0 && j->epoch == sl + 1)
+        scale_values(j->vals.as<float>(), j->nnz, 1.0f / j->params.early_exaggeration, j->stream);
+    BHT_HIP(hipStreamSynchronize(j->stream));
+    j->fit_kind = j->method;
+    j->fit_theta = j->theta;
+    j->prepared = false;
+}
+
+bool has_cached_affinities(const bht_job *j) {  // lib.rs:1192-1196
+    return j->have_p && j->have_cached_perplexity &&
+           j->cached_perplexity == j->params.perplexity;
+}
+
+void validate_init(const bht_job *j) {
+    if (j->have_init && j->init.size() != j->n * j->dim())
+        throw Error(ERR_INVALID, "error: initial embedding has " +
+                                     std::to_string(j->init.size()) +
+                                     " values, expected n_samples * D = " +
+                                     std::to_string(j->n * j->dim()));
+}
+
+void fit(bht_job *j, int method, float theta, bht_epoch_cb cb, void *user, bool cached) {
+    validate_init(j);
+    prepare_run(j, method, theta, cached);
+    run_epochs(j, j->params.epochs, cb, user);
+    finish_run(j);
+}
+
+void check_dims(const bht_job *j, int method) {
+    if (method == FIT_BH && j->dim() != 2)
+        throw Error(ERR_INVALID, "the Barnes-Hut path supports d_out = 2");
+    if (method == FIT_FITSNE && j->dim() != 1 && j->dim() != 2)
+        throw Error(ERR_INVALID, "the FIt-SNE path supports d_out in {1, 2}");
+}
+
+// check_neighbors + cache match + range check (lib.rs:781-815, 922-951)
+void fit_with_neighbors(bht_job *j, int method, float theta, const uint64_t *idx,
+                        const float *dist, const uint64_t *row_lens, uint64_t n_rows,
+                        bht_epoch_cb cb, void *user) {
+    if (n_rows != j->n)
+        throw Error(ERR_INVALID, "error: neighbors has " + std::to_string(n_rows) +
+                                     " rows, expected one per sample = " + std::to_string(j->n));
+    uint64_t k = n_rows ? row_lens[0] : 0;
+    bool ok = k > 0;
+    for (uint64_t i = 0; ok && i < n_rows; i++) ok = row_lens[i] == k;
+    if (!ok)
+        throw Error(ERR_INVALID,
+                    "error: every neighbors row must have the same length, greater than zero.");
+    // cached_affinities_match_neighbors (lib.rs:1201-1220)
+    if (j->have_p && j->have_cached_perplexity &&
+        j->cached_perplexity == j->params.perplexity) {
+        std::vector<uint32_t> hrows(j->n + 1), hcols(j->nnz);
+        BHT_HIP(hipMemcpy(hrows.data(), j->rows.p, (j->n + 1) * 4, hipMemcpyDeviceToHost));
+        BHT_HIP(hipMemcpy(hcols.data(), j->cols.p, j->nnz * 4, hipMemcpyDeviceToHost));
+        bool match = true;
+        for (uint64_t i = 0; match && i < j->n; i++) {
+            if (hrows[i + 1] - hrows[i] != k) { match = false; break; }
+            for (uint64_t t = 0; t < k; t++)
+                if (hcols[hrows[i] + t] != (uint32_t)idx[i * k + t]) { match = false; break; }
+        }
+        if (match) {
+            fit(j, method, theta, cb, user, true);
+            return;
+        }
+    }
+    for (uint64_t e = 0; e < n_rows * k; e++)
+        if (idx[e] >= j->n)
+            throw Error(ERR_INVALID,
+                        "error: a neighbor index is out of range, every index must be < "
+                        "n_samples = " + std::to_string(j->n) + ".");
+    bht_ctx *c = j->ctx;
+    DevBuf didx(c), ddist(c);
+    didx.ensure(c, n_rows * k * 4);
+    ddist.ensure(c, n_rows * k * 4);
+    std::vector<uint32_t> hidx(n_rows * k);
+    for (uint64_t e = 0; e < n_rows * k; e++) hidx[e] = (uint32_t)idx[e];
+    BHT_HIP(hipMemcpy(didx.p, hidx.data(), hidx.size() * 4, hipMemcpyHostToDevice));
+    BHT_HIP(hipMemcpy(ddist.p, dist, n_rows * k * 4, hipMemcpyHostToDevice));
+    affinities_from_knn(j, didx.as<uint32_t>(), ddist.as<float>(), (uint32_t)k);
+    fit(j, method, theta, cb, user, false);
+}
+
+}  // namespace
+
+// ---- exported functions ----------------------------------------------------------------------
+
+extern "C" {
+
+void bht_params_default(bht_params *p) {  // lib.rs:241-266
+    std::memset(p, 0, sizeof(*p));
+    p->d_out = 2;
+    p->bandwidth_mode = BHT_BANDWIDTH_FP64;
+    p->epochs = 1000;
+    p->momentum_switch_epoch = 250;
+    p->stop_lying_epoch = 250;
+    p->learning_rate = 0.0f;
+    p->momentum = 0.5f;
+    p->final_momentum = 0.8f;
+    p->early_exaggeration = 12.0f;
+    p->perplexity = 20.0f;
+    p->min_boxes = 10;
+    p->max_boxes = 1000;
+}
+
+const char *bht_last_error(void) { return g_last_error.c_str(); }
+
+bht_status bht_ctx_create(int device, uint64_t hbm_budget_bytes, bht_ctx **out) {
+    if (!out) return fail(BHT_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    return guarded([&] {
+        int count = 0;
+        BHT_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count)
+            throw Error(ERR_INVALID, "device index out of range");
+        auto ctx = std::make_unique<bht_ctx>();
+        ctx->device = device;
+        ctx->budget = hbm_budget_bytes;
+        BHT_HIP(hipSetDevice(device));
+        BHT_HIP(hipStreamCreate(&ctx->stream));
+        build_twiddles(ctx.get());
+        *out = ctx.release();
+    });
+}
+
+void bht_ctx_destroy(bht_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    ctx->release(ctx->tw.table, ctx->tw.total * sizeof(float2));
+    ctx->release(ctx->tw.len_of_boxes, (MAX_BOXES + 1) * sizeof(int));
+    ctx->release(ctx->tw.off_of_boxes, (MAX_BOXES + 1) * sizeof(int));
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+uint64_t bht_ctx_bytes_in_use(const bht_ctx *ctx) { return ctx ? ctx->in_use : 0; }
+
+bht_status bht_job_create(bht_ctx *ctx, uint64_t n_samples, const bht_params *params,
+                          bht_job **out) {
+    if (!ctx || !out) return fail(BHT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        bht_params p;
+        if (params) p = *params;
+        else bht_params_default(&p);
+        if (n_samples >= (1ull << 31)) throw Error(ERR_INVALID, "n_samples must be below 2^31");
+        *out = new bht_job(ctx, n_samples, p);
+    });
+}
+
+void bht_job_destroy(bht_job *job) {
+    if (!job) return;
+    hipSetDevice(job->ctx->device);
+    delete job;
+}
+
+bht_status bht_job_set_params(bht_job *job, const bht_params *params) {
+    if (!job || !params) return fail(BHT_ERR_INVALID, "null argument");
+    job->params = *params;
+    return BHT_OK;
+}
+
+bht_status bht_job_get_params(const bht_job *job, bht_params *out) {
+    if (!job || !out) return fail(BHT_ERR_INVALID, "null argument");
+    *out = job->params;
+    return BHT_OK;
+}
+
+bht_status bht_job_set_data_f32(bht_job *job, const float *x, uint64_t n, uint32_t d_in) {
+    if (!job || !x) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (n != job->n) throw Error(ERR_INVALID, "data rows must equal n_samples");
+        if (d_in == 0) throw Error(ERR_INVALID, "d_in must be positive");
+        job->x.ensure(job->ctx, (size_t)n * d_in * 4);
+        BHT_HIP(hipMemcpy(job->x.p, x, (size_t)n * d_in * 4, hipMemcpyHostToDevice));
+        job->d_in = d_in;
+        job->have_data = true;
+    });
+}
+
+bht_status bht_job_set_initial_embedding(bht_job *job, const float *y, uint64_t len) {
+    if (!job || (!y && len)) return fail(BHT_ERR_INVALID, "null argument");
+    job->init.assign(y, y + len);
+    job->have_init = true;
+    return BHT_OK;
+}
+
+bht_status bht_job_fit_sne(bht_job *job, bht_epoch_cb cb, void *user) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        check_dims(job, FIT_FITSNE);
+        check_perplexity(job);
+        if (has_cached_affinities(job)) {
+            fit(job, FIT_FITSNE, 0.0f, cb, user, true);
+            return;
+        }
+        build_affinities_from_data(job);
+        fit(job, FIT_FITSNE, 0.0f, cb, user, false);
+    });
+}
+
+bht_status bht_job_fit_sne_with_neighbors(bht_job *job, const uint64_t *nbr_index,
+                                          const float *nbr_distance, const uint64_t *row_lens,
+                                          uint64_t n_rows, bht_epoch_cb cb, void *user) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        check_dims(job, FIT_FITSNE);
+        fit_with_neighbors(job, FIT_FITSNE, 0.0f, nbr_index, nbr_distance, row_lens, n_rows, cb,
+                           user);
+    });
+}
+
+bht_status bht_job_barnes_hut(bht_job *job, float theta, bht_epoch_cb cb, void *user) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (!(theta > 0.0f)) throw Error(ERR_INVALID, MSG_THETA);  // lib.rs:1182-1188
+        check_perplexity(job);
+        check_dims(job, FIT_BH);
+        if (has_cached_affinities(job)) {
+            fit(job, FIT_BH, theta, cb, user, true);
+            return;
+        }
+        build_affinities_from_data(job);
+        fit(job, FIT_BH, theta, cb, user, false);
+    });
+}
+
+bht_status bht_job_barnes_hut_with_neighbors(bht_job *job, float theta, const uint64_t *nbr_index,
+                                             const float *nbr_distance, const uint64_t *row_lens,
+                                             uint64_t n_rows, bht_epoch_cb cb, void *user) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (!(theta > 0.0f)) throw Error(ERR_INVALID, MSG_THETA);
+        check_perplexity(job);
+        check_dims(job, FIT_BH);
+        fit_with_neighbors(job, FIT_BH, theta, nbr_index, nbr_distance, row_lens, n_rows, cb,
+                           user);
+    });
+}
+
+bht_status bht_job_get_embedding(const bht_job *job, float *out, uint64_t len) {
+    if (!job || !out) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (len != job->n * job->dim()) throw Error(ERR_INVALID, "length must be n * d_out");
+        if (!job->y[job->cur].p) throw Error(ERR_STATE, "no embedding yet");
+        BHT_HIP(hipStreamSynchronize(job->stream));
+        BHT_HIP(hipMemcpy(out, job->ycur(), len * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+bht_status bht_job_get_affinities(const bht_job *job, uint64_t *rows, uint32_t *cols, float *vals,
+                                  uint64_t *nnz, float *perplexity) {
+    if (!job || !nnz) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (!job->have_p) throw Error(ERR_STATE, "no affinities");  // affinities() -> None
+        if (perplexity) *perplexity = job->params.perplexity;
+        if (!rows) {
+            *nnz = job->nnz;
+            return;
+        }
+        if (*nnz < job->nnz) throw Error(ERR_INVALID, "output arrays too small");
+        BHT_HIP(hipStreamSynchronize(job->stream));
+        std::vector<uint32_t> r(job->n + 1);
+        BHT_HIP(hipMemcpy(r.data(), job->rows.p, r.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < r.size(); i++) rows[i] = r[i];
+        if (cols) BHT_HIP(hipMemcpy(cols, job->cols.p, job->nnz * 4, hipMemcpyDeviceToHost));
+        if (vals) {
+            BHT_HIP(hipMemcpy(vals, job->vals.p, job->nnz * 4, hipMemcpyDeviceToHost));
+            if (!job->stop_lying_fired) {  // lib.rs:1023-1029
+                const float scale = 1.0f / job->params.early_exaggeration;
+                for (uint64_t i = 0; i < job->nnz; i++) vals[i] = vals[i] * scale;
+            }
+        }
+        *nnz = job->nnz;
+    });
+}
+
+bht_status bht_job_set_affinities(bht_job *job, const uint64_t *rows, const uint32_t *cols,
+                                  const float *vals, uint64_t n_rows_plus_one, uint64_t nnz,
+                                  float perplexity) {
+    if (!job || !rows || (nnz && (!cols || !vals))) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (nnz >= (1ull << 32)) throw Error(ERR_INVALID, "nnz must be below 2^32");
+        if (rows[n_rows_plus_one - 1] != nnz) throw Error(ERR_INVALID, "rows[n] must equal nnz");
+        std::vector<uint32_t> r(n_rows_plus_one);
+        for (uint64_t i = 0; i < n_rows_plus_one; i++) r[i] = (uint32_t)rows[i];
+        job->rows.ensure(job->ctx, n_rows_plus_one * 4);
+        job->cols.ensure(job->ctx, nnz * 4);
+        job->vals.ensure(job->ctx, nnz * 4);
+        BHT_HIP(hipMemcpy(job->rows.p, r.data(), r.size() * 4, hipMemcpyHostToDevice));
+        BHT_HIP(hipMemcpy(job->cols.p, cols, nnz * 4, hipMemcpyHostToDevice));
+        BHT_HIP(hipMemcpy(job->vals.p, vals, nnz * 4, hipMemcpyHostToDevice));
+        job->nnz = nnz;
+        // with_affinities (lib.rs:1043-1051): adopt the perplexity; the cache key is rows.len()
+        job->params.perplexity = perplexity;
+        job->cached_perplexity = perplexity;
+        job->have_cached_perplexity = true;
+        job->have_p = n_rows_plus_one == job->n + 1 && n_rows_plus_one > 1;
+    });
+}
+
+bht_status bht_job_prepare(bht_job *job, uint32_t method, float theta) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (method != BHT_METHOD_FITSNE && method != BHT_METHOD_BARNES_HUT)
+            throw Error(ERR_INVALID, "unknown method");
+        int m = method == BHT_METHOD_FITSNE ? FIT_FITSNE : FIT_BH;
+        if (m == FIT_BH && !(theta > 0.0f)) throw Error(ERR_INVALID, MSG_THETA);
+        check_dims(job, m);
+        if (!job->have_p) {
+            check_perplexity(job);
+            build_affinities_from_data(job);
+        }
+        validate_init(job);
+        prepare_run(job, m, theta, true);
+    });
+}
+
+bht_status bht_job_step(bht_job *job, uint64_t n_epochs, bht_epoch_cb cb, void *user) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] { run_epochs(job, n_epochs, cb, user); });
+}
+
+bht_status bht_job_finish(bht_job *job) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] { finish_run(job); });
+}
+
+bht_status bht_job_synchronize(bht_job *job) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] { BHT_HIP(hipStreamSynchronize(job->stream)); });
+}
+
+bht_status bht_job_forces(bht_job *job, float *attractive, float *repulsive, float *inverse_z) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (!job->prepared) throw Error(ERR_STATE, "bht_job_prepare must precede bht_job_forces");
+        repulsion(job);
+        attract_only((int)job->n, (int)job->dim(), job->rows.as<uint32_t>(),
+                     job->cols.as<uint32_t>(), job->vals.as<float>(), job->ycur(),
+                     job->pos.as<float>(), job->stream);
+        const uint64_t len = job->n * job->dim();
+        EpochScalars sc;
+        BHT_HIP(hipMemcpyAsync(&sc, job->scal.p, sizeof(sc), hipMemcpyDeviceToHost, job->stream));
+        if (attractive)
+            BHT_HIP(hipMemcpyAsync(attractive, job->pos.p, len * 4, hipMemcpyDeviceToHost,
+                                   job->stream));
+        if (repulsive)
+            BHT_HIP(hipMemcpyAsync(repulsive, job->frep.p, len * 4, hipMemcpyDeviceToHost,
+                                   job->stream));
+        BHT_HIP(hipStreamSynchronize(job->stream));
+        if (inverse_z) *inverse_z = sc.inv_z;
+    });
+}
+
+bht_status bht_job_get_state(const bht_job *job, float *y, float *uy, float *gains) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (!job->uy.p) throw Error(ERR_STATE, "no optimiser state");
+        const uint64_t len = job->n * job->dim();
+        center_only(job->ycur(), (int)job->n, (int)job->dim(), job->scal.as<EpochScalars>(),
+                    job->dpart.as<double>(), job->stream);
+        BHT_HIP(hipStreamSynchronize(job->stream));
+        if (y) BHT_HIP(hipMemcpy(y, job->ycur(), len * 4, hipMemcpyDeviceToHost));
+        if (uy) BHT_HIP(hipMemcpy(uy, job->uy.p, len * 4, hipMemcpyDeviceToHost));
+        if (gains) BHT_HIP(hipMemcpy(gains, job->gains.p, len * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, const float *gains) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (!job->prepared) throw Error(ERR_STATE, "bht_job_prepare must precede set_state");
+        const uint64_t len = job->n * job->dim();
+        BHT_HIP(hipStreamSynchronize(job->stream));
+        if (y) BHT_HIP(hipMemcpy(job->ycur(), y, len * 4, hipMemcpyHostToDevice));
+        if (uy) BHT_HIP(hipMemcpy(job->uy.p, uy, len * 4, hipMemcpyHostToDevice));
+        if (gains) BHT_HIP(hipMemcpy(job->gains.p, gains, len * 4, hipMemcpyHostToDevice));
+        EpochScalars *sc = job->scal.as<EpochScalars>();
+        BHT_HIP(hipMemset(sc->mean, 0, sizeof(sc->mean)));
+    });
+}
+
+bht_status bht_job_grid_info(const bht_job *job, uint32_t *boxes, uint32_t *fft_len, float *lo,
+                             float *hi) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (!job->scal.p) throw Error(ERR_STATE, "no run");
+        EpochScalars sc;
+        BHT_HIP(hipStreamSynchronize(job->stream));
+        BHT_HIP(hipMemcpy(&sc, job->scal.p, sizeof(sc), hipMemcpyDeviceToHost));
+        if (boxes) *boxes = (uint32_t)sc.plan.boxes;
+        if (fft_len) *fft_len = (uint32_t)sc.plan.len;
+        if (lo) *lo = sc.plan.lo;
+        if (hi) *hi = sc.plan.hi;
+    });
+}
+
+bht_status bht_job_set_timing(bht_job *job, int enable) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        BHT_HIP(hipStreamSynchronize(job->stream));
+        job->timing = enable != 0;
+        job->events_used = 0;
+    });
+}
+
+bht_status bht_job_get_timing(bht_job *job, double *out3) {
+    if (!job || !out3) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        BHT_HIP(hipStreamSynchronize(job->stream));
+        double rep = 0.0, upd = 0.0;
+        size_t epochs = job->events_used / 3;
+        for (size_t e = 0; e < epochs; e++) {
+            float a = 0.f, b = 0.f;
+            BHT_HIP(hipEventElapsedTime(&a, job->events[3 * e], job->events[3 * e + 1]));
+            BHT_HIP(hipEventElapsedTime(&b, job->events[3 * e + 1], job->events[3 * e + 2]));
+            rep += a;
+            upd += b;
+        }
+        out3[0] = rep;
+        out3[1] = upd;
+        out3[2] = (double)epochs;
+    });
+}
+
+bht_status bht_job_kl_divergence(bht_job *job, double *reference_kl, double *true_kl) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return fail(BHT_ERR_STATE, "kl_divergence is not available in this build");
+}
+
+bht_status bht_knn_f32(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,
+                       uint32_t *idx_out, float *dist_out) {
+    if (!ctx || !x || !idx_out || !dist_out) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (k == 0 || k >= n) throw Error(ERR_INVALID, "k must be in [1, n - 1]");
+        DevBuf dx(ctx), di(ctx), dd(ctx);
+        dx.ensure(ctx, (size_t)n * d_in * 4);
+        di.ensure(ctx, (size_t)n * k * 4);
+        dd.ensure(ctx, (size_t)n * k * 4);
+        BHT_HIP(hipMemcpy(dx.p, x, (size_t)n * d_in * 4, hipMemcpyHostToDevice));
+        knn_exact(dx.as<float>(), (int64_t)n, (int)d_in, (int)k, di.as<uint32_t>(),
+                  dd.as<float>(), ctx->stream);
+        BHT_HIP(hipStreamSynchronize(ctx->stream));
+        BHT_HIP(hipMemcpy(idx_out, di.p, (size_t)n * k * 4, hipMemcpyDeviceToHost));
+        BHT_HIP(hipMemcpy(dist_out, dd.p, (size_t)n * k * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+bht_status bht_search_beta_f32(bht_ctx *ctx, const float *dist, uint64_t n, uint32_t k,
+                               float perplexity, uint32_t mode, float *p_out) {
+    if (!ctx || !dist || !p_out) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        DevBuf dd(ctx), dp(ctx);
+        dd.ensure(ctx, (size_t)n * k * 4);
+        dp.ensure(ctx, (size_t)n * k * 4);
+        BHT_HIP(hipMemcpy(dd.p, dist, (size_t)n * k * 4, hipMemcpyHostToDevice));
+        search_beta(dd.as<float>(), dp.as<float>(), (int64_t)n, (int)k, perplexity, (int)mode,
+                    ctx->stream);
+        BHT_HIP(hipStreamSynchronize(ctx->stream));
+        BHT_HIP(hipMemcpy(p_out, dp.p, (size_t)n * k * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+bht_status bht_symmetrize(bht_ctx *ctx, const uint32_t *cols, const float *vals, uint64_t n,
+                          uint32_t k, uint64_t *rows_out, uint32_t *cols_out, float *vals_out,
+                          uint64_t *nnz) {
+    if (!ctx || !cols || !vals || !nnz) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        bht_params p;
+        bht_params_default(&p);
+        bht_job job(ctx, n, p);
+        for (uint64_t e = 0; e < n * k; e++)
+            if (cols[e] >= n) throw Error(ERR_INVALID, "column index out of range");
+        DevBuf dc(ctx), dv(ctx);
+        dc.ensure(ctx, n * k * 4);
+        dv.ensure(ctx, n * k * 4);
+        BHT_HIP(hipMemcpy(dc.p, cols, n * k * 4, hipMemcpyHostToDevice));
+        BHT_HIP(hipMemcpy(dv.p, vals, n * k * 4, hipMemcpyHostToDevice));
+        // reuse the job's affinity path without the bandwidth search: p is given
+        const int64_t nn = (int64_t)n;
+        const uint64_t ne = n * k;
+        DevBuf pair(ctx), nm(ctx), nmpos(ctx), inc(ctx), rowc(ctx), inoff(ctx), dest(ctx),
+            edge(ctx), dest2(ctx), edge2(ctx), tot(ctx), stemp(ctx), rtemp(ctx);
+        pair.ensure(ctx, ne * 4);
+        nm.ensure(ctx, ne * 4);
+        nmpos.ensure(ctx, ne * 4);
+        inc.ensure(ctx, n * 4);
+        rowc.ensure(ctx, n * 4);
+        inoff.ensure(ctx, n * 4);
+        dest.ensure(ctx, ne * 4);
+        edge.ensure(ctx, ne * 4);
+        dest2.ensure(ctx, ne * 4);
+        edge2.ensure(ctx, ne * 4);
+        tot.ensure(ctx, 16);
+        stemp.ensure(ctx, scan_temp_bytes(ne));
+        rtemp.ensure(ctx, radix_sort_temp_bytes(ne));
+        job.rows.ensure(ctx, (n + 1) * 4);
+        SymWork wk;
+        wk.pair = pair.as<float>();
+        wk.nonmutual = nm.as<uint32_t>();
+        wk.nm_pos = nmpos.as<uint32_t>();
+        wk.in_count = inc.as<uint32_t>();
+        wk.row_count = rowc.as<uint32_t>();
+        wk.in_off = inoff.as<uint32_t>();
+        wk.rows = job.rows.as<uint32_t>();
+        wk.dest = dest.as<uint32_t>();
+        wk.edge = edge.as<uint32_t>();
+        wk.dest_alt = dest2.as<uint32_t>();
+        wk.edge_alt = edge2.as<uint32_t>();
+        wk.total = tot.as<uint32_t>();
+        wk.scan_temp = stemp.p;
+        wk.sort_temp = rtemp.p;
+        wk.alloc_out = [&](uint64_t m) {
+            job.cols.ensure(ctx, m * 4);
+            job.vals.ensure(ctx, m * 4);
+            wk.out_cols = job.cols.as<uint32_t>();
+            wk.out_vals = job.vals.as<float>();
+        };
+        uint64_t m = symmetrize_device(dc.as<uint32_t>(), dv.as<float>(), nn, (int)k, wk,
+                                       job.stream);
+        BHT_HIP(hipStreamSynchronize(job.stream));
+        if (!rows_out) {
+            *nnz = m;
+            return;
+        }
+        if (*nnz < m) throw Error(ERR_INVALID, "output arrays too small");
+        std::vector<uint32_t> r(n + 1);
+        BHT_HIP(hipMemcpy(r.data(), job.rows.p, (n + 1) * 4, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i <= n; i++) rows_out[i] = r[i];
+        if (cols_out) BHT_HIP(hipMemcpy(cols_out, job.cols.p, m * 4, hipMemcpyDeviceToHost));
+        if (vals_out) BHT_HIP(hipMemcpy(vals_out, job.vals.p, m * 4, hipMemcpyDeviceToHost));
+        *nnz = m;
+    });
+}
+
+}  // extern "C"

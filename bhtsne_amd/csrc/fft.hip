@@ -1,0 +1,370 @@
+// Hand-written FP32 FFT convolution of the FIt-SNE charge grids (replaces src/tsne/fft.rs and the
+// convolution loop of src/tsne/interpolation.rs:198-240).
+//
+// Layout and algebra:
+//  * The D + 2 = 4 real charge grids (D = 2) are packed into two complex grids a = t0 + i t1 and
+//    b = t2 + i t3.  The kernel K is real and even in both axes, so its spectrum is real and
+//    conv(x + i y, K) = conv(x, K) + i conv(y, K): one complex transform carries two terms and
+//    needs no real/complex split.
+//  * Only rows/columns [0, nodes) of the len x len zero-padded grid are non-zero on input and only
+//    those are needed on output, so the row passes run on `nodes` rows and the column pass
+//    (forward FFT, multiply by the kernel spectrum, inverse FFT) is fused into one kernel per
+//    column.
+//  * The kernel spectrum is real-even: its quarter (len/2+1)^2 is computed from two real-even
+//    passes, two rows (then two columns) per complex transform, pre-scaled by 1/len^2 as the
+//    reference folds the inverse normalisation into it (interpolation.rs:221-225).
+//  * Each line is transformed in LDS by a Stockham auto-sort FFT with radices 4, 2, 3, 5, 7
+//    (every length here is 7-smooth, interpolation.rs:364-378); twiddles come from a
+//    per-length table computed in double precision on the host.
+#include "internal.hpp"
+
+namespace bht {
+
+constexpr int FFT_THREADS = 256;
+constexpr int FFT_MAX_LEN = 6144;  // >= 6 * MAX_BOXES
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 w) {
+    return make_float2(fmaf(a.x, w.x, -a.y * w.y), fmaf(a.x, w.y, a.y * w.x));
+}
+
+template <int R>
+struct RootsOfUnity;  // cos / sin of 2 pi j / R, j < R, rounded from double
+template <>
+struct RootsOfUnity<3> {
+    static constexpr float c[3] = {1.0f, -0.5f, -0.5f};
+    static constexpr float s[3] = {0.0f, 0.86602540378443865f, -0.86602540378443865f};
+};
+template <>
+struct RootsOfUnity<5> {
+    static constexpr float c[5] = {1.0f, 0.30901699437494742f, -0.80901699437494742f,
+                                   -0.80901699437494742f, 0.30901699437494742f};
+    static constexpr float s[5] = {0.0f, 0.95105651629515357f, 0.58778525229247313f,
+                                   -0.58778525229247313f, -0.95105651629515357f};
+};
+template <>
+struct RootsOfUnity<7> {
+    static constexpr float c[7] = {1.0f, 0.62348980185873353f, -0.22252093395631440f,
+                                   -0.90096886790241913f, -0.90096886790241913f,
+                                   -0.22252093395631440f, 0.62348980185873353f};
+    static constexpr float s[7] = {0.0f, 0.78183148246802981f, 0.97492791218182361f,
+                                   0.43388373911755812f, -0.43388373911755812f,
+                                   -0.97492791218182361f, -0.78183148246802981f};
+};
+
+// In-register DFT of length R; forward uses exp(-2 pi i jk / R), inverse exp(+...).
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2 *x) {
+    if constexpr (R == 2) {
+        float2 a = x[0], b = x[1];
+        x[0] = make_float2(a.x + b.x, a.y + b.y);
+        x[1] = make_float2(a.x - b.x, a.y - b.y);
+    } else if constexpr (R == 4) {
+        float2 t0 = make_float2(x[0].x + x[2].x, x[0].y + x[2].y);
+        float2 t1 = make_float2(x[0].x - x[2].x, x[0].y - x[2].y);
+        float2 t2 = make_float2(x[1].x + x[3].x, x[1].y + x[3].y);
+        float2 t3 = make_float2(x[1].x - x[3].x, x[1].y - x[3].y);
+        x[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+        x[2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+        if (!INV) {
+            x[1] = make_float2(t1.x + t3.y, t1.y - t3.x);
+            x[3] = make_float2(t1.x - t3.y, t1.y + t3.x);
+        } else {
+            x[1] = make_float2(t1.x - t3.y, t1.y + t3.x);
+            x[3] = make_float2(t1.x + t3.y, t1.y - t3.x);
+        }
+    } else {
+        float2 in[R];
+#pragma unroll
+        for (int m = 0; m < R; m++) in[m] = x[m];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            float re = in[0].x, im = in[0].y;
+#pragma unroll
+            for (int m = 1; m < R; m++) {
+                const int j = (m * q) % R;
+                const float c = RootsOfUnity<R>::c[j];
+                const float s = INV ? RootsOfUnity<R>::s[j] : -RootsOfUnity<R>::s[j];
+                re = fmaf(in[m].x, c, fmaf(-in[m].y, s, re));
+                im = fmaf(in[m].x, s, fmaf(in[m].y, c, im));
+            }
+            x[q] = make_float2(re, im);
+        }
+    }
+}
+
+// One Stockham stage of radix R over a length-L line in LDS (all inputs are read into registers
+// before any output is written, so one LDS buffer suffices).
+template <int R, bool INV>
+__device__ __forceinline__ void stockham_stage(float2 *buf, int L, int Ns, const float2 *tw) {
+    constexpr int NB = (FFT_MAX_LEN / R + FFT_THREADS - 1) / FFT_THREADS;
+    const int M = L / R;
+    const int tstep = L / (Ns * R);
+    float2 v[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = threadIdx.x + b * FFT_THREADS;
+        if (j < M) {
+            const int k0 = j % Ns;
+            float2 x[R];
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                float2 a = buf[j + k * M];
+                if (k > 0 && k0 > 0) {
+                    float2 w = tw[k0 * k * tstep];
+                    if (INV) w.y = -w.y;
+                    a = cmul(a, w);
+                }
+                x[k] = a;
+            }
+            dft<R, INV>(x);
+#pragma unroll
+            for (int q = 0; q < R; q++) v[b][q] = x[q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = threadIdx.x + b * FFT_THREADS;
+        if (j < M) {
+            const int k0 = j % Ns;
+            const int base = (j - k0) * R + k0;
+#pragma unroll
+            for (int q = 0; q < R; q++) buf[base + q * Ns] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool INV>
+__device__ void fft_lds(float2 *buf, int L, const float2 *tw) {
+    int Ns = 1, rem = L;
+    while (rem % 4 == 0) { stockham_stage<4, INV>(buf, L, Ns, tw); Ns *= 4; rem /= 4; }
+    while (rem % 2 == 0) { stockham_stage<2, INV>(buf, L, Ns, tw); Ns *= 2; rem /= 2; }
+    while (rem % 3 == 0) { stockham_stage<3, INV>(buf, L, Ns, tw); Ns *= 3; rem /= 3; }
+    while (rem % 5 == 0) { stockham_stage<5, INV>(buf, L, Ns, tw); Ns *= 5; rem /= 5; }
+    while (rem % 7 == 0) { stockham_stage<7, INV>(buf, L, Ns, tw); Ns *= 7; rem /= 7; }
+}
+
+__device__ __forceinline__ int signed_lag(int c, int L) { return c <= L / 2 ? c : c - L; }
+
+// ---- kernel spectrum -------------------------------------------------------------------------
+
+// Rows r1 = 2q, r2 = 2q + 1 of the squared Cauchy kernel (interpolation.rs:198-217), transformed
+// together; both transforms are real (even rows), so real and imaginary parts separate directly.
+__global__ void __launch_bounds__(FFT_THREADS) k_kernel_rows(const EpochScalars *scal,
+                                                             const float2 *tw_all, float *rker,
+                                                             int dim) {
+    extern __shared__ float2 buf[];
+    const GridPlan p = scal->plan;
+    const int L = p.len, h = L / 2 + 1;
+    const float2 *tw = tw_all + p.tw_off;
+    const int npairs = dim == 2 ? (h + 1) / 2 : 1;
+    for (int q = blockIdx.x; q < npairs; q += gridDim.x) {
+        const int r1 = 2 * q, r2 = 2 * q + 1;
+        for (int c = threadIdx.x; c < L; c += FFT_THREADS) {
+            float dc = p.spacing * (float)signed_lag(c, L);
+            float v[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                float sq = 0.0f;
+                sq = sq + dc * dc;
+                if (dim == 2) {
+                    float dr = p.spacing * (float)signed_lag(t == 0 ? r1 : r2, L);
+                    sq = sq + dr * dr;
+                }
+                float cauchy = 1.0f / (1.0f + sq);
+                v[t] = cauchy * cauchy;
+            }
+            buf[c] = make_float2(v[0], dim == 2 ? v[1] : 0.0f);
+        }
+        __syncthreads();
+        fft_lds<false>(buf, L, tw);
+        if (dim == 2) {
+            for (int c = threadIdx.x; c < h; c += FFT_THREADS) {
+                rker[(size_t)r1 * h + c] = buf[c].x;
+                if (r2 < h) rker[(size_t)r2 * h + c] = buf[c].y;
+            }
+        } else {
+            const float inv_total = 1.0f / (float)L;
+            for (int c = threadIdx.x; c < h; c += FFT_THREADS) rker[c] = buf[c].x * inv_total;
+        }
+        __syncthreads();
+    }
+}
+
+// Columns v1 = 2q, v2 = 2q + 1 of the row transforms, extended evenly over r, transformed
+// together; the quarter spectrum is stored pre-scaled by 1 / len^2.
+__global__ void __launch_bounds__(FFT_THREADS) k_kernel_cols(const EpochScalars *scal,
+                                                             const float2 *tw_all,
+                                                             const float *rker, float *kspec) {
+    extern __shared__ float2 buf[];
+    const GridPlan p = scal->plan;
+    const int L = p.len, h = L / 2 + 1;
+    const float2 *tw = tw_all + p.tw_off;
+    const float inv_total = 1.0f / (float)(L * L);
+    const int npairs = (h + 1) / 2;
+    for (int q = blockIdx.x; q < npairs; q += gridDim.x) {
+        const int v1 = 2 * q, v2 = 2 * q + 1;
+        for (int r = threadIdx.x; r < L; r += FFT_THREADS) {
+            int rr = r <= L / 2 ? r : L - r;
+            float a = rker[(size_t)rr * h + v1];
+            float b = v2 < h ? rker[(size_t)rr * h + v2] : 0.0f;
+            buf[r] = make_float2(a, b);
+        }
+        __syncthreads();
+        fft_lds<false>(buf, L, tw);
+        for (int u = threadIdx.x; u < h; u += FFT_THREADS) {
+            kspec[(size_t)u * h + v1] = buf[u].x * inv_total;
+            if (v2 < h) kspec[(size_t)u * h + v2] = buf[u].y * inv_total;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- data passes -----------------------------------------------------------------------------
+
+// Forward row transforms of the packed grids: line (g, r), r < nodes.
+__global__ void __launch_bounds__(FFT_THREADS) k_rows_fwd(const EpochScalars *scal,
+                                                          const float2 *tw_all,
+                                                          const float4 *grid, float2 *spec,
+                                                          size_t gstride) {
+    extern __shared__ float2 buf[];
+    const GridPlan p = scal->plan;
+    const int L = p.len, nodes = p.nodes;
+    const float2 *tw = tw_all + p.tw_off;
+    const float2 *g2 = reinterpret_cast<const float2 *>(grid);
+    for (int line = blockIdx.x; line < 2 * nodes; line += gridDim.x) {
+        const int g = line / nodes, r = line % nodes;
+        for (int c = threadIdx.x; c < L; c += FFT_THREADS)
+            buf[c] = c < nodes ? g2[2 * ((size_t)r * nodes + c) + g] : make_float2(0.f, 0.f);
+        __syncthreads();
+        fft_lds<false>(buf, L, tw);
+        float2 *out = spec + g * gstride + (size_t)r * L;
+        for (int c = threadIdx.x; c < L; c += FFT_THREADS) out[c] = buf[c];
+        __syncthreads();
+    }
+}
+
+// Column pass: forward transform, multiply by the real kernel spectrum, inverse transform.
+__global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const EpochScalars *scal,
+                                                           const float2 *tw_all,
+                                                           const float *kspec, float2 *spec,
+                                                           size_t gstride) {
+    extern __shared__ float2 buf[];
+    const GridPlan p = scal->plan;
+    const int L = p.len, nodes = p.nodes, h = L / 2 + 1;
+    const float2 *tw = tw_all + p.tw_off;
+    for (int line = blockIdx.x; line < 2 * L; line += gridDim.x) {
+        const int g = line / L, c = line % L;
+        float2 *col = spec + g * gstride + c;
+        for (int r = threadIdx.x; r < L; r += FFT_THREADS)
+            buf[r] = r < nodes ? col[(size_t)r * L] : make_float2(0.f, 0.f);
+        __syncthreads();
+        fft_lds<false>(buf, L, tw);
+        const int cc = c <= L / 2 ? c : L - c;
+        for (int u = threadIdx.x; u < L; u += FFT_THREADS) {
+            const int uu = u <= L / 2 ? u : L - u;
+            const float k = kspec[(size_t)uu * h + cc];
+            float2 z = buf[u];
+            buf[u] = make_float2(z.x * k, z.y * k);
+        }
+        __syncthreads();
+        fft_lds<true>(buf, L, tw);
+        for (int r = threadIdx.x; r < nodes; r += FFT_THREADS) col[(size_t)r * L] = buf[r];
+        __syncthreads();
+    }
+}
+
+// Inverse row transforms; the potentials of terms (t0, t1) / (t2, t3) land in the two halves of
+// the float4 node grid.
+__global__ void __launch_bounds__(FFT_THREADS) k_rows_inv(const EpochScalars *scal,
+                                                          const float2 *tw_all,
+                                                          const float2 *spec, float4 *grid,
+                                                          size_t gstride) {
+    extern __shared__ float2 buf[];
+    const GridPlan p = scal->plan;
+    const int L = p.len, nodes = p.nodes;
+    const float2 *tw = tw_all + p.tw_off;
+    float2 *g2 = reinterpret_cast<float2 *>(grid);
+    for (int line = blockIdx.x; line < 2 * nodes; line += gridDim.x) {
+        const int g = line / nodes, r = line % nodes;
+        const float2 *in = spec + g * gstride + (size_t)r * L;
+        for (int c = threadIdx.x; c < L; c += FFT_THREADS) buf[c] = in[c];
+        __syncthreads();
+        fft_lds<true>(buf, L, tw);
+        for (int c = threadIdx.x; c < nodes; c += FFT_THREADS)
+            g2[2 * ((size_t)r * nodes + c) + g] = buf[c];
+        __syncthreads();
+    }
+}
+
+// D = 1: one line, both packed signals (t0 + i t1, t2 + i 0) transformed, multiplied, inverted.
+__global__ void __launch_bounds__(FFT_THREADS) k_line_conv_1d(const EpochScalars *scal,
+                                                              const float2 *tw_all,
+                                                              const float *kspec, float4 *grid) {
+    extern __shared__ float2 buf[];
+    const GridPlan p = scal->plan;
+    const int L = p.len, nodes = p.nodes;
+    const float2 *tw = tw_all + p.tw_off;
+    const int g = blockIdx.x;  // 0: (t0, t1), 1: (t2, 0)
+    if (g > 1) return;
+    for (int c = threadIdx.x; c < L; c += FFT_THREADS) {
+        float4 v = c < nodes ? grid[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+        buf[c] = g == 0 ? make_float2(v.x, v.y) : make_float2(v.z, 0.0f);
+    }
+    __syncthreads();
+    fft_lds<false>(buf, L, tw);
+    for (int u = threadIdx.x; u < L; u += FFT_THREADS) {
+        const int uu = u <= L / 2 ? u : L - u;
+        const float k = kspec[uu];
+        float2 z = buf[u];
+        buf[u] = make_float2(z.x * k, z.y * k);
+    }
+    __syncthreads();
+    fft_lds<true>(buf, L, tw);
+    // Block 0 reads (x, y) and block 1 reads z; block 1 writes its potential into the unused .w
+    // lane so that neither block overwrites what the other still reads.  The gather reads the
+    // third potential from .w.
+    for (int c = threadIdx.x; c < nodes; c += FFT_THREADS) {
+        if (g == 0) {
+            grid[c].x = buf[c].x;
+            grid[c].y = buf[c].y;
+        } else {
+            grid[c].w = buf[c].x;
+        }
+    }
+}
+
+void fft_kernel_spectrum(const FitsneDev &d, hipStream_t s) {
+    const size_t lds = (size_t)d.len_cap * sizeof(float2);
+    const int hcap = d.len_cap / 2 + 1;
+    const int rows_blocks = std::max(1, std::min(1024, (hcap + 1) / 2));
+    hipLaunchKernelGGL(k_kernel_rows, dim3(rows_blocks), dim3(FFT_THREADS), lds, s, d.scal,
+                       d.tw->table, d.dim == 2 ? d.rker : d.kspec, d.dim);
+    if (d.dim == 2)
+        hipLaunchKernelGGL(k_kernel_cols, dim3(rows_blocks), dim3(FFT_THREADS), lds, s, d.scal,
+                           d.tw->table, d.rker, d.kspec);
+    BHT_LAUNCH_CHECK();
+}
+
+void fft_convolve(const FitsneDev &d, hipStream_t s) {
+    const size_t lds = (size_t)d.len_cap * sizeof(float2);
+    if (d.dim == 1) {
+        hipLaunchKernelGGL(k_line_conv_1d, dim3(2), dim3(FFT_THREADS), lds, s, d.scal,
+                           d.tw->table, d.kspec, d.grid);
+        BHT_LAUNCH_CHECK();
+        return;
+    }
+    const size_t gstride = (size_t)d.nodes_cap * d.len_cap;
+    const int row_blocks = std::min(2048, 2 * d.nodes_cap);
+    const int col_blocks = std::min(2048, 2 * d.len_cap);
+    hipLaunchKernelGGL(k_rows_fwd, dim3(row_blocks), dim3(FFT_THREADS), lds, s, d.scal,
+                       d.tw->table, d.grid, d.spec, gstride);
+    hipLaunchKernelGGL(k_cols_conv, dim3(col_blocks), dim3(FFT_THREADS), lds, s, d.scal,
+                       d.tw->table, d.kspec, d.spec, gstride);
+    hipLaunchKernelGGL(k_rows_inv, dim3(row_blocks), dim3(FFT_THREADS), lds, s, d.scal,
+                       d.tw->table, d.spec, d.grid, gstride);
+    BHT_LAUNCH_CHECK();
+}
+
+}  // namespace bht

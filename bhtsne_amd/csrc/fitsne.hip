@@ -1,0 +1,479 @@
+// FIt-SNE repulsion on the device (src/tsne/interpolation.rs:100-332): recentring + joint bounds +
+// grid plan, box keys + stable sort, box-segmented charge spreading, FFT convolution (fft.hip),
+// potential gather + force / Z reconstruction.
+//
+// Per-point arithmetic (box location, Lagrange weights, charges, reconstruction) repeats the
+// reference's f32 operation order; only the summation order of the spread and of Z differs (Z and
+// the recentring mean are summed in FP64).
+#include <cfloat>
+
+#include "internal.hpp"
+
+namespace bht {
+
+constexpr int PT_THREADS = 256;
+
+int point_blocks(int n) {
+    int b = (int)ceil_div((uint64_t)(n > 0 ? n : 1), PT_THREADS);
+    return b < 1024 ? b : 1024;
+}
+
+// ---- per-point helpers (interpolation.rs:162-187, 400-456) ---------------------------------
+
+struct Lagrange3 {
+    float node[3];
+    float inv_denom[3];
+    __device__ __forceinline__ Lagrange3() {
+        const float half = 0.5f;
+        const float p_recip = 1.0f / 3.0f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) node[k] = ((float)k + half) * p_recip;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float denom = 1.0f;
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+                if (m != k) denom = denom * (node[k] - node[m]);
+            inv_denom[k] = 1.0f / denom;
+        }
+    }
+    __device__ __forceinline__ void weights(float local, float *out) const {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float num = 1.0f;
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+                if (m != k) num = num * (local - node[m]);
+            out[k] = num * inv_denom[k];
+        }
+    }
+};
+
+__device__ __forceinline__ int locate_box(float v, const GridPlan &p, float &local) {
+    float offset = (v - p.lo) * p.inv_box_width;
+    float b = floorf(offset);
+    int last = p.boxes - 1;
+    int bid = 0;
+    if (b > 0.0f) bid = b >= (float)last ? last : (int)b;
+    local = offset - (float)bid;
+    return bid;
+}
+
+// ---- 1. recentring + bounds + plan -----------------------------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(PT_THREADS) k_center_plan(float *y, int n, EpochScalars *scal,
+                                                            float *fpart, int min_boxes,
+                                                            int max_boxes,
+                                                            const int *len_of_boxes,
+                                                            const int *off_of_boxes,
+                                                            int do_plan) {
+    __shared__ float red[4][PT_THREADS / 64];
+    const float m0 = scal->mean[0], m1 = D == 2 ? scal->mean[1] : 0.0f;
+    float mn0 = INFINITY, mn1 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY;
+    const int stride = gridDim.x * PT_THREADS;
+    for (int i = blockIdx.x * PT_THREADS + threadIdx.x; i < n; i += stride) {
+        if (D == 2) {
+            float2 v = reinterpret_cast<float2 *>(y)[i];
+            v.x = v.x - m0;
+            v.y = v.y - m1;
+            reinterpret_cast<float2 *>(y)[i] = v;
+            mn0 = fminf(mn0, v.x); mx0 = fmaxf(mx0, v.x);
+            mn1 = fminf(mn1, v.y); mx1 = fmaxf(mx1, v.y);
+        } else {
+            float v = y[i] - m0;
+            y[i] = v;
+            mn0 = fminf(mn0, v); mx0 = fmaxf(mx0, v);
+        }
+    }
+    mn0 = wave_min_f32(mn0); mx0 = wave_max_f32(mx0);
+    mn1 = wave_min_f32(mn1); mx1 = wave_max_f32(mx1);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[0][w] = mn0; red[1][w] = mx0; red[2][w] = mn1; red[3][w] = mx1; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < PT_THREADS / 64; q++) {
+            mn0 = fminf(mn0, red[0][q]); mx0 = fmaxf(mx0, red[1][q]);
+            mn1 = fminf(mn1, red[2][q]); mx1 = fmaxf(mx1, red[3][q]);
+        }
+        float4 pv = make_float4(mn0, mx0, mn1, mx1);
+        reinterpret_cast<float4 *>(fpart)[blockIdx.x] = pv;
+    }
+    if (!arrive_last(&scal->ctr[CTR_CENTER], gridDim.x)) return;
+    // last block: combine the per-block bounds
+    mn0 = INFINITY; mx0 = -INFINITY; mn1 = INFINITY; mx1 = -INFINITY;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += PT_THREADS) {
+        float4 pv = reinterpret_cast<const float4 *>(fpart)[b];
+        mn0 = fminf(mn0, pv.x); mx0 = fmaxf(mx0, pv.y);
+        mn1 = fminf(mn1, pv.z); mx1 = fmaxf(mx1, pv.w);
+    }
+    mn0 = wave_min_f32(mn0); mx0 = wave_max_f32(mx0);
+    mn1 = wave_min_f32(mn1); mx1 = wave_max_f32(mx1);
+    __syncthreads();
+    if (lane == 0) { red[0][w] = mn0; red[1][w] = mx0; red[2][w] = mn1; red[3][w] = mx1; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int q = 1; q < PT_THREADS / 64; q++) {
+        mn0 = fminf(mn0, red[0][q]); mx0 = fmaxf(mx0, red[1][q]);
+        mn1 = fminf(mn1, red[2][q]); mx1 = fmaxf(mx1, red[3][q]);
+    }
+    scal->amin[0] = mn0; scal->amax[0] = mx0;
+    scal->amin[1] = mn1; scal->amax[1] = mx1;
+    scal->mean[0] = 0.0f;
+    scal->mean[1] = 0.0f;
+    scal->ctr[CTR_CENTER] = 0;
+    if (!do_plan) return;
+    // coordinate_bounds + box_count + next_smooth (interpolation.rs:115-128, 337-378)
+    float lo = D == 2 ? fminf(mn0, mn1) : mn0;
+    float hi = D == 2 ? fmaxf(mx0, mx1) : mx0;
+    float extent = fmaxf(hi - lo, FLT_MIN);
+    float scaled = ceilf(extent / 1.0f);
+    int boxes;
+    if (scaled > 0.0f) boxes = scaled >= (float)MAX_BOXES ? MAX_BOXES : (int)scaled;
+    else boxes = 0;
+    if (boxes < min_boxes) boxes = min_boxes;
+    if (boxes > MAX_BOXES) boxes = MAX_BOXES;
+    // the job's grid capacity (MAX_BOXES unless the caller lowered it)
+    if (boxes > max_boxes) boxes = max_boxes;
+    GridPlan p;
+    p.lo = lo;
+    p.hi = hi;
+    p.boxes = boxes;
+    p.nodes = boxes * IP;
+    p.len = len_of_boxes[boxes];
+    p.tw_off = off_of_boxes[boxes];
+    p.box_width = extent / (float)boxes;
+    p.spacing = p.box_width / (float)IP;
+    p.inv_box_width = 1.0f / p.box_width;
+    p.pad = 0;
+    scal->plan = p;
+}
+
+void fitsne_center_plan(const FitsneDev &d, hipStream_t s) {
+    int nb = point_blocks(d.n);
+    const int maxb = d.nodes_cap / IP;
+    if (d.dim == 2)
+        hipLaunchKernelGGL(k_center_plan<2>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
+                           d.fpart, d.min_boxes, maxb, d.tw->len_of_boxes, d.tw->off_of_boxes,
+                           1);
+    else
+        hipLaunchKernelGGL(k_center_plan<1>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
+                           d.fpart, d.min_boxes, maxb, d.tw->len_of_boxes, d.tw->off_of_boxes,
+                           1);
+    BHT_LAUNCH_CHECK();
+}
+
+void center_only(float *y, int n, int dim, EpochScalars *scal, double *dpart, hipStream_t s) {
+    int nb = point_blocks(n);
+    float *fpart = reinterpret_cast<float *>(dpart);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_center_plan<2>, dim3(nb), dim3(PT_THREADS), 0, s, y, n, scal, fpart,
+                           10, MAX_BOXES, nullptr, nullptr, 0);
+    else
+        hipLaunchKernelGGL(k_center_plan<1>, dim3(nb), dim3(PT_THREADS), 0, s, y, n, scal, fpart,
+                           10, MAX_BOXES, nullptr, nullptr, 0);
+    BHT_LAUNCH_CHECK();
+}
+
+// ---- 2. box keys ---------------------------------------------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(PT_THREADS) k_box_keys(const float *y, int n,
+                                                         const EpochScalars *scal,
+                                                         uint32_t *keys, uint32_t *vals) {
+    const GridPlan p = scal->plan;
+    const int stride = gridDim.x * PT_THREADS;
+    for (int i = blockIdx.x * PT_THREADS + threadIdx.x; i < n; i += stride) {
+        float l;
+        uint32_t key;
+        if (D == 2) {
+            float2 v = reinterpret_cast<const float2 *>(y)[i];
+            uint32_t b0 = (uint32_t)locate_box(v.x, p, l);
+            uint32_t b1 = (uint32_t)locate_box(v.y, p, l);
+            key = b0 * (uint32_t)p.boxes + b1;
+        } else {
+            key = (uint32_t)locate_box(y[i], p, l);
+        }
+        keys[i] = key;
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_zero_grid(float4 *grid, const EpochScalars *scal, int dim) {
+    const GridPlan p = scal->plan;
+    int total = dim == 2 ? p.nodes * p.nodes : p.nodes;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+        grid[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---- 3. box-segmented spreading (interpolation.rs:247-269) ----------------------------------
+//
+// Points sorted by box form runs; each box's 3^D nodes receive only that run's charges.  One wave
+// reduces 64 sorted points with a segmented scan; a run wholly inside a chunk is written directly,
+// a run crossing chunk boundaries leaves partials that k_spread_fixup adds in chunk order.
+
+constexpr int NV2 = 36;  // 9 nodes x 4 terms (D = 2)
+constexpr int NV1 = 9;   // 3 nodes x 3 terms (D = 1)
+
+template <int D>
+__device__ __forceinline__ void point_contrib(const float *y, uint32_t i, const GridPlan &p,
+                                              const Lagrange3 &lb, float *v) {
+    if (D == 2) {
+        float2 pt = reinterpret_cast<const float2 *>(y)[i];
+        float l0, l1, w0[3], w1[3];
+        locate_box(pt.x, p, l0);
+        locate_box(pt.y, p, l1);
+        lb.weights(l0, w0);
+        lb.weights(l1, w1);
+        float c[4];
+        c[0] = 1.0f;
+        c[1] = pt.x;
+        c[2] = pt.y;
+        c[3] = pt.x * pt.x + pt.y * pt.y;
+#pragma unroll
+        for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+            for (int k0 = 0; k0 < 3; k0++) {
+                float wt = w0[k0] * w1[k1];
+#pragma unroll
+                for (int t = 0; t < 4; t++) v[(k0 + 3 * k1) * 4 + t] = wt * c[t];
+            }
+    } else {
+        float pt = y[i];
+        float l0, w0[3];
+        locate_box(pt, p, l0);
+        lb.weights(l0, w0);
+        float c[3] = {1.0f, pt, pt * pt};
+#pragma unroll
+        for (int k0 = 0; k0 < 3; k0++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) v[k0 * 3 + t] = w0[k0] * c[t];
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void write_box(float4 *grid, const GridPlan &p, uint32_t key,
+                                          const float *v) {
+    if (D == 2) {
+        uint32_t b0 = key / (uint32_t)p.boxes, b1 = key % (uint32_t)p.boxes;
+#pragma unroll
+        for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+            for (int k0 = 0; k0 < 3; k0++) {
+                int c = k0 + 3 * k1;
+                size_t node = (size_t)(3 * b0 + k0) * p.nodes + (3 * b1 + k1);
+                grid[node] = make_float4(v[c * 4 + 0], v[c * 4 + 1], v[c * 4 + 2], v[c * 4 + 3]);
+            }
+    } else {
+#pragma unroll
+        for (int k0 = 0; k0 < 3; k0++)
+            grid[3 * key + k0] = make_float4(v[k0 * 3 + 0], v[k0 * 3 + 1], v[k0 * 3 + 2], 0.0f);
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(PT_THREADS) k_spread(const float *y, const uint32_t *keys,
+                                                       const uint32_t *vals, int n,
+                                                       const EpochScalars *scal, float4 *grid,
+                                                       float *part, int nchunks) {
+    constexpr int NV = D == 2 ? NV2 : NV1;
+    const GridPlan p = scal->plan;
+    const Lagrange3 lb;
+    const int lane = threadIdx.x & 63;
+    const int wave0 = (blockIdx.x * PT_THREADS + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * PT_THREADS) >> 6;
+    float *part_first = part;
+    float *part_last = part + (size_t)nchunks * NV;
+    for (int c = wave0; c < nchunks; c += nwaves) {
+        int s = c * 64 + lane;
+        bool valid = s < n;
+        uint32_t key = valid ? keys[s] : 0xffffffffu;
+        float v[NV];
+        if (valid) {
+            point_contrib<D>(y, vals[s], p, lb, v);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NV; t++) v[t] = 0.0f;
+        }
+        uint32_t prev = __shfl_up(key, 1, 64);
+        bool head = lane == 0 || prev != key;
+        uint64_t heads = __ballot(head);
+        uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        int my_head = 63 - __clzll(heads & upto);
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            bool take = lane - off >= my_head;
+#pragma unroll
+            for (int t = 0; t < NV; t++) {
+                float u = __shfl_up(v[t], off, 64);
+                if (take) v[t] += u;
+            }
+        }
+        if (!valid) continue;
+        bool tail = (s == n - 1) || lane == 63 || ((heads >> (lane + 1)) & 1ull);
+        if (!tail) continue;
+        int s_head = c * 64 + my_head;
+        bool starts = s_head == 0 || keys[s_head - 1] != key;
+        bool ends = s == n - 1 || keys[s + 1] != key;
+        if (starts && ends) {
+            write_box<D>(grid, p, key, v);
+        } else {
+            if (!starts)
+                for (int t = 0; t < NV; t++) part_first[(size_t)c * NV + t] = v[t];
+            if (!ends)
+                for (int t = 0; t < NV; t++) part_last[(size_t)c * NV + t] = v[t];
+        }
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const uint32_t *keys, int n,
+                                                             const EpochScalars *scal,
+                                                             float4 *grid, const float *part,
+                                                             int nchunks) {
+    constexpr int NV = D == 2 ? NV2 : NV1;
+    const GridPlan p = scal->plan;
+    const float *part_first = part;
+    const float *part_last = part + (size_t)nchunks * NV;
+    for (int c = blockIdx.x * PT_THREADS + threadIdx.x; c < nchunks;
+         c += gridDim.x * PT_THREADS) {
+        int last_s = min(c * 64 + 63, n - 1);
+        uint32_t key = keys[last_s];
+        if (!(last_s + 1 < n && keys[last_s + 1] == key)) continue;  // run ends in this chunk
+        int first_s = c * 64;
+        bool whole = keys[first_s] == key;
+        if (whole && first_s > 0 && keys[first_s - 1] == key) continue;  // not the run's owner
+        float tot[NV];
+        for (int t = 0; t < NV; t++) tot[t] = part_last[(size_t)c * NV + t];
+        for (int j = c + 1; j < nchunks; j++) {
+            for (int t = 0; t < NV; t++) tot[t] += part_first[(size_t)j * NV + t];
+            int lj = min(j * 64 + 63, n - 1);
+            if (!(keys[lj] == key && lj + 1 < n && keys[lj + 1] == key)) break;
+        }
+        write_box<D>(grid, p, key, tot);
+    }
+}
+
+// ---- 5. gather + reconstruct (interpolation.rs:282-332) -------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(PT_THREADS) k_gather(const float *y, int n,
+                                                       EpochScalars *scal, const float4 *grid,
+                                                       float *frep, double *dpart) {
+    __shared__ double red[PT_THREADS / 64];
+    const GridPlan p = scal->plan;
+    const Lagrange3 lb;
+    double zacc = 0.0;
+    const int stride = gridDim.x * PT_THREADS;
+    for (int i = blockIdx.x * PT_THREADS + threadIdx.x; i < n; i += stride) {
+        if (D == 2) {
+            float2 pt = reinterpret_cast<const float2 *>(y)[i];
+            float l0, l1, w0[3], w1[3];
+            int b0 = locate_box(pt.x, p, l0);
+            int b1 = locate_box(pt.y, p, l1);
+            lb.weights(l0, w0);
+            lb.weights(l1, w1);
+            float phi0 = 0.f, phi1 = 0.f, phi2 = 0.f, phi3 = 0.f;
+#pragma unroll
+            for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+                for (int k0 = 0; k0 < 3; k0++) {
+                    float wt = w0[k0] * w1[k1];
+                    float4 g = grid[(size_t)(3 * b0 + k0) * p.nodes + (3 * b1 + k1)];
+                    phi0 = phi0 + wt * g.x;
+                    phi1 = phi1 + wt * g.y;
+                    phi2 = phi2 + wt * g.z;
+                    phi3 = phi3 + wt * g.w;
+                }
+            float2 f;
+            f.x = pt.x * phi0 - phi1;
+            f.y = pt.y * phi0 - phi2;
+            float norm_sq = 0.0f, cross = 0.0f;
+            norm_sq = norm_sq + pt.x * pt.x;
+            cross = cross + pt.x * phi1;
+            norm_sq = norm_sq + pt.y * pt.y;
+            cross = cross + pt.y * phi2;
+            reinterpret_cast<float2 *>(frep)[i] = f;
+            float zt = (1.0f + norm_sq) * phi0 - 2.0f * cross + phi3;
+            zacc += (double)zt;
+        } else {
+            float pt = y[i];
+            float l0, w0[3];
+            int b0 = locate_box(pt, p, l0);
+            lb.weights(l0, w0);
+            float phi0 = 0.f, phi1 = 0.f, phi2 = 0.f;
+#pragma unroll
+            for (int k0 = 0; k0 < 3; k0++) {
+                float4 g = grid[3 * b0 + k0];
+                phi0 = phi0 + w0[k0] * g.x;
+                phi1 = phi1 + w0[k0] * g.y;
+                phi2 = phi2 + w0[k0] * g.w;
+            }
+            frep[i] = pt * phi0 - phi1;
+            float norm_sq = 0.0f + pt * pt;
+            float cross = 0.0f + pt * phi1;
+            float zt = (1.0f + norm_sq) * phi0 - 2.0f * cross + phi2;
+            zacc += (double)zt;
+        }
+    }
+    double bs = block_sum_f64(zacc, red);
+    if (threadIdx.x == 0) dpart[blockIdx.x] = bs;
+    if (!arrive_last(&scal->ctr[CTR_GATHER], gridDim.x)) return;
+    double t = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += PT_THREADS)
+        t += dpart[b];
+    t = block_sum_f64(t, red);
+    if (threadIdx.x == 0) {
+        scal->zsum = t;
+        float z = (float)(t - (double)n);
+        scal->z = z;
+        scal->inv_z = 1.0f / z;
+        scal->ctr[CTR_GATHER] = 0;
+    }
+}
+
+// ---- driver ----------------------------------------------------------------------------------
+
+void fitsne_repulsion(FitsneDev &d, hipStream_t s) {
+    const int nb = point_blocks(d.n);
+    const int nchunks = (int)ceil_div(d.n, 64);
+    const int spread_blocks = (int)std::min<uint64_t>(ceil_div(nchunks, PT_THREADS / 64), 2048);
+    const int fix_blocks = (int)std::min<uint64_t>(ceil_div(nchunks, PT_THREADS), 1024);
+    const int zero_blocks = 1024;
+    if (d.dim == 2)
+        hipLaunchKernelGGL(k_box_keys<2>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
+                           d.keys, d.vals);
+    else
+        hipLaunchKernelGGL(k_box_keys<1>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
+                           d.keys, d.vals);
+    BHT_LAUNCH_CHECK();
+    // boxes^D <= 10^6 < 2^20: three 8-bit passes
+    bool alt = radix_sort_pairs<uint32_t>(d.keys, d.vals, d.keys_alt, d.vals_alt, (uint64_t)d.n,
+                                          0, d.dim == 2 ? 24 : 16, d.sort_temp, s);
+    const uint32_t *sk = alt ? d.keys_alt : d.keys;
+    const uint32_t *sv = alt ? d.vals_alt : d.vals;
+    hipLaunchKernelGGL(k_zero_grid, dim3(zero_blocks), dim3(256), 0, s, d.grid, d.scal, d.dim);
+    if (d.dim == 2) {
+        hipLaunchKernelGGL(k_spread<2>, dim3(spread_blocks), dim3(PT_THREADS), 0, s, d.y, sk, sv,
+                           d.n, d.scal, d.grid, d.part, nchunks);
+        hipLaunchKernelGGL(k_spread_fixup<2>, dim3(fix_blocks), dim3(PT_THREADS), 0, s, sk, d.n,
+                           d.scal, d.grid, d.part, nchunks);
+    } else {
+        hipLaunchKernelGGL(k_spread<1>, dim3(spread_blocks), dim3(PT_THREADS), 0, s, d.y, sk, sv,
+                           d.n, d.scal, d.grid, d.part, nchunks);
+        hipLaunchKernelGGL(k_spread_fixup<1>, dim3(fix_blocks), dim3(PT_THREADS), 0, s, sk, d.n,
+                           d.scal, d.grid, d.part, nchunks);
+    }
+    BHT_LAUNCH_CHECK();
+    fft_kernel_spectrum(d, s);
+    fft_convolve(d, s);
+    if (d.dim == 2)
+        hipLaunchKernelGGL(k_gather<2>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
+                           d.grid, d.frep, d.dpart);
+    else
+        hipLaunchKernelGGL(k_gather<1>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
+                           d.grid, d.frep, d.dpart);
+    BHT_LAUNCH_CHECK();
+}
+
+}  // namespace bht

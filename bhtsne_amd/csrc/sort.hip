@@ -1,0 +1,227 @@
+// Device-wide exclusive scan and stable LSD radix sort of (key, u32 value) pairs.
+//
+// Used to group points by FIt-SNE box (u32 keys), to order Barnes-Hut Morton codes (u64 keys) and
+// to bucket the non-mutual kNN edges by destination row during symmetrisation.  Stability is what
+// makes every downstream reduction order (and therefore every result) independent of scheduling.
+#include "internal.hpp"
+
+namespace bht {
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 4096
+
+// Per-tile sums.
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tile_sums(const uint32_t *in, uint64_t n,
+                                                                 T *sums) {
+    __shared__ double red[SCAN_THREADS / 64];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint64_t s = 0;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t idx = base + (uint64_t)i * SCAN_THREADS + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    // integer sum through doubles is exact below 2^53
+    double t = block_sum_f64((double)s, red);
+    if (threadIdx.x == 0) sums[blockIdx.x] = (T)t;
+}
+
+// Exclusive scan of the tile sums by one block (tiles <= a few hundred thousand).
+template <typename T>
+__global__ void __launch_bounds__(1024) k_scan_sums(T *sums, uint64_t nt) {
+    __shared__ T part[1024];
+    uint64_t per = (nt + 1023) / 1024;
+    uint64_t b = threadIdx.x * per, e = b + per < nt ? b + per : nt;
+    T s = 0;
+    for (uint64_t i = b; i < e; i++) s += sums[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T run = 0;
+        for (int i = 0; i < 1024; i++) {
+            T v = part[i];
+            part[i] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    T run = part[threadIdx.x];
+    for (uint64_t i = b; i < e; i++) {
+        T v = sums[i];
+        sums[i] = run;
+        run += v;
+    }
+}
+
+// Rescan each tile with its offset: thread-sequential runs of SCAN_ITEMS, block scan of runs.
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t *in, T *out,
+                                                             uint64_t n, const T *sums,
+                                                             T *total_out) {
+    __shared__ T part[SCAN_THREADS];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t idx = base + i;
+        v[i] = idx < n ? in[idx] : 0u;
+        s += v[i];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the 256 runs
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+        T t = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    T run = sums[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t idx = base + i;
+        if (idx < n) out[idx] = run;
+        run += v[i];
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1)
+        *total_out = run;
+}
+
+}  // namespace
+
+size_t scan_temp_bytes(uint64_t n) {
+    return (size_t)(ceil_div(n, SCAN_TILE) + 1) * sizeof(uint64_t);
+}
+
+template <typename T>
+void scan_exclusive(const uint32_t *in, T *out, uint64_t n, void *temp, T *total_out,
+                    hipStream_t s) {
+    if (n == 0) return;
+    unsigned nt = ceil_div(n, SCAN_TILE);
+    T *sums = (T *)temp;
+    hipLaunchKernelGGL(k_scan_tile_sums<T>, dim3(nt), dim3(SCAN_THREADS), 0, s, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(1024), 0, s, sums, (uint64_t)nt);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3(nt), dim3(SCAN_THREADS), 0, s, in, out, n, sums,
+                       total_out);
+    BHT_LAUNCH_CHECK();
+}
+
+template void scan_exclusive<uint32_t>(const uint32_t *, uint32_t *, uint64_t, void *, uint32_t *,
+                                       hipStream_t);
+template void scan_exclusive<uint64_t>(const uint32_t *, uint64_t *, uint64_t, void *, uint64_t *,
+                                       hipStream_t);
+
+// ---- radix sort ------------------------------------------------------------------------------
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 2048 keys per tile
+constexpr int RS_BINS = 256;
+
+template <typename K>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K *keys, uint64_t n, int shift,
+                                                        uint32_t *hist, unsigned ntiles) {
+    __shared__ uint32_t h[RS_BINS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        uint64_t idx = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(unsigned)(keys[idx] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+template <typename K>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *keys, const uint32_t *vals,
+                                                           K *keys_out, uint32_t *vals_out,
+                                                           uint64_t n, int shift,
+                                                           const uint32_t *offs,
+                                                           unsigned ntiles) {
+    __shared__ uint32_t base[RS_BINS];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][RS_BINS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    base[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint64_t tile0 = (uint64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+#pragma unroll
+        for (int q = 0; q < RS_THREADS / 64; q++) wcnt[q][threadIdx.x] = 0;
+        __syncthreads();
+        uint64_t idx = tile0 + (uint64_t)r * RS_THREADS + threadIdx.x;
+        bool valid = idx < n;
+        K key = valid ? keys[idx] : (K)0;
+        unsigned d = (unsigned)(key >> shift) & 0xffu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        unsigned rank = (unsigned)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[d] + rank;
+            for (int q = 0; q < w; q++) pos += wcnt[q][d];
+            keys_out[pos] = key;
+            vals_out[pos] = vals[idx];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int q = 0; q < RS_THREADS / 64; q++) add += wcnt[q][threadIdx.x];
+        base[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+size_t radix_sort_temp_bytes(uint64_t n) {
+    uint64_t ntiles = ceil_div(n, RS_TILE);
+    uint64_t hist = ntiles * RS_BINS;
+    return (size_t)hist * 2 * sizeof(uint32_t) + scan_temp_bytes(hist) + 256;
+}
+
+// Sorts (keys, vals) by bits [begin_bit, end_bit) of the key, stable.  Ping-pongs between the
+// two buffer pairs; returns true when the result is in (keys_alt, vals_alt).
+template <typename K>
+bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
+                      int begin_bit, int end_bit, void *temp, hipStream_t s) {
+    if (n == 0) return false;
+    unsigned ntiles = ceil_div(n, RS_TILE);
+    uint64_t nh = (uint64_t)ntiles * RS_BINS;
+    uint32_t *hist = (uint32_t *)temp;
+    uint32_t *offs = hist + nh;
+    void *stemp = (void *)(((uintptr_t)(offs + nh) + 255) & ~(uintptr_t)255);
+    K *ki = keys, *ko = keys_alt;
+    uint32_t *vi = vals, *vo = vals_alt;
+    bool alt = false;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist<K>, dim3(ntiles), dim3(RS_THREADS), 0, s, ki, n, shift, hist,
+                           ntiles);
+        scan_exclusive<uint32_t>(hist, offs, nh, stemp, nullptr, s);
+        hipLaunchKernelGGL(k_rs_scatter<K>, dim3(ntiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo,
+                           n, shift, offs, ntiles);
+        BHT_LAUNCH_CHECK();
+        K *tk = ki; ki = ko; ko = tk;
+        uint32_t *tv = vi; vi = vo; vo = tv;
+        alt = !alt;
+    }
+    return alt;
+}
+
+template bool radix_sort_pairs<uint32_t>(uint32_t *, uint32_t *, uint32_t *, uint32_t *, uint64_t,
+                                         int, int, void *, hipStream_t);
+template bool radix_sort_pairs<uint64_t>(uint64_t *, uint32_t *, uint64_t *, uint32_t *, uint64_t,
+                                         int, int, void *, hipStream_t);
+
+}  // namespace bht

@@ -1,0 +1,167 @@
+/*
+ * bhtsne_hip.h -- C ABI of the MI355X (gfx950) t-SNE engine: sparse affinity construction and the
+ * per-epoch gradient of the bhtsne crate, for the Euclidean case, as a drop-in behind the crate's
+ * builder.
+ *
+ * Reference interface each entry point replaces (paths relative to the bhtsne 0.7.10 crate root):
+ *
+ *   tSNE::new(&data)                         src/lib.rs:241-266   -> bht_job_create
+ *   learning_rate .. perplexity setters      src/lib.rs:277-371   -> bht_params fields
+ *   initial_embedding(..)                    src/lib.rs:431-435   -> bht_job_set_initial_embedding
+ *   barnes_hut(theta, euclidean)             src/lib.rs:701-742   -> bht_job_barnes_hut
+ *   barnes_hut_with_neighbors(theta, nbrs)   src/lib.rs:757-772   -> bht_job_barnes_hut_with_neighbors
+ *   fit_sne(euclidean)                       src/lib.rs:835-871   -> bht_job_fit_sne
+ *   fit_sne_with_neighbors(nbrs)             src/lib.rs:884-894   -> bht_job_fit_sne_with_neighbors
+ *   epoch_callback(..)                       src/lib.rs:415-422   -> bht_epoch_cb argument
+ *   embedding()                              src/lib.rs:438-440   -> bht_job_get_embedding
+ *   kl_divergence()                          src/lib.rs:448-477   -> bht_job_kl_divergence
+ *   affinities() / with_affinities(..)       src/lib.rs:1019-1051 -> bht_job_get_affinities /
+ *                                                                    bht_job_set_affinities
+ *   VPTree::search (k nearest neighbours)    src/tsne/vptree.rs:274-319 -> bht_knn_f32
+ *   search_beta (bandwidth bisection)        src/tsne/mod.rs:201-275    -> bht_search_beta_f32
+ *   symmetrize_sparse_matrix                 src/tsne/mod.rs:306-406    -> bht_symmetrize
+ *
+ * Every function returns a bht_status.  On failure bht_last_error() (thread-local) holds a message;
+ * for the inputs the reference rejects with a panic the message text is the reference's own
+ * ("error: the provided perplexity is too large ...", "... must have the same length ...",
+ * "... out of range ...", "error: initial embedding has ...", "error: theta value must be ...").
+ *
+ * All pointers are host pointers owned by the caller; the library copies in and out during the
+ * call.  Device state (data, P, embedding, optimiser buffers) lives in the job between calls.
+ * A context and its jobs may be used from one host thread at a time.
+ */
+#ifndef BHTSNE_HIP_H
+#define BHTSNE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t bht_status;
+#define BHT_OK 0
+#define BHT_ERR_INVALID 1 /* input rejected (reference panic message in bht_last_error) */
+#define BHT_ERR_HIP 2     /* HIP runtime error */
+#define BHT_ERR_OOM 3     /* device allocation failed or HBM budget exceeded */
+#define BHT_ERR_STATE 4   /* call out of order */
+
+typedef struct bht_ctx bht_ctx;
+typedef struct bht_job bht_job;
+
+/* Called after every epoch with the epoch index and the (recentred) embedding, n * d_out values
+ * (src/lib.rs:1175-1178).  Runs on the calling thread. */
+typedef void (*bht_epoch_cb)(uint64_t epoch, const float *embedding, uint64_t len, void *user);
+
+#define BHT_BANDWIDTH_FP64 0u     /* bandwidth-search sums in FP64 (default) */
+#define BHT_BANDWIDTH_F32_SEQ 1u  /* sums in f32, sequential, as the reference's loop */
+
+/* Builder state of tSNE (src/lib.rs:149-176) with the defaults of tSNE::new (src/lib.rs:241-266). */
+typedef struct bht_params {
+    uint32_t d_out;                 /* embedding dimension: 2 (Barnes-Hut), 1 or 2 (FIt-SNE) */
+    uint32_t bandwidth_mode;        /* BHT_BANDWIDTH_* */
+    uint64_t epochs;                /* 1000 */
+    uint64_t momentum_switch_epoch; /* 250 */
+    uint64_t stop_lying_epoch;      /* 250 */
+    float learning_rate;            /* <= 0: auto, max(n / early_exaggeration / 4, 50) */
+    float momentum;                 /* 0.5 */
+    float final_momentum;           /* 0.8 */
+    float early_exaggeration;       /* 12 */
+    float perplexity;               /* 20 */
+    uint32_t min_boxes;             /* FIt-SNE boxes per axis lower bound; 10 is the reference's */
+    uint32_t max_boxes;             /* FIt-SNE grid capacity per axis, <= 1000 (the reference's) */
+    uint32_t reserved;
+} bht_params;
+
+void bht_params_default(bht_params *p);
+
+/* ---- context --------------------------------------------------------------------------------- */
+bht_status bht_ctx_create(int device, uint64_t hbm_budget_bytes, bht_ctx **out);
+void bht_ctx_destroy(bht_ctx *ctx);
+const char *bht_last_error(void);
+/* Bytes of device memory currently held by the context's jobs. */
+uint64_t bht_ctx_bytes_in_use(const bht_ctx *ctx);
+
+/* ---- job (one tSNE instance) ----------------------------------------------------------------- */
+bht_status bht_job_create(bht_ctx *ctx, uint64_t n_samples, const bht_params *params,
+                          bht_job **out);
+void bht_job_destroy(bht_job *job);
+bht_status bht_job_set_params(bht_job *job, const bht_params *params);
+bht_status bht_job_get_params(const bht_job *job, bht_params *out);
+
+/* Sample vectors for the Euclidean metric: n_samples rows of d_in f32, row-major. */
+bht_status bht_job_set_data_f32(bht_job *job, const float *x, uint64_t n, uint32_t d_in);
+/* Seed embedding for the next fit, consumed by it (src/lib.rs:672-684). */
+bht_status bht_job_set_initial_embedding(bht_job *job, const float *y, uint64_t len);
+
+/* Fits.  theta > 0 (src/lib.rs:1182-1188).  Neighbours are given as concatenated rows:
+ * row i holds row_lens[i] entries of (index, distance), like &[Vec<Neighbor<f32>>]. */
+bht_status bht_job_barnes_hut(bht_job *job, float theta, bht_epoch_cb cb, void *user);
+bht_status bht_job_barnes_hut_with_neighbors(bht_job *job, float theta, const uint64_t *nbr_index,
+                                             const float *nbr_distance, const uint64_t *row_lens,
+                                             uint64_t n_rows, bht_epoch_cb cb, void *user);
+bht_status bht_job_fit_sne(bht_job *job, bht_epoch_cb cb, void *user);
+bht_status bht_job_fit_sne_with_neighbors(bht_job *job, const uint64_t *nbr_index,
+                                          const float *nbr_distance, const uint64_t *row_lens,
+                                          uint64_t n_rows, bht_epoch_cb cb, void *user);
+
+/* Results. */
+bht_status bht_job_get_embedding(const bht_job *job, float *out, uint64_t len);
+/* KL divergence of the last fit with the fit's own Z estimate: the reference's value
+ * (src/tsne/mod.rs:738-775, which weights every edge of row i by p_values[i]) and the true
+ * sum over edges of p log(p / q).  BHT_ERR_STATE before any fit (the reference returns None). */
+bht_status bht_job_kl_divergence(bht_job *job, double *reference_kl, double *true_kl);
+
+/* Affinity cache (src/lib.rs:1019-1051): two-phase.  With rows == NULL, *nnz receives the number
+ * of entries; otherwise rows (n+1, usize offsets), cols and vals (nnz each) are filled.  Values are
+ * un-exaggerated unless stop-lying already fired, as affinities() returns them. */
+bht_status bht_job_get_affinities(const bht_job *job, uint64_t *rows, uint32_t *cols, float *vals,
+                                  uint64_t *nnz, float *perplexity);
+bht_status bht_job_set_affinities(bht_job *job, const uint64_t *rows, const uint32_t *cols,
+                                  const float *vals, uint64_t n_rows_plus_one, uint64_t nnz,
+                                  float perplexity);
+
+/* ---- stepping interface (the epoch loop of src/lib.rs:961-1014 in pieces) ---------------------
+ * prepare: P from data / cached affinities normalised and exaggerated, optimiser reset, embedding
+ * seeded (src/lib.rs:663-685, 1227-1236).  method: BHT_METHOD_*.  step: run n epochs.
+ * finish: recentre, apply a pending stop-lying, record the fit for kl_divergence(). */
+#define BHT_METHOD_BARNES_HUT 0u
+#define BHT_METHOD_FITSNE 1u
+bht_status bht_job_prepare(bht_job *job, uint32_t method, float theta);
+bht_status bht_job_step(bht_job *job, uint64_t n_epochs, bht_epoch_cb cb, void *user);
+bht_status bht_job_finish(bht_job *job);
+/* Blocks until the job's queued device work is done. */
+bht_status bht_job_synchronize(bht_job *job);
+
+/* State access for parity checks: forces of the current state (attractive, repulsive, 1/Z;
+ * src/repulsion.rs:95-158, 206-234) without updating, and the optimiser state (n * d_out each). */
+bht_status bht_job_forces(bht_job *job, float *attractive, float *repulsive, float *inverse_z);
+bht_status bht_job_get_state(const bht_job *job, float *y, float *uy, float *gains);
+bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, const float *gains);
+/* FIt-SNE grid of the last repulsion: boxes per axis, FFT length, joint min and max. */
+bht_status bht_job_grid_info(const bht_job *job, uint32_t *boxes, uint32_t *fft_len, float *lo,
+                             float *hi);
+
+/* Per-phase device time of the epochs run while timing is enabled, measured with HIP events on
+ * the job's own stream.  out[0] = repulsion ms (grid plan .. gather), out[1] = attraction+update
+ * ms, out[2] = epochs timed.  Enabling resets the counters. */
+bht_status bht_job_set_timing(bht_job *job, int enable);
+bht_status bht_job_get_timing(bht_job *job, double *out3);
+
+/* ---- operators on host buffers (the reference's internal steps, for parity checks) ----------- */
+/* k nearest neighbours of every row (self excluded), ascending (distance, index). */
+bht_status bht_knn_f32(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,
+                       uint32_t *idx_out, float *dist_out);
+/* Per-row bandwidth search: conditional P (n * k) from neighbour distances (n * k). */
+bht_status bht_search_beta_f32(bht_ctx *ctx, const float *dist, uint64_t n, uint32_t k,
+                               float perplexity, uint32_t mode, float *p_out);
+/* Symmetrisation into CSR with the reference's row order.  Two-phase like get_affinities. */
+bht_status bht_symmetrize(bht_ctx *ctx, const uint32_t *cols, const float *vals, uint64_t n,
+                          uint32_t k, uint64_t *rows_out, uint32_t *cols_out, float *vals_out,
+                          uint64_t *nnz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BHTSNE_HIP_H */

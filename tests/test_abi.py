@@ -1,0 +1,68 @@
+"""CPU-side checks of the C-ABI library: it loads and exports every function the header declares
+(no compute call is made without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "bhtsne_hip.h")
+LIB = os.path.join(ROOT, "bhtsne_amd", "libbhtsne_hip.so")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bht_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        import bhtsne_amd.build as b
+        b.build()
+    return ctypes.CDLL(LIB)
+
+
+def test_header_declares_the_builder_surface():
+    names = _declared()
+    for required in ["bht_job_barnes_hut", "bht_job_barnes_hut_with_neighbors",
+                     "bht_job_fit_sne", "bht_job_fit_sne_with_neighbors", "bht_job_get_embedding",
+                     "bht_job_kl_divergence", "bht_job_get_affinities", "bht_job_set_affinities",
+                     "bht_job_set_initial_embedding", "bht_knn_f32", "bht_search_beta_f32",
+                     "bht_symmetrize"]:
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_signatures_cover_the_header():
+    from bhtsne_amd import _lib
+    assert set(_declared()) == set(_lib.SIGNATURES)
+
+
+def test_params_default_is_the_reference_builder_default(lib):
+    from bhtsne_amd import _lib
+    L = _lib.load()
+    p = _lib.Params()
+    L.bht_params_default(ctypes.byref(p))
+    # src/lib.rs:241-266
+    assert (p.epochs, p.momentum, p.final_momentum, p.momentum_switch_epoch,
+            p.stop_lying_epoch, p.early_exaggeration, p.perplexity, p.learning_rate) == \
+        (1000, 0.5, pytest.approx(0.8), 250, 250, 12.0, 20.0, 0.0)
+    assert p.d_out == 2 and p.min_boxes == 10 and p.max_boxes == 1000
+
+
+def test_ctx_create_without_gpu_reports_an_error(lib):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from bhtsne_amd import _lib
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    st = L.bht_ctx_create(0, 0, ctypes.byref(h))
+    assert st != 0 and L.bht_last_error()
