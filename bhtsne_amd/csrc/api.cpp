@@ -17,6 +17,7 @@
 
 #include "../../include/bhtsne_hip.h"
 #include "affinity.hpp"
+#include "bh.hpp"
 #include "internal.hpp"
 
 using namespace bht;
@@ -188,6 +189,9 @@ struct bht_job {
         sort_temp, scal;
     int nodes_cap = 0, len_cap = 0;
     FitsneDev fd{};
+    // Barnes-Hut workspace
+    DevBuf codes, codes_alt, bidx, bidx_alt, bsort_temp, bnode, bmeta, brange, bhw2, bcount;
+    BhDev bd{};
 
     // run state
     int method = -1;
@@ -294,6 +298,36 @@ void ensure_fitsne_workspace(bht_job *j) {
     d.tw = &c->tw;
 }
 
+void ensure_bh_workspace(bht_job *j) {
+    ensure_state_buffers(j);
+    bht_ctx *c = j->ctx;
+    const uint64_t n = std::max<uint64_t>(j->n, 1);
+    j->codes.ensure(c, n * 8);
+    j->codes_alt.ensure(c, n * 8);
+    j->bidx.ensure(c, n * 4);
+    j->bidx_alt.ensure(c, n * 4);
+    j->bsort_temp.ensure(c, radix_sort_temp_bytes(n));
+    j->bnode.ensure(c, 2 * n * sizeof(float4));
+    j->bmeta.ensure(c, 2 * n * 4);
+    j->brange.ensure(c, 4 * n * 4);
+    j->bhw2.ensure(c, 64 * 4);
+    j->bcount.ensure(c, 16);
+    BhDev &d = j->bd;
+    d.n = (int)j->n;
+    d.codes = j->codes.as<uint64_t>();
+    d.codes_alt = j->codes_alt.as<uint64_t>();
+    d.idx = j->bidx.as<uint32_t>();
+    d.idx_alt = j->bidx_alt.as<uint32_t>();
+    d.sort_temp = j->bsort_temp.p;
+    d.nodes.node = j->bnode.as<float4>();
+    d.nodes.meta = j->bmeta.as<uint32_t>();
+    d.nodes.range = j->brange.as<uint32_t>();
+    d.nodes.hw2 = j->bhw2.as<float>();
+    d.nodes.counts = j->bcount.as<uint32_t>();
+    d.scal = j->scal.as<EpochScalars>();
+    d.dpart = j->dpart.as<double>();
+}
+
 // Builds P (CSR) from per-row neighbour indices / distances already on the device.
 void affinities_from_knn(bht_job *j, uint32_t *d_idx, float *d_dist, uint32_t k) {
     bht_ctx *c = j->ctx;
@@ -365,7 +399,7 @@ void build_affinities_from_data(bht_job *j) {
 void prepare_run(bht_job *j, int method, float theta, bool cached) {
     if (cached) j->stop_lying_fired = false;
     if (method == FIT_FITSNE) ensure_fitsne_workspace(j);
-    else ensure_state_buffers(j);
+    else ensure_bh_workspace(j);
     const uint64_t len = j->n * j->dim();
     hipStream_t s = j->stream;
     // normalise P with the exaggeration folded in (mod.rs:284-293)
@@ -420,7 +454,13 @@ void repulsion(bht_job *j) {
         fitsne_center_plan(d, j->stream);
         fitsne_repulsion(d, j->stream);
     } else {
-        throw Error(ERR_INVALID, "Barnes-Hut repulsion is not available in this build");
+        // recentre + per-axis bounds (arena.rs:65-94), then the arena and its traversal
+        center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
+                    j->dpart.as<double>(), j->stream);
+        BhDev &b = j->bd;
+        b.y = j->ycur();
+        b.frep = j->frep.as<float>();
+        if (j->n > 0) bh_repulsion(b, j->theta, j->stream, CTR_BH);
     }
 }
 
@@ -903,7 +943,31 @@ bht_status bht_job_get_timing(bht_job *job, double *out3) {
 
 bht_status bht_job_kl_divergence(bht_job *job, double *reference_kl, double *true_kl) {
     if (!job) return fail(BHT_ERR_INVALID, "null argument");
-    return fail(BHT_ERR_STATE, "kl_divergence is not available in this build");
+    return guarded([&] {
+        if (job->fit_kind == FIT_NONE) throw Error(ERR_STATE, "no fit yet");  // -> None
+        // the reference indexes p_values by the sample index (mod.rs:768-770)
+        if (job->nnz < job->n)
+            throw Error(ERR_INVALID, "kl_divergence needs at least one affinity per sample");
+        // Z from the fit's own estimator on the current embedding (mod.rs:635-720)
+        const int saved_method = job->method;
+        const float saved_theta = job->theta;
+        job->method = job->fit_kind;
+        job->theta = job->fit_theta;
+        if (job->method == FIT_FITSNE) ensure_fitsne_workspace(job);
+        else ensure_bh_workspace(job);
+        repulsion(job);
+        EpochScalars *sc = job->scal.as<EpochScalars>();
+        sparse_kl(job->ycur(), (int)job->n, (int)job->dim(), job->rows.as<uint32_t>(),
+                  job->cols.as<uint32_t>(), job->vals.as<float>(), sc, job->dpart.as<double>(),
+                  job->stream);
+        EpochScalars h;
+        BHT_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, job->stream));
+        BHT_HIP(hipStreamSynchronize(job->stream));
+        job->method = saved_method;
+        job->theta = saved_theta;
+        if (reference_kl) *reference_kl = h.kl_ref;
+        if (true_kl) *true_kl = h.kl_true;
+    });
 }
 
 bht_status bht_knn_f32(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,

@@ -1,0 +1,287 @@
+// Barnes-Hut repulsion on the device (src/tsne/arena.rs:137-392, src/tsne/morton.rs:103-120,
+// src/tsne/morton/d2.rs:8-47, src/repulsion.rs:95-158), D = 2.
+//
+//  1. Morton codes of the quantised points (32 bits per axis, u64 words) from the per-axis
+//     bounding box, with the reference's f32 quantisation.
+//  2. Stable LSD radix sort of (code, index): ties keep index order.
+//  3. Breadth-first emission of the compressed quadtree in one workgroup: every round expands the
+//     current frontier in node order (children located by binary search of the sorted codes, a
+//     block scan assigns their slots), which yields exactly the reference's node numbering; then
+//     centres of mass bottom-up, round by round, with the reference's f32 summation order.
+//  4. One lane per point walks the tree with an LDS stack in the reference's push/pop order;
+//     lanes take points in Morton order so a wave's walks share their paths.  Z is summed in FP64.
+#include <cfloat>
+
+#include "internal.hpp"
+#include "bh.hpp"
+
+namespace bht {
+
+constexpr uint32_t SENTINEL = 0xffffffffu;
+constexpr int BITS = 32;
+constexpr int BUILD_THREADS = 1024;
+constexpr int TRAV_THREADS = 64;
+constexpr int STACK = 112;  // >= 4 + 3 * 33
+constexpr int MAX_ROUNDS = 64;
+
+__device__ __forceinline__ uint64_t part1by1(uint64_t x) {
+    x &= 0x00000000ffffffffull;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    return (x | (x << 1)) & 0x5555555555555555ull;
+}
+
+__device__ __forceinline__ uint32_t quantize(float p, float mn, float inv_scale) {
+    float scaled = floorf((p - mn) * inv_scale);
+    if (scaled > 0.0f) {
+        if (scaled >= 18446744073709551616.0f) return 0xffffffffu;
+        uint64_t v = (uint64_t)scaled;
+        return v < 0xffffffffull ? (uint32_t)v : 0xffffffffu;
+    }
+    return 0u;
+}
+
+__global__ void k_bh_codes(const float *y, int n, const EpochScalars *scal, uint64_t *codes,
+                           uint32_t *idx) {
+    const float scale = 4294967296.0f;  // 2^32 as f32
+    float mn[2], inv[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        mn[a] = scal->amin[a];
+        float extent = scal->amax[a] - scal->amin[a];
+        inv[a] = extent > 0.0f ? scale / extent : 0.0f;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float2 p = reinterpret_cast<const float2 *>(y)[i];
+        uint32_t q0 = quantize(p.x, mn[0], inv[0]);
+        uint32_t q1 = quantize(p.y, mn[1], inv[1]);
+        codes[i] = part1by1(q0) | (part1by1(q1) << 1);
+        idx[i] = (uint32_t)i;
+    }
+}
+
+// block-wide exclusive scan of one u32 per thread (BUILD_THREADS threads)
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *buf, uint32_t &total) {
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < BUILD_THREADS; off <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)off ? buf[threadIdx.x - off] : 0u;
+        __syncthreads();
+        buf[threadIdx.x] += t;
+        __syncthreads();
+    }
+    total = buf[BUILD_THREADS - 1];
+    uint32_t r = buf[threadIdx.x] - v;
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ uint32_t lower_bound_digit(const uint64_t *codes, uint32_t lo,
+                                                      uint32_t hi, int shift, uint32_t g) {
+    // first position in [lo, hi) whose digit at `shift` is >= g (digits are nondecreasing)
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if ((uint32_t)((codes[mid] >> shift) & 3ull) < g) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const float *y, int n,
+                                                            const uint64_t *codes,
+                                                            const uint32_t *sidx,
+                                                            EpochScalars *scal, BhNodes t) {
+    __shared__ uint32_t sbuf[BUILD_THREADS];
+    __shared__ uint32_t round_start[MAX_ROUNDS + 2];
+    __shared__ uint32_t s_nn;
+    __shared__ int s_rounds;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        // level half-widths from the largest axis extent (arena.rs:167-174)
+        float e0 = scal->amax[0] - scal->amin[0];
+        float e1 = scal->amax[1] - scal->amin[1];
+        float max_extent = fmaxf(fmaxf(0.0f, e0), e1);
+        for (int level = 0; level <= BITS; level++) {
+            float hw = max_extent / (float)(1ull << (level + 1));
+            t.hw2[level] = hw * hw;
+        }
+        t.node[0] = make_float4(0.f, 0.f, __uint_as_float((uint32_t)n), __uint_as_float(SENTINEL));
+        t.meta[0] = (uint32_t)BITS << 8;
+        t.range[0] = 0;
+        t.range[1] = (uint32_t)n;
+        s_nn = 1;
+        round_start[0] = 0;
+        round_start[1] = 1;
+        s_rounds = 1;
+    }
+    __syncthreads();
+    if (n == 0) {
+        if (tid == 0) t.counts[0] = 0;
+        return;
+    }
+    // breadth-first emission (arena.rs:197-259)
+    for (int r = 0; r < MAX_ROUNDS; r++) {
+        const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
+        if (f0 == f1) break;
+        for (uint32_t base = f0; base < f1; base += BUILD_THREADS) {
+            const uint32_t i = base + tid;
+            uint32_t nchild = 0, b[5] = {0, 0, 0, 0, 0};
+            int level = BITS;
+            bool internal = false;
+            if (i < f1) {
+                const uint32_t s = t.range[2 * i], e = t.range[2 * i + 1];
+                if (e - s > 1 && codes[s] != codes[e - 1]) {
+                    internal = true;
+                    const uint64_t x = codes[s] ^ codes[e - 1];
+                    const int highest = 63 - __clzll((long long)x);
+                    level = (BITS - 1) - highest / 2;
+                    const int shift = 2 * (BITS - 1 - level);
+                    b[0] = s;
+                    b[4] = e;
+                    for (uint32_t g = 1; g < 4; g++) b[g] = lower_bound_digit(codes, s, e, shift, g);
+                    for (int g = 0; g < 4; g++) nchild += b[g + 1] > b[g] ? 1u : 0u;
+                }
+            }
+            uint32_t total;
+            const uint32_t off = block_excl_scan(nchild, sbuf, total);
+            const uint32_t first = s_nn + off;
+            if (i < f1 && internal) {
+                float4 nd = t.node[i];
+                nd.w = __uint_as_float(first);
+                t.node[i] = nd;
+                t.meta[i] = nchild | ((uint32_t)level << 8);
+                uint32_t c = first;
+                for (int g = 0; g < 4; g++) {
+                    if (b[g + 1] <= b[g]) continue;
+                    t.node[c] = make_float4(0.f, 0.f, __uint_as_float(b[g + 1] - b[g]),
+                                            __uint_as_float(SENTINEL));
+                    t.meta[c] = (uint32_t)BITS << 8;
+                    t.range[2 * c] = b[g];
+                    t.range[2 * c + 1] = b[g + 1];
+                    c++;
+                }
+            }
+            __syncthreads();
+            if (tid == 0) s_nn += total;
+            __syncthreads();
+        }
+        if (tid == 0) {
+            round_start[r + 2] = s_nn;
+            s_rounds = r + 1;
+        }
+        __syncthreads();
+    }
+    const int rounds = s_rounds;
+    // centres of mass, deepest round first (arena.rs:261-309)
+    for (int r = rounds - 1; r >= 0; r--) {
+        const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
+        for (uint32_t i = f0 + tid; i < f1; i += BUILD_THREADS) {
+            float4 nd = t.node[i];
+            const uint32_t count = __float_as_uint(nd.z);
+            const uint32_t fc = __float_as_uint(nd.w);
+            float cx = 0.0f, cy = 0.0f;
+            if (fc == SENTINEL) {
+                for (uint32_t s = t.range[2 * i]; s < t.range[2 * i + 1]; s++) {
+                    float2 p = reinterpret_cast<const float2 *>(y)[sidx[s]];
+                    cx += p.x;
+                    cy += p.y;
+                }
+            } else {
+                const uint32_t cc = t.meta[i] & 0xffu;
+                for (uint32_t ch = fc; ch < fc + cc; ch++) {
+                    float4 cn = t.node[ch];
+                    const float w = (float)__float_as_uint(cn.z);
+                    cx += cn.x * w;
+                    cy += cn.y * w;
+                }
+            }
+            const float inv = 1.0f / (float)count;
+            nd.x = cx * inv;
+            nd.y = cy * inv;
+            t.node[i] = nd;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) t.counts[0] = s_nn;
+}
+
+__global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const float *y, int n,
+                                                              const uint32_t *sidx, BhNodes t,
+                                                              float theta_sq, float *frep,
+                                                              EpochScalars *scal,
+                                                              double *dpart, int slot) {
+    __shared__ uint32_t stack[STACK][TRAV_THREADS];
+    __shared__ double red[1];
+    const int lane = threadIdx.x;
+    double zacc = 0.0;
+    for (int s = blockIdx.x * TRAV_THREADS + lane; s < n; s += gridDim.x * TRAV_THREADS) {
+        const uint32_t i = sidx[s];
+        const float2 qp = reinterpret_cast<const float2 *>(y)[i];
+        float fx = 0.0f, fy = 0.0f, q_sum = 0.0f;
+        int top = 0;
+        stack[top++][lane] = 0;
+        while (top > 0) {
+            top--;
+            const uint32_t id = stack[top][lane];
+            const float4 nd = t.node[id];
+            const float dx = qp.x - nd.x, dy = qp.y - nd.y;
+            float distance = 0.0f;
+            distance += dx * dx;
+            distance += dy * dy;
+            const uint32_t fc = __float_as_uint(nd.w);
+            if (fc == SENTINEL) {
+                if (distance == 0.0f) continue;  // the query's own leaf
+            } else {
+                const uint32_t m = t.meta[id];
+                if (t.hw2[m >> 8] >= theta_sq * distance) {
+                    const uint32_t cc = m & 0xffu;
+                    for (uint32_t c = 0; c < cc; c++) stack[top++][lane] = fc + c;
+                    continue;
+                }
+            }
+            const float inverse = 1.0f / (1.0f + distance);
+            float magnitude = (float)__float_as_uint(nd.z) * inverse;
+            q_sum += magnitude;
+            magnitude = magnitude * inverse;
+            fx += magnitude * dx;
+            fy += magnitude * dy;
+        }
+        reinterpret_cast<float2 *>(frep)[i] = make_float2(fx, fy);
+        zacc += (double)q_sum;
+    }
+    zacc = wave_sum_f64(zacc);
+    if (lane == 0) dpart[blockIdx.x] = zacc;
+    if (!arrive_last(&scal->ctr[slot], gridDim.x)) return;
+    double z = 0.0;
+    for (int b = lane; b < (int)gridDim.x; b += TRAV_THREADS) z += dpart[b];
+    z = wave_sum_f64(z);
+    if (lane == 0) {
+        red[0] = z;
+        scal->zsum = z;
+        const float zf = (float)z;
+        scal->z = zf;
+        scal->inv_z = 1.0f / zf;
+        scal->ctr[slot] = 0;
+    }
+}
+
+void bh_repulsion(const BhDev &d, float theta, hipStream_t s, int slot) {
+    const int n = d.n;
+    int nb = (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), 256), 1024);
+    hipLaunchKernelGGL(k_bh_codes, dim3(nb), dim3(256), 0, s, d.y, n, d.scal, d.codes, d.idx);
+    BHT_LAUNCH_CHECK();
+    bool alt = radix_sort_pairs<uint64_t>(d.codes, d.idx, d.codes_alt, d.idx_alt, (uint64_t)n, 0,
+                                          64, d.sort_temp, s);
+    const uint64_t *sc = alt ? d.codes_alt : d.codes;
+    const uint32_t *si = alt ? d.idx_alt : d.idx;
+    hipLaunchKernelGGL(k_bh_build, dim3(1), dim3(BUILD_THREADS), 0, s, d.y, n, sc, si, d.scal,
+                       d.nodes);
+    const int tb = (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), TRAV_THREADS), 4096);
+    hipLaunchKernelGGL(k_bh_traverse, dim3(tb), dim3(TRAV_THREADS), 0, s, d.y, n, si, d.nodes,
+                       theta * theta, d.frep, d.scal, d.dpart, slot);
+    BHT_LAUNCH_CHECK();
+}
+
+}  // namespace bht

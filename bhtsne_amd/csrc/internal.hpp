@@ -30,6 +30,7 @@ struct EpochScalars {
     int epoch;             // index of the epoch being executed
     int pad0;
     double psum;           // FP64 sum of P (normalisation)
+    double kl_ref, kl_true;// KL divergence: reference formula, true formula
     unsigned ctr[8];       // arrival counters of the single-pass reductions
 };
 
@@ -105,6 +106,9 @@ void attract_update(const UpdateArgs &a, hipStream_t s);
 void attract_only(int n, int dim, const uint32_t *rows, const uint32_t *cols, const float *vals,
                   const float *y, float *pos, hipStream_t s);
 void scale_values(float *vals, uint64_t nnz, float scale, hipStream_t s);
+// Sparse KL (kl.hip): writes scal->kl_ref / kl_true using scal->inv_z.
+void sparse_kl(const float *y, int n, int dim, const uint32_t *rows, const uint32_t *cols,
+               const float *vals, EpochScalars *scal, double *dpart, hipStream_t s);
 void sum_values_f64(const float *vals, uint64_t nnz, EpochScalars *scal, double *dpart,
                     hipStream_t s);
 
