@@ -38,7 +38,7 @@ class Params(C.Structure):
         ("perplexity", C.c_float),
         ("min_boxes", C.c_uint32),
         ("max_boxes", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 

@@ -19,6 +19,7 @@
 #include "affinity.hpp"
 #include "bh.hpp"
 #include "internal.hpp"
+#include "reorder.hpp"
 
 using namespace bht;
 
@@ -193,6 +194,12 @@ struct bht_job {
     DevBuf codes, codes_alt, bidx, bidx_alt, bsort_temp, bnode, bmeta, brange, bhw2, bcount;
     BhDev bd{};
 
+    // point renumbering: perm[i] = sample index of the engine's point i (identity until the
+    // first reorder); qv[s] = P value at position s of the reference's CSR layout (s < n)
+    DevBuf perm, perm_tmp, iperm, rows2, cols2, vals2, qv, ptmp, lentmp, pscan;
+    bool permuted = false;
+    uint64_t reorders = 0;
+
     // run state
     int method = -1;
     float theta = 0.5f;
@@ -328,6 +335,139 @@ void ensure_bh_workspace(bht_job *j) {
     d.dpart = j->dpart.as<double>();
 }
 
+// ---- point order -----------------------------------------------------------------------------
+
+// A freshly built or injected CSR is in the reference's layout: identity order, and the values
+// at positions [0, n) are what kl_divergence weights rows by (mod.rs:768-770).
+void reset_order(bht_job *j) {
+    j->permuted = false;
+    const uint64_t n = std::max<uint64_t>(j->n, 1);
+    j->qv.ensure(j->ctx, n * 4);
+    BHT_HIP(hipMemsetAsync(j->qv.p, 0, n * 4, j->stream));
+    const uint64_t m = std::min<uint64_t>(j->n, j->nnz);
+    if (m) BHT_HIP(hipMemcpyAsync(j->qv.p, j->vals.p, m * 4, hipMemcpyDeviceToDevice, j->stream));
+}
+
+void ensure_order_buffers(bht_job *j) {
+    bht_ctx *c = j->ctx;
+    const uint64_t n = std::max<uint64_t>(j->n, 1);
+    j->perm.ensure(c, n * 4);
+    j->perm_tmp.ensure(c, n * 4);
+    j->iperm.ensure(c, n * 4);
+    j->lentmp.ensure(c, n * 4);
+    j->pscan.ensure(c, scan_temp_bytes(n));
+    j->ptmp.ensure(c, n * j->dim() * 4);
+}
+
+// host (sample order) -> device (engine order)
+void upload_points(bht_job *j, const float *host, float *dev) {
+    const uint64_t len = j->n * j->dim();
+    if (!j->permuted) {
+        BHT_HIP(hipMemcpyAsync(dev, host, len * 4, hipMemcpyHostToDevice, j->stream));
+        return;
+    }
+    ensure_order_buffers(j);
+    BHT_HIP(hipMemcpyAsync(j->ptmp.p, host, len * 4, hipMemcpyHostToDevice, j->stream));
+    gather_rows(j->ptmp.as<float>(), j->perm.as<uint32_t>(), (int)j->n, (int)j->dim(), dev,
+                j->stream);
+}
+
+// device (engine order) -> host (sample order); synchronises the job's stream
+void download_points(bht_job *j, const float *dev, float *host) {
+    const uint64_t len = j->n * j->dim();
+    const float *src = dev;
+    if (j->permuted) {
+        ensure_order_buffers(j);
+        scatter_rows(dev, j->perm.as<uint32_t>(), (int)j->n, (int)j->dim(), j->ptmp.as<float>(),
+                     j->stream);
+        src = j->ptmp.as<float>();
+    }
+    BHT_HIP(hipMemcpyAsync(host, src, len * 4, hipMemcpyDeviceToHost, j->stream));
+    BHT_HIP(hipStreamSynchronize(j->stream));
+}
+
+// CSR in the reference's layout (sample order, original row order) on the host
+void download_csr(bht_job *j, std::vector<uint32_t> &rows, std::vector<uint32_t> &cols,
+                  std::vector<float> *vals) {
+    rows.resize(j->n + 1);
+    cols.resize(j->nnz);
+    if (vals) vals->resize(j->nnz);
+    const uint32_t *r = j->rows.as<uint32_t>();
+    const uint32_t *c = j->cols.as<uint32_t>();
+    const float *v = j->vals.as<float>();
+    if (j->permuted) {
+        ensure_order_buffers(j);
+        bht_ctx *cx = j->ctx;
+        j->rows2.ensure(cx, (j->n + 1) * 4);
+        j->cols2.ensure(cx, std::max<uint64_t>(j->nnz, 1) * 4);
+        j->vals2.ensure(cx, std::max<uint64_t>(j->nnz, 1) * 4);
+        // sample s is engine point iperm[s]; engine label c is sample perm[c]
+        inverse_perm(j->perm.as<uint32_t>(), (int)j->n, j->iperm.as<uint32_t>(), j->stream);
+        csr_permute(r, c, v, j->iperm.as<uint32_t>(), j->perm.as<uint32_t>(), (int)j->n,
+                    j->lentmp.as<uint32_t>(), j->pscan.p, j->rows2.as<uint32_t>(),
+                    j->cols2.as<uint32_t>(), j->vals2.as<float>(), j->stream);
+        r = j->rows2.as<uint32_t>();
+        c = j->cols2.as<uint32_t>();
+        v = j->vals2.as<float>();
+    }
+    BHT_HIP(hipMemcpyAsync(rows.data(), r, (j->n + 1) * 4, hipMemcpyDeviceToHost, j->stream));
+    if (j->nnz) {
+        BHT_HIP(hipMemcpyAsync(cols.data(), c, j->nnz * 4, hipMemcpyDeviceToHost, j->stream));
+        if (vals)
+            BHT_HIP(hipMemcpyAsync(vals->data(), v, j->nnz * 4, hipMemcpyDeviceToHost, j->stream));
+    }
+    BHT_HIP(hipStreamSynchronize(j->stream));
+}
+
+// Renumber the points in `order` (new point i = current point order[i]).
+void reorder_points(bht_job *j, const uint32_t *order) {
+    ensure_order_buffers(j);
+    bht_ctx *c = j->ctx;
+    hipStream_t s = j->stream;
+    const int n = (int)j->n, D = (int)j->dim();
+    if (!j->permuted) {
+        std::vector<uint32_t> id(j->n);
+        for (uint64_t i = 0; i < j->n; i++) id[i] = (uint32_t)i;
+        BHT_HIP(hipMemcpyAsync(j->perm.p, id.data(), j->n * 4, hipMemcpyHostToDevice, s));
+        BHT_HIP(hipStreamSynchronize(s));
+    }
+    j->rows2.ensure(c, (j->n + 1) * 4);
+    j->cols2.ensure(c, std::max<uint64_t>(j->nnz, 1) * 4);
+    j->vals2.ensure(c, std::max<uint64_t>(j->nnz, 1) * 4);
+    inverse_perm(order, n, j->iperm.as<uint32_t>(), s);
+    csr_permute(j->rows.as<uint32_t>(), j->cols.as<uint32_t>(), j->vals.as<float>(), order,
+                j->iperm.as<uint32_t>(), n, j->lentmp.as<uint32_t>(), j->pscan.p,
+                j->rows2.as<uint32_t>(), j->cols2.as<uint32_t>(), j->vals2.as<float>(), s);
+    std::swap(j->rows.p, j->rows2.p);
+    std::swap(j->rows.bytes, j->rows2.bytes);
+    std::swap(j->cols.p, j->cols2.p);
+    std::swap(j->cols.bytes, j->cols2.bytes);
+    std::swap(j->vals.p, j->vals2.p);
+    std::swap(j->vals.bytes, j->vals2.bytes);
+    // per-point state: embedding into the idle ping-pong buffer, uy and gains through ptmp
+    gather_rows(j->ycur(), order, n, D, j->y[1 - j->cur].as<float>(), s);
+    j->cur = 1 - j->cur;
+    gather_rows(j->uy.as<float>(), order, n, D, j->ptmp.as<float>(), s);
+    std::swap(j->uy.p, j->ptmp.p);
+    std::swap(j->uy.bytes, j->ptmp.bytes);
+    j->ptmp.ensure(c, j->n * D * 4);
+    gather_rows(j->gains.as<float>(), order, n, D, j->ptmp.as<float>(), s);
+    std::swap(j->gains.p, j->ptmp.p);
+    std::swap(j->gains.bytes, j->ptmp.bytes);
+    gather_u32(j->perm.as<uint32_t>(), order, n, j->perm_tmp.as<uint32_t>(), s);
+    std::swap(j->perm.p, j->perm_tmp.p);
+    std::swap(j->perm.bytes, j->perm_tmp.bytes);
+    j->permuted = true;
+    j->reorders++;
+}
+
+// Epochs after which the points are renumbered (large jobs only, where y outgrows an L2).
+bool reorder_due(const bht_job *j, uint64_t epoch) {
+    if ((j->params.flags & BHT_FLAG_NO_REORDER) || j->n < 32768) return false;
+    if (epoch == 4 || epoch == 14 || epoch == 39) return true;
+    return epoch % 100 == 99;
+}
+
 // Builds P (CSR) from per-row neighbour indices / distances already on the device.
 void affinities_from_knn(bht_job *j, uint32_t *d_idx, float *d_dist, uint32_t k) {
     bht_ctx *c = j->ctx;
@@ -381,6 +521,7 @@ void affinities_from_knn(bht_job *j, uint32_t *d_idx, float *d_dist, uint32_t k)
     j->have_p = true;
     j->cached_perplexity = j->params.perplexity;  // lib.rs:1123-1124
     j->have_cached_perplexity = true;
+    reset_order(j);
 }
 
 void build_affinities_from_data(bht_job *j) {
@@ -412,7 +553,11 @@ void prepare_run(bht_job *j, int method, float theta, bool cached) {
     const float exag = j->params.early_exaggeration;
     const float scale = exag / ((float)psum + FLT_EPSILON);
     scale_values(j->vals.as<float>(), j->nnz, scale, s);
-    if (j->params.stop_lying_epoch == 0) scale_values(j->vals.as<float>(), j->nnz, 1.0f / exag, s);
+    scale_values(j->qv.as<float>(), std::min<uint64_t>(j->n, j->nnz), scale, s);
+    if (j->params.stop_lying_epoch == 0) {
+        scale_values(j->vals.as<float>(), j->nnz, 1.0f / exag, s);
+        scale_values(j->qv.as<float>(), std::min<uint64_t>(j->n, j->nnz), 1.0f / exag, s);
+    }
     // seed
     std::vector<float> host(len);
     if (j->have_init) {
@@ -427,7 +572,7 @@ void prepare_run(bht_job *j, int method, float theta, bool cached) {
         for (auto &v : host) v = (float)nd(rng);
     }
     j->cur = 0;
-    BHT_HIP(hipMemcpyAsync(j->ycur(), host.data(), len * 4, hipMemcpyHostToDevice, s));
+    upload_points(j, host.data(), j->ycur());
     BHT_HIP(hipMemsetAsync(j->uy.p, 0, len * 4, s));
     std::vector<float> ones(len, 1.0f);
     BHT_HIP(hipMemcpyAsync(j->gains.p, ones.data(), len * 4, hipMemcpyHostToDevice, s));
@@ -446,22 +591,24 @@ float resolve_learning_rate(const bht_job *j) {  // lib.rs:652-659
     return std::max(a, 50.0f);
 }
 
-void repulsion(bht_job *j) {
+// Repulsive forces and Z of the current embedding; returns the point indices in the spatial
+// order the repulsion sorted them into (device), or null.
+const uint32_t *repulsion(bht_job *j) {
     FitsneDev &d = j->fd;
     d.y = j->ycur();
     d.frep = j->frep.as<float>();
     if (j->method == FIT_FITSNE) {
         fitsne_center_plan(d, j->stream);
-        fitsne_repulsion(d, j->stream);
-    } else {
-        // recentre + per-axis bounds (arena.rs:65-94), then the arena and its traversal
-        center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
-                    j->dpart.as<double>(), j->stream);
-        BhDev &b = j->bd;
-        b.y = j->ycur();
-        b.frep = j->frep.as<float>();
-        if (j->n > 0) bh_repulsion(b, j->theta, j->stream, CTR_BH);
+        return fitsne_repulsion(d, j->stream);
     }
+    // recentre + per-axis bounds (arena.rs:65-94), then the arena and its traversal
+    center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
+                j->dpart.as<double>(), j->stream);
+    BhDev &b = j->bd;
+    b.y = j->ycur();
+    b.frep = j->frep.as<float>();
+    if (j->n > 0) return bh_repulsion(b, j->theta, j->stream, CTR_BH);
+    return nullptr;
 }
 
 void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
@@ -471,11 +618,6 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     UpdateArgs a{};
     a.n = (int)j->n;
     a.dim = (int)j->dim();
-    a.rows = j->rows.as<uint32_t>();
-    a.cols = j->cols.as<uint32_t>();
-    a.vals = j->vals.as<float>();
-    a.uy = j->uy.as<float>();
-    a.gains = j->gains.as<float>();
     a.frep = j->frep.as<float>();
     a.learning_rate = resolve_learning_rate(j);
     a.momentum = j->run_momentum;
@@ -489,22 +631,31 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     a.dpart = j->dpart.as<double>();
     for (uint64_t e = 0; e < count; e++) {
         j->mark();
-        repulsion(j);
+        const uint32_t *order = repulsion(j);
         j->mark();
+        a.rows = j->rows.as<uint32_t>();
+        a.cols = j->cols.as<uint32_t>();
+        a.vals = j->vals.as<float>();
+        a.uy = j->uy.as<float>();
+        a.gains = j->gains.as<float>();
         a.y = j->ycur();
         a.y_next = j->y[1 - j->cur].as<float>();
         attract_update(a, j->stream);
         j->mark();
         j->cur = 1 - j->cur;
         const uint64_t epoch = j->epoch++;
-        if (epoch == j->params.stop_lying_epoch && epoch != 0) j->stop_lying_fired = true;
+        if (epoch == j->params.stop_lying_epoch && epoch != 0) {
+            j->stop_lying_fired = true;
+            // the values at the reference's positions [0, n) follow the same rescale
+            scale_values(j->qv.as<float>(), std::min<uint64_t>(j->n, j->nnz),
+                         1.0f / j->params.early_exaggeration, j->stream);
+        }
         if (epoch == j->params.momentum_switch_epoch) j->params.momentum = j->params.final_momentum;
+        if (order && reorder_due(j, epoch)) reorder_points(j, order);
         if (cb) {
             center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
                         j->dpart.as<double>(), j->stream);
-            BHT_HIP(hipMemcpyAsync(snap.data(), j->ycur(), len * 4, hipMemcpyDeviceToHost,
-                                   j->stream));
-            BHT_HIP(hipStreamSynchronize(j->stream));
+            download_points(j, j->ycur(), snap.data());
             cb(epoch, snap.data(), len, user);
         }
     }
@@ -567,9 +718,8 @@ void fit_with_neighbors(bht_job *j, int method, float theta, const uint64_t *idx
     // cached_affinities_match_neighbors (lib.rs:1201-1220)
     if (j->have_p && j->have_cached_perplexity &&
         j->cached_perplexity == j->params.perplexity) {
-        std::vector<uint32_t> hrows(j->n + 1), hcols(j->nnz);
-        BHT_HIP(hipMemcpy(hrows.data(), j->rows.p, (j->n + 1) * 4, hipMemcpyDeviceToHost));
-        BHT_HIP(hipMemcpy(hcols.data(), j->cols.p, j->nnz * 4, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> hrows, hcols;
+        download_csr(j, hrows, hcols, nullptr);
         bool match = true;
         for (uint64_t i = 0; match && i < j->n; i++) {
             if (hrows[i + 1] - hrows[i] != k) { match = false; break; }
@@ -760,8 +910,8 @@ bht_status bht_job_get_embedding(const bht_job *job, float *out, uint64_t len) {
     return guarded([&] {
         if (len != job->n * job->dim()) throw Error(ERR_INVALID, "length must be n * d_out");
         if (!job->y[job->cur].p) throw Error(ERR_STATE, "no embedding yet");
-        BHT_HIP(hipStreamSynchronize(job->stream));
-        BHT_HIP(hipMemcpy(out, job->ycur(), len * 4, hipMemcpyDeviceToHost));
+        bht_job *j = const_cast<bht_job *>(job);
+        download_points(j, j->ycur(), out);
     });
 }
 
@@ -776,13 +926,13 @@ bht_status bht_job_get_affinities(const bht_job *job, uint64_t *rows, uint32_t *
             return;
         }
         if (*nnz < job->nnz) throw Error(ERR_INVALID, "output arrays too small");
-        BHT_HIP(hipStreamSynchronize(job->stream));
-        std::vector<uint32_t> r(job->n + 1);
-        BHT_HIP(hipMemcpy(r.data(), job->rows.p, r.size() * 4, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> r, c;
+        std::vector<float> v;
+        download_csr(const_cast<bht_job *>(job), r, c, &v);
         for (size_t i = 0; i < r.size(); i++) rows[i] = r[i];
-        if (cols) BHT_HIP(hipMemcpy(cols, job->cols.p, job->nnz * 4, hipMemcpyDeviceToHost));
+        if (cols && job->nnz) std::memcpy(cols, c.data(), job->nnz * 4);
         if (vals) {
-            BHT_HIP(hipMemcpy(vals, job->vals.p, job->nnz * 4, hipMemcpyDeviceToHost));
+            if (job->nnz) std::memcpy(vals, v.data(), job->nnz * 4);
             if (!job->stop_lying_fired) {  // lib.rs:1023-1029
                 const float scale = 1.0f / job->params.early_exaggeration;
                 for (uint64_t i = 0; i < job->nnz; i++) vals[i] = vals[i] * scale;
@@ -813,6 +963,8 @@ bht_status bht_job_set_affinities(bht_job *job, const uint64_t *rows, const uint
         job->cached_perplexity = perplexity;
         job->have_cached_perplexity = true;
         job->have_p = n_rows_plus_one == job->n + 1 && n_rows_plus_one > 1;
+        if (job->have_p) reset_order(job);
+        BHT_HIP(hipStreamSynchronize(job->stream));
     });
 }
 
@@ -856,15 +1008,10 @@ bht_status bht_job_forces(bht_job *job, float *attractive, float *repulsive, flo
         attract_only((int)job->n, (int)job->dim(), job->rows.as<uint32_t>(),
                      job->cols.as<uint32_t>(), job->vals.as<float>(), job->ycur(),
                      job->pos.as<float>(), job->stream);
-        const uint64_t len = job->n * job->dim();
         EpochScalars sc;
         BHT_HIP(hipMemcpyAsync(&sc, job->scal.p, sizeof(sc), hipMemcpyDeviceToHost, job->stream));
-        if (attractive)
-            BHT_HIP(hipMemcpyAsync(attractive, job->pos.p, len * 4, hipMemcpyDeviceToHost,
-                                   job->stream));
-        if (repulsive)
-            BHT_HIP(hipMemcpyAsync(repulsive, job->frep.p, len * 4, hipMemcpyDeviceToHost,
-                                   job->stream));
+        if (attractive) download_points(job, job->pos.as<float>(), attractive);
+        if (repulsive) download_points(job, job->frep.as<float>(), repulsive);
         BHT_HIP(hipStreamSynchronize(job->stream));
         if (inverse_z) *inverse_z = sc.inv_z;
     });
@@ -874,13 +1021,12 @@ bht_status bht_job_get_state(const bht_job *job, float *y, float *uy, float *gai
     if (!job) return fail(BHT_ERR_INVALID, "null argument");
     return guarded([&] {
         if (!job->uy.p) throw Error(ERR_STATE, "no optimiser state");
-        const uint64_t len = job->n * job->dim();
-        center_only(job->ycur(), (int)job->n, (int)job->dim(), job->scal.as<EpochScalars>(),
-                    job->dpart.as<double>(), job->stream);
-        BHT_HIP(hipStreamSynchronize(job->stream));
-        if (y) BHT_HIP(hipMemcpy(y, job->ycur(), len * 4, hipMemcpyDeviceToHost));
-        if (uy) BHT_HIP(hipMemcpy(uy, job->uy.p, len * 4, hipMemcpyDeviceToHost));
-        if (gains) BHT_HIP(hipMemcpy(gains, job->gains.p, len * 4, hipMemcpyDeviceToHost));
+        bht_job *j = const_cast<bht_job *>(job);
+        center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
+                    j->dpart.as<double>(), j->stream);
+        if (y) download_points(j, j->ycur(), y);
+        if (uy) download_points(j, j->uy.as<float>(), uy);
+        if (gains) download_points(j, j->gains.as<float>(), gains);
     });
 }
 
@@ -888,13 +1034,12 @@ bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, cons
     if (!job) return fail(BHT_ERR_INVALID, "null argument");
     return guarded([&] {
         if (!job->prepared) throw Error(ERR_STATE, "bht_job_prepare must precede set_state");
-        const uint64_t len = job->n * job->dim();
-        BHT_HIP(hipStreamSynchronize(job->stream));
-        if (y) BHT_HIP(hipMemcpy(job->ycur(), y, len * 4, hipMemcpyHostToDevice));
-        if (uy) BHT_HIP(hipMemcpy(job->uy.p, uy, len * 4, hipMemcpyHostToDevice));
-        if (gains) BHT_HIP(hipMemcpy(job->gains.p, gains, len * 4, hipMemcpyHostToDevice));
+        if (y) upload_points(job, y, job->ycur());
+        if (uy) upload_points(job, uy, job->uy.as<float>());
+        if (gains) upload_points(job, gains, job->gains.as<float>());
         EpochScalars *sc = job->scal.as<EpochScalars>();
-        BHT_HIP(hipMemset(sc->mean, 0, sizeof(sc->mean)));
+        BHT_HIP(hipMemsetAsync(sc->mean, 0, sizeof(sc->mean), job->stream));
+        BHT_HIP(hipStreamSynchronize(job->stream));
     });
 }
 
@@ -958,8 +1103,9 @@ bht_status bht_job_kl_divergence(bht_job *job, double *reference_kl, double *tru
         repulsion(job);
         EpochScalars *sc = job->scal.as<EpochScalars>();
         sparse_kl(job->ycur(), (int)job->n, (int)job->dim(), job->rows.as<uint32_t>(),
-                  job->cols.as<uint32_t>(), job->vals.as<float>(), sc, job->dpart.as<double>(),
-                  job->stream);
+                  job->cols.as<uint32_t>(), job->vals.as<float>(), job->qv.as<float>(),
+                  job->permuted ? job->perm.as<uint32_t>() : nullptr, sc,
+                  job->dpart.as<double>(), job->stream);
         EpochScalars h;
         BHT_HIP(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, job->stream));
         BHT_HIP(hipStreamSynchronize(job->stream));

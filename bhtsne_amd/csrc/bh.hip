@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const float *y, in
     }
 }
 
-void bh_repulsion(const BhDev &d, float theta, hipStream_t s, int slot) {
+const uint32_t *bh_repulsion(const BhDev &d, float theta, hipStream_t s, int slot) {
     const int n = d.n;
     int nb = (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), 256), 1024);
     hipLaunchKernelGGL(k_bh_codes, dim3(nb), dim3(256), 0, s, d.y, n, d.scal, d.codes, d.idx);
@@ -282,6 +282,7 @@ void bh_repulsion(const BhDev &d, float theta, hipStream_t s, int slot) {
     hipLaunchKernelGGL(k_bh_traverse, dim3(tb), dim3(TRAV_THREADS), 0, s, d.y, n, si, d.nodes,
                        theta * theta, d.frep, d.scal, d.dpart, slot);
     BHT_LAUNCH_CHECK();
+    return si;  // point indices in Morton order
 }
 
 }  // namespace bht

@@ -27,7 +27,7 @@ struct BhDev {
 };
 
 // Requires scal->amin / amax of the current y (the centring kernel writes them).  Writes frep
-// (non-edge forces) and scal->zsum / z / inv_z.
-void bh_repulsion(const BhDev &d, float theta, hipStream_t s, int slot);
+// (non-edge forces) and scal->zsum / z / inv_z.  Returns the point indices in Morton order.
+const uint32_t *bh_repulsion(const BhDev &d, float theta, hipStream_t s, int slot);
 
 }  // namespace bht

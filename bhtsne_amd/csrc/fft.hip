@@ -233,10 +233,15 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rows_fwd(const EpochScalars *sc
     const int L = p.len, nodes = p.nodes;
     const float2 *tw = tw_all + p.tw_off;
     const float2 *g2 = reinterpret_cast<const float2 *>(grid);
+    const float inv_q = 1.0f / p.qscale;
     for (int line = blockIdx.x; line < 2 * nodes; line += gridDim.x) {
         const int g = line / nodes, r = line % nodes;
-        for (int c = threadIdx.x; c < L; c += FFT_THREADS)
-            buf[c] = c < nodes ? g2[2 * ((size_t)r * nodes + c) + g] : make_float2(0.f, 0.f);
+        // (t0, t1 / q) and (t2 / q, t3 / q^2)
+        const float sa = g == 0 ? 1.0f : inv_q, sb = g == 0 ? inv_q : inv_q * inv_q;
+        for (int c = threadIdx.x; c < L; c += FFT_THREADS) {
+            float2 v = c < nodes ? g2[2 * ((size_t)r * nodes + c) + g] : make_float2(0.f, 0.f);
+            buf[c] = make_float2(v.x * sa, v.y * sb);
+        }
         __syncthreads();
         fft_lds<false>(buf, L, tw);
         float2 *out = spec + g * gstride + (size_t)r * L;
@@ -286,14 +291,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rows_inv(const EpochScalars *sc
     const int L = p.len, nodes = p.nodes;
     const float2 *tw = tw_all + p.tw_off;
     float2 *g2 = reinterpret_cast<float2 *>(grid);
+    const float q = p.qscale;
     for (int line = blockIdx.x; line < 2 * nodes; line += gridDim.x) {
         const int g = line / nodes, r = line % nodes;
+        const float sa = g == 0 ? 1.0f : q, sb = g == 0 ? q : q * q;
         const float2 *in = spec + g * gstride + (size_t)r * L;
         for (int c = threadIdx.x; c < L; c += FFT_THREADS) buf[c] = in[c];
         __syncthreads();
         fft_lds<true>(buf, L, tw);
         for (int c = threadIdx.x; c < nodes; c += FFT_THREADS)
-            g2[2 * ((size_t)r * nodes + c) + g] = buf[c];
+            g2[2 * ((size_t)r * nodes + c) + g] = make_float2(buf[c].x * sa, buf[c].y * sb);
         __syncthreads();
     }
 }
@@ -306,11 +313,12 @@ __global__ void __launch_bounds__(FFT_THREADS) k_line_conv_1d(const EpochScalars
     const GridPlan p = scal->plan;
     const int L = p.len, nodes = p.nodes;
     const float2 *tw = tw_all + p.tw_off;
-    const int g = blockIdx.x;  // 0: (t0, t1), 1: (t2, 0)
+    const int g = blockIdx.x;  // 0: (t0, t1 / q), 1: (t2 / q^2, 0)
     if (g > 1) return;
+    const float q = p.qscale, inv_q = 1.0f / q;
     for (int c = threadIdx.x; c < L; c += FFT_THREADS) {
         float4 v = c < nodes ? grid[c] : make_float4(0.f, 0.f, 0.f, 0.f);
-        buf[c] = g == 0 ? make_float2(v.x, v.y) : make_float2(v.z, 0.0f);
+        buf[c] = g == 0 ? make_float2(v.x, v.y * inv_q) : make_float2(v.z * inv_q * inv_q, 0.0f);
     }
     __syncthreads();
     fft_lds<false>(buf, L, tw);
@@ -328,9 +336,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_line_conv_1d(const EpochScalars
     for (int c = threadIdx.x; c < nodes; c += FFT_THREADS) {
         if (g == 0) {
             grid[c].x = buf[c].x;
-            grid[c].y = buf[c].y;
+            grid[c].y = buf[c].y * q;
         } else {
-            grid[c].w = buf[c].x;
+            grid[c].w = buf[c].x * q * q;
         }
     }
 }

@@ -145,7 +145,11 @@ __global__ void __launch_bounds__(PT_THREADS) k_center_plan(float *y, int n, Epo
     p.box_width = extent / (float)boxes;
     p.spacing = p.box_width / (float)IP;
     p.inv_box_width = 1.0f / p.box_width;
-    p.pad = 0;
+    // the charges y and |y|^2 are divided by qscale and qscale^2 inside the FFT so that both
+    // halves of each packed complex grid have similar magnitude (powers of two: exact)
+    int e2 = 0;
+    frexpf(fmaxf(fmaxf(fabsf(lo), fabsf(hi)), 1.0f), &e2);
+    p.qscale = ldexpf(1.0f, e2);
     scal->plan = p;
 }
 
@@ -326,6 +330,9 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const float *y, const uin
     }
 }
 
+// One wave per chunk; the chunk where a spanning run starts (its owner) finds the run's last chunk
+// by binary search of the sorted keys and sums the chunk partials with the lanes (strided, then a
+// fixed butterfly), so a run of any length costs O(length / 64 / 64) steps.
 template <int D>
 __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const uint32_t *keys, int n,
                                                              const EpochScalars *scal,
@@ -335,22 +342,37 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const uint32_t *key
     const GridPlan p = scal->plan;
     const float *part_first = part;
     const float *part_last = part + (size_t)nchunks * NV;
-    for (int c = blockIdx.x * PT_THREADS + threadIdx.x; c < nchunks;
-         c += gridDim.x * PT_THREADS) {
-        int last_s = min(c * 64 + 63, n - 1);
-        uint32_t key = keys[last_s];
+    const int lane = threadIdx.x & 63;
+    const int wave0 = (blockIdx.x * PT_THREADS + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * PT_THREADS) >> 6;
+    for (int c = wave0; c < nchunks; c += nwaves) {
+        const int last_s = min(c * 64 + 63, n - 1);
+        const uint32_t key = keys[last_s];
         if (!(last_s + 1 < n && keys[last_s + 1] == key)) continue;  // run ends in this chunk
-        int first_s = c * 64;
-        bool whole = keys[first_s] == key;
+        const int first_s = c * 64;
+        const bool whole = keys[first_s] == key;
         if (whole && first_s > 0 && keys[first_s - 1] == key) continue;  // not the run's owner
-        float tot[NV];
-        for (int t = 0; t < NV; t++) tot[t] = part_last[(size_t)c * NV + t];
-        for (int j = c + 1; j < nchunks; j++) {
-            for (int t = 0; t < NV; t++) tot[t] += part_first[(size_t)j * NV + t];
-            int lj = min(j * 64 + 63, n - 1);
-            if (!(keys[lj] == key && lj + 1 < n && keys[lj + 1] == key)) break;
+        // first sorted position past the run
+        int lo = last_s + 1, hi = n;
+        while (lo < hi) {
+            int mid = lo + (hi - lo) / 2;
+            if (keys[mid] <= key) lo = mid + 1;
+            else hi = mid;
         }
-        write_box<D>(grid, p, key, tot);
+        const int c_end = (lo - 1) / 64;  // chunk holding the run's last point
+        float tot[NV];
+#pragma unroll
+        for (int t = 0; t < NV; t++) tot[t] = 0.0f;
+        for (int j = c + 1 + lane; j <= c_end; j += 64)
+#pragma unroll
+            for (int t = 0; t < NV; t++) tot[t] += part_first[(size_t)j * NV + t];
+#pragma unroll
+        for (int t = 0; t < NV; t++) tot[t] = wave_sum_f32(tot[t]);
+        if (lane == 0) {
+#pragma unroll
+            for (int t = 0; t < NV; t++) tot[t] = part_last[(size_t)c * NV + t] + tot[t];
+            write_box<D>(grid, p, key, tot);
+        }
     }
 }
 
@@ -434,11 +456,11 @@ __global__ void __launch_bounds__(PT_THREADS) k_gather(const float *y, int n,
 
 // ---- driver ----------------------------------------------------------------------------------
 
-void fitsne_repulsion(FitsneDev &d, hipStream_t s) {
+const uint32_t *fitsne_repulsion(FitsneDev &d, hipStream_t s) {
     const int nb = point_blocks(d.n);
     const int nchunks = (int)ceil_div(d.n, 64);
     const int spread_blocks = (int)std::min<uint64_t>(ceil_div(nchunks, PT_THREADS / 64), 2048);
-    const int fix_blocks = (int)std::min<uint64_t>(ceil_div(nchunks, PT_THREADS), 1024);
+    const int fix_blocks = spread_blocks;
     const int zero_blocks = 1024;
     if (d.dim == 2)
         hipLaunchKernelGGL(k_box_keys<2>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
@@ -474,6 +496,7 @@ void fitsne_repulsion(FitsneDev &d, hipStream_t s) {
         hipLaunchKernelGGL(k_gather<1>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
                            d.grid, d.frep, d.dpart);
     BHT_LAUNCH_CHECK();
+    return sv;  // point indices in box order: a spatial order of the current embedding
 }
 
 }  // namespace bht

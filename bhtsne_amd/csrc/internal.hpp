@@ -18,7 +18,7 @@ struct GridPlan {
     int nodes;             // 3 * boxes
     int len;               // FFT length per axis (7-smooth >= 2 * nodes - 1)
     int tw_off;            // offset of this length's twiddle table
-    int pad;
+    float qscale;          // power of two >= max |coordinate|: balances the packed FFT pairs
 };
 
 struct EpochScalars {
@@ -83,7 +83,8 @@ bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, 
 int point_blocks(int n);  // grid size of the point-parallel kernels
 void fitsne_center_plan(const FitsneDev &d, hipStream_t s);
 void center_only(float *y, int n, int dim, EpochScalars *scal, double *dpart, hipStream_t s);
-void fitsne_repulsion(FitsneDev &d, hipStream_t s);  // after center_plan: frep, Z
+// after center_plan: frep, Z; returns the point indices in box order (device)
+const uint32_t *fitsne_repulsion(FitsneDev &d, hipStream_t s);
 void fft_kernel_spectrum(const FitsneDev &d, hipStream_t s);
 void fft_convolve(const FitsneDev &d, hipStream_t s);
 
@@ -108,7 +109,8 @@ void attract_only(int n, int dim, const uint32_t *rows, const uint32_t *cols, co
 void scale_values(float *vals, uint64_t nnz, float scale, hipStream_t s);
 // Sparse KL (kl.hip): writes scal->kl_ref / kl_true using scal->inv_z.
 void sparse_kl(const float *y, int n, int dim, const uint32_t *rows, const uint32_t *cols,
-               const float *vals, EpochScalars *scal, double *dpart, hipStream_t s);
+               const float *vals, const float *qv, const uint32_t *perm, EpochScalars *scal,
+               double *dpart, hipStream_t s);
 void sum_values_f64(const float *vals, uint64_t nnz, EpochScalars *scal, double *dpart,
                     hipStream_t s);
 

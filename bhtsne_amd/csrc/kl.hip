@@ -10,15 +10,18 @@
 
 namespace bht {
 
+// qv[s] is the value at CSR position s of the reference's layout (s < n); perm maps the engine's
+// point order to sample indices (null when the points are in input order).
 template <int D>
 __global__ void __launch_bounds__(256) k_sparse_kl(const float *y, int n, const uint32_t *rows,
                                                    const uint32_t *cols, const float *vals,
+                                                   const float *qv, const uint32_t *perm,
                                                    EpochScalars *scal, double *dpart) {
     __shared__ double red[4];
     const float inv_z = scal->inv_z;
     double ref = 0.0, tru = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const float pi = vals[i];
+        const float pi = qv[perm ? perm[i] : (uint32_t)i];
         float a[2] = {y[(size_t)i * D], D == 2 ? y[(size_t)i * D + 1] : 0.0f};
         float cost = 0.0f;
         for (uint32_t e = rows[i]; e < rows[i + 1]; e++) {
@@ -60,14 +63,15 @@ __global__ void __launch_bounds__(256) k_sparse_kl(const float *y, int n, const 
 }
 
 void sparse_kl(const float *y, int n, int dim, const uint32_t *rows, const uint32_t *cols,
-               const float *vals, EpochScalars *scal, double *dpart, hipStream_t s) {
+               const float *vals, const float *qv, const uint32_t *perm, EpochScalars *scal,
+               double *dpart, hipStream_t s) {
     int nb = (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), 256), 2048);
     if (dim == 2)
-        hipLaunchKernelGGL(k_sparse_kl<2>, dim3(nb), dim3(256), 0, s, y, n, rows, cols, vals,
-                           scal, dpart);
+        hipLaunchKernelGGL(k_sparse_kl<2>, dim3(nb), dim3(256), 0, s, y, n, rows, cols, vals, qv,
+                           perm, scal, dpart);
     else
-        hipLaunchKernelGGL(k_sparse_kl<1>, dim3(nb), dim3(256), 0, s, y, n, rows, cols, vals,
-                           scal, dpart);
+        hipLaunchKernelGGL(k_sparse_kl<1>, dim3(nb), dim3(256), 0, s, y, n, rows, cols, vals, qv,
+                           perm, scal, dpart);
     BHT_LAUNCH_CHECK();
 }
 

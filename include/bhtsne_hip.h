@@ -70,8 +70,13 @@ typedef struct bht_params {
     float perplexity;               /* 20 */
     uint32_t min_boxes;             /* FIt-SNE boxes per axis lower bound; 10 is the reference's */
     uint32_t max_boxes;             /* FIt-SNE grid capacity per axis, <= 1000 (the reference's) */
-    uint32_t reserved;
+    uint32_t flags;                 /* BHT_FLAG_* */
 } bht_params;
+
+/* Large jobs renumber their points spatially a few times per run so that the attraction's
+ * gathers stay in cache (results are unchanged up to summation order; outputs are always in
+ * sample order).  This flag keeps the input order. */
+#define BHT_FLAG_NO_REORDER 1u
 
 void bht_params_default(bht_params *p);
 

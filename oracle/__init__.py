@@ -62,6 +62,8 @@ def lib():
         "oracle_lagrange_weights": (None, [C.c_float, f32p]),
         "oracle_fitsne_repulsion": (None, [f32p, C.c_int64, C.c_int, C.c_int, f32p,
                                            C.POINTER(C.c_float), f64p]),
+        "oracle_fitsne_repulsion_f64": (None, [f32p, C.c_int64, C.c_int, C.c_int, f64p,
+                                               C.POINTER(C.c_double)]),
         "oracle_morton_encode2": (C.c_uint64, [C.c_uint32, C.c_uint32]),
         "oracle_quantize": (C.c_uint32, [C.c_float, C.c_float, C.c_float, C.c_uint32]),
         "oracle_bh_build": (C.c_int64, [f32p, C.c_int64, f32p, u32p, u32p, u8p, u8p, f32p]),
@@ -190,6 +192,16 @@ def fitsne_repulsion(y, min_boxes=10):
     info = np.zeros(4, np.float64)
     lib().oracle_fitsne_repulsion(y, n, D, int(min_boxes), neg, C.byref(z), info)
     return neg, z.value, dict(boxes=int(info[0]), fft_len=int(info[1]), min=info[2], max=info[3])
+
+
+def fitsne_repulsion_f64(y, min_boxes=10):
+    """The interpolated repulsion and Z in T = f64 throughout (same grid plan as the f32 path)."""
+    y = _f32(y)
+    n, D = y.shape
+    neg = np.zeros((n, D), np.float64)
+    z = C.c_double(0.0)
+    lib().oracle_fitsne_repulsion_f64(y, n, D, int(min_boxes), neg, C.byref(z))
+    return neg, z.value
 
 
 def next_smooth(n):

@@ -335,10 +335,23 @@ EXPORT int64_t oracle_symmetrize(const uint32_t *cols_in, const float *vals_in, 
     return (int64_t)total;
 }
 
-/* P <- P * exag / (sum(P) + eps)  (src/tsne/mod.rs:284-293); sequential sum.  Returns scale. */
+/* A rayon `par_iter().sum()` splits the slice in halves down to small pieces, sums each piece
+ * left to right and adds the halves back: a pairwise tree.  The split points depend on the pool
+ * size; this uses halving down to 256 elements.  (A one-thread pool would add left to right,
+ * whose error at 10^6 terms is ~1e-5 relative, far from any real run of the reference.) */
+static float pairwise_sum_f32(const float *v, int64_t n, int64_t stride) {
+    if (n <= 256) {
+        float s = 0.0f;
+        for (int64_t i = 0; i < n; i++) s = s + v[i * stride];
+        return s;
+    }
+    int64_t h = n / 2;
+    return pairwise_sum_f32(v, h, stride) + pairwise_sum_f32(v + h * stride, n - h, stride);
+}
+
+/* P <- P * exag / (sum(P) + eps)  (src/tsne/mod.rs:284-293).  Returns scale. */
 EXPORT float oracle_normalize_p(float *vals, int64_t nnz, float exag) {
-    float sum = 0.0f;
-    for (int64_t i = 0; i < nnz; i++) sum = sum + vals[i];
+    float sum = pairwise_sum_f32(vals, nnz, 1);
     float scale = exag / (sum + FLT_EPSILON);
     for (int64_t i = 0; i < nnz; i++) vals[i] *= scale;
     return scale;
@@ -404,11 +417,11 @@ EXPORT void oracle_gradient_step(float *y, const float *pos, const float *neg, f
     }
 }
 
-/* zero_mean with a sequential sum (one-thread pool).  Writes the mean used into mean_out. */
+/* zero_mean (mod.rs:534-568): parallel fold/reduce of the per-axis sums, pairwise here.  Writes
+ * the mean used into mean_out. */
 EXPORT void oracle_zero_mean(float *y, int64_t n, int D, float *mean_out) {
     float m[8] = {0};
-    for (int64_t i = 0; i < n; i++)
-        for (int a = 0; a < D; a++) m[a] += y[(size_t)i * D + a];
+    for (int a = 0; a < D; a++) m[a] = pairwise_sum_f32(y + a, n, D);
     float nf = (float)n;
     for (int a = 0; a < D; a++) m[a] /= nf;
     for (int64_t i = 0; i < n; i++)
@@ -866,8 +879,8 @@ EXPORT void oracle_fitsne_repulsion(const float *y, int64_t n, int D, int min_bo
         }
         zterm[i] = (1.0f + norm_sq) * density - 2.0f * cross + sqn;
     }
-    float zs = 0.0f;
-    for (int64_t i = 0; i < n; i++) zs = zs + zterm[i];
+    /* par_iter ... .sum() over points (interpolation.rs:309-328): pairwise */
+    float zs = pairwise_sum_f32(zterm, n, 1);
     *z_out = zs - (float)n;
 
     free(zterm);
@@ -875,6 +888,162 @@ EXPORT void oracle_fitsne_repulsion(const float *y, int64_t n, int D, int min_bo
     free(wspec);
     free(kspec);
     free(grid);
+    plan_free(&pl);
+    free(w);
+    free(box);
+}
+
+/* The same interpolated repulsion in T = f64 (the reference's f64 instantiation,
+ * interpolation.rs:100-332 with DBL constants), every grid and transform in double.  This is the
+ * interpolation scheme's answer without f32 rounding: the yardstick for the f32 noise of any
+ * implementation (the reference's and this engine's alike).  The box plan is taken from the f32
+ * coordinates so that both use the same grid. */
+EXPORT void oracle_fitsne_repulsion_f64(const float *yf, int64_t n, int D, int min_boxes,
+                                        double *neg, double *z_out) {
+    if (n == 0) {
+        *z_out = 0.0;
+        return;
+    }
+    float mnf = INFINITY, mxf = -INFINITY;
+    for (int64_t i = 0; i < n * D; i++) {
+        mnf = fminf(mnf, yf[i]);
+        mxf = fmaxf(mxf, yf[i]);
+    }
+    float extf = fmaxf(mxf - mnf, FLT_MIN);
+    int boxes = oracle_box_count(extf, min_boxes);
+    int nodes = boxes * IP;
+    int len = (int)oracle_next_smooth(2 * (int64_t)nodes - 1);
+    const double mn = mnf, extent = fmax((double)mxf - (double)mnf, DBL_MIN);
+    const double box_width = extent / boxes, spacing = box_width / IP;
+    size_t grid_len = D == 2 ? (size_t)len * len : (size_t)len;
+    int terms = D + 2;
+    double nodes_pos[IP], inv_denom[IP];
+    for (int k = 0; k < IP; k++) nodes_pos[k] = (k + 0.5) / IP;
+    for (int k = 0; k < IP; k++) {
+        double d = 1.0;
+        for (int m = 0; m < IP; m++)
+            if (m != k) d *= nodes_pos[k] - nodes_pos[m];
+        inv_denom[k] = 1.0 / d;
+    }
+    int64_t *box = (int64_t *)malloc(sizeof(int64_t) * (size_t)n * D);
+    double *w = (double *)malloc(sizeof(double) * (size_t)n * D * IP);
+    for (int64_t i = 0; i < n; i++)
+        for (int a = 0; a < D; a++) {
+            double off = ((double)yf[i * D + a] - mn) / box_width;
+            double b = floor(off);
+            int64_t bid = b > 0.0 ? (int64_t)b : 0;
+            if (bid > boxes - 1) bid = boxes - 1;
+            box[i * D + a] = bid;
+            double local = off - (double)bid;
+            for (int k = 0; k < IP; k++) {
+                double num = 1.0;
+                for (int m = 0; m < IP; m++)
+                    if (m != k) num *= local - nodes_pos[m];
+                w[(i * D + a) * IP + k] = num * inv_denom[k];
+            }
+        }
+    /* kernel on the grid and its full complex 2-D DFT (double) */
+    fftplan pl;
+    plan_init(&pl, len);
+    cplx *kern = (cplx *)malloc(sizeof(cplx) * grid_len);
+    cplx *g = (cplx *)malloc(sizeof(cplx) * grid_len);
+    cplx *tmp_in = (cplx *)malloc(sizeof(cplx) * len), *tmp_out = (cplx *)malloc(sizeof(cplx) * len);
+    int rows = D == 2 ? len : 1;
+    for (int64_t f = 0; f < (int64_t)grid_len; f++) {
+        double sq = 0.0;
+        int64_t rem = f;
+        for (int a = 0; a < D; a++) {
+            int64_t lag = signed_lag(rem % len, len);
+            rem /= len;
+            sq += (spacing * lag) * (spacing * lag);
+        }
+        double c = 1.0 / (1.0 + sq);
+        kern[f].re = c * c;
+        kern[f].im = 0.0;
+    }
+#define FFT2(buf, sign)                                                                       \
+    do {                                                                                      \
+        for (int r = 0; r < rows; r++) {                                                      \
+            fft_exec(&pl, tmp_out, buf + (size_t)r * len, sign);                               \
+            memcpy(buf + (size_t)r * len, tmp_out, sizeof(cplx) * len);                        \
+        }                                                                                     \
+        if (D == 2)                                                                           \
+            for (int c = 0; c < len; c++) {                                                   \
+                for (int r = 0; r < len; r++) tmp_in[r] = buf[(size_t)r * len + c];           \
+                fft_exec(&pl, tmp_out, tmp_in, sign);                                          \
+                for (int r = 0; r < len; r++) buf[(size_t)r * len + c] = tmp_out[r];          \
+            }                                                                                 \
+    } while (0)
+    FFT2(kern, -1);
+    double *pot = (double *)malloc(sizeof(double) * (size_t)n * terms);
+    int combos = D == 2 ? IP * IP : IP;
+    for (int term = 0; term < terms; term++) {
+        memset(g, 0, sizeof(cplx) * grid_len);
+        for (int64_t i = 0; i < n; i++) {
+            double ch;
+            if (term == 0) ch = 1.0;
+            else if (term <= D) ch = yf[i * D + term - 1];
+            else {
+                ch = 0.0;
+                for (int a = 0; a < D; a++) ch += (double)yf[i * D + a] * yf[i * D + a];
+            }
+            for (int c = 0; c < combos; c++) {
+                int rem = c;
+                int64_t flat = 0;
+                double wt = 1.0;
+                for (int a = 0; a < D; a++) {
+                    int k = rem % IP;
+                    rem /= IP;
+                    flat = flat * len + box[i * D + a] * IP + k;
+                    wt *= w[(i * D + a) * IP + k];
+                }
+                g[flat].re += wt * ch;
+            }
+        }
+        FFT2(g, -1);
+        for (size_t f = 0; f < grid_len; f++) {
+            double ar = g[f].re, ai = g[f].im, br = kern[f].re, bi = kern[f].im;
+            g[f].re = ar * br - ai * bi;
+            g[f].im = ar * bi + ai * br;
+        }
+        FFT2(g, +1);
+        double inv_total = 1.0 / (double)grid_len;
+        for (int64_t i = 0; i < n; i++) {
+            double phi = 0.0;
+            for (int c = 0; c < combos; c++) {
+                int rem = c;
+                int64_t flat = 0;
+                double wt = 1.0;
+                for (int a = 0; a < D; a++) {
+                    int k = rem % IP;
+                    rem /= IP;
+                    flat = flat * len + box[i * D + a] * IP + k;
+                    wt *= w[(i * D + a) * IP + k];
+                }
+                phi += wt * g[flat].re * inv_total;
+            }
+            pot[i * terms + term] = phi;
+        }
+    }
+#undef FFT2
+    double zs = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        const double *phi = pot + i * terms;
+        double ns = 0.0, cross = 0.0;
+        for (int a = 0; a < D; a++) {
+            double p = yf[i * D + a];
+            neg[i * D + a] = p * phi[0] - phi[1 + a];
+            ns += p * p;
+            cross += p * phi[1 + a];
+        }
+        zs += (1.0 + ns) * phi[0] - 2.0 * cross + phi[D + 1];
+    }
+    *z_out = zs - (double)n;
+    free(pot);
+    free(tmp_in);
+    free(tmp_out);
+    free(g);
+    free(kern);
     plan_free(&pl);
     free(w);
     free(box);
@@ -1195,9 +1364,7 @@ EXPORT void oracle_sparse_kl(const float *y, int64_t n, int D, const uint64_t *r
         }
         partials[i] = cost;
     }
-    float s = 0.0f;
-    for (int64_t i = 0; i < n; i++) s = s + partials[i];
-    *ref_out = s;
+    *ref_out = pairwise_sum_f32(partials, n, 1); /* par_iter().sum() (mod.rs:774) */
     *true_out = tk;
     free(partials);
 }

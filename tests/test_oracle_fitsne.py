@@ -115,6 +115,21 @@ def test_finer_grid_is_more_accurate():
     assert abs(zf - ref_z) / ref_z < 1e-3
 
 
+@pytest.mark.parametrize("D,n,lo,hi", [(2, 800, -5.0, 5.0), (1, 500, -4.0, 4.0)])
+def test_f64_repulsion_matches_brute_force_and_f32(D, n, lo, hi):
+    # the f64 instantiation meets the same brute-force bounds, and the f32 path stays within
+    # f32 rounding of it
+    rng = np.random.default_rng(0xBADC0DE + D)
+    y = rng.uniform(lo, hi, (n, D)).astype(np.float32)
+    ref_neg, ref_z = _brute(y)
+    neg64, z64 = oracle.fitsne_repulsion_f64(y)
+    assert abs(z64 - ref_z) / ref_z < 1e-3
+    assert _rel_l2(neg64, ref_neg) < 3e-2
+    neg32, z32, _ = oracle.fitsne_repulsion(y)
+    assert _rel_l2(neg32, neg64) < 1e-5
+    assert abs(z32 - z64) / z64 < 1e-5
+
+
 def test_repulsion_is_translation_invariant():
     rng = np.random.default_rng(9)
     y = rng.uniform(-3, 3, (300, 2)).astype(np.float32)
