@@ -186,7 +186,7 @@ struct bht_job {
     uint64_t seed_counter = 0;
 
     // epoch workspace
-    DevBuf keys, svals, keys_alt, svals_alt, grid, spec, rker, kspec, part, dpart, fpart,
+    DevBuf keys, svals, keys_alt, svals_alt, grid, spec, rker, kspec, part, dpart, mpart, fpart,
         sort_temp, scal;
     int nodes_cap = 0, len_cap = 0;
     FitsneDev fd{};
@@ -258,7 +258,8 @@ void ensure_state_buffers(bht_job *j) {
     j->pos.ensure(j->ctx, b);
     j->scal.ensure(j->ctx, sizeof(EpochScalars));
     j->dpart.ensure(j->ctx, 4096 * sizeof(double));
-    j->fpart.ensure(j->ctx, 4096 * sizeof(float));
+    j->mpart.ensure(j->ctx, 2 * 4096 * sizeof(double));
+    j->fpart.ensure(j->ctx, 4 * 1024 * sizeof(float));
 }
 
 void ensure_fitsne_workspace(bht_job *j) {
@@ -299,6 +300,7 @@ void ensure_fitsne_workspace(bht_job *j) {
     d.kspec = j->kspec.as<float>();
     d.part = j->part.as<float>();
     d.dpart = j->dpart.as<double>();
+    d.mpart = j->mpart.as<double>();
     d.fpart = j->fpart.as<float>();
     d.sort_temp = j->sort_temp.p;
     d.scal = j->scal.as<EpochScalars>();
@@ -333,6 +335,13 @@ void ensure_bh_workspace(bht_job *j) {
     d.nodes.counts = j->bcount.as<uint32_t>();
     d.scal = j->scal.as<EpochScalars>();
     d.dpart = j->dpart.as<double>();
+}
+
+// Applies the recentring left pending by the last update (zero_mean, mod.rs:534-568) and writes
+// the per-axis bounds.
+void recenter(bht_job *j) {
+    center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
+                j->fpart.as<float>(), j->mpart.as<double>(), j->stream);
 }
 
 // ---- point order -----------------------------------------------------------------------------
@@ -602,8 +611,7 @@ const uint32_t *repulsion(bht_job *j) {
         return fitsne_repulsion(d, j->stream);
     }
     // recentre + per-axis bounds (arena.rs:65-94), then the arena and its traversal
-    center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
-                j->dpart.as<double>(), j->stream);
+    recenter(j);
     BhDev &b = j->bd;
     b.y = j->ycur();
     b.frep = j->frep.as<float>();
@@ -628,7 +636,7 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     a.stop_lying_epoch =
         (int)std::min<uint64_t>(j->params.stop_lying_epoch, (uint64_t)INT32_MAX - 2);
     a.scal = j->scal.as<EpochScalars>();
-    a.dpart = j->dpart.as<double>();
+    a.mpart = j->mpart.as<double>();
     for (uint64_t e = 0; e < count; e++) {
         j->mark();
         const uint32_t *order = repulsion(j);
@@ -653,8 +661,7 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
         if (epoch == j->params.momentum_switch_epoch) j->params.momentum = j->params.final_momentum;
         if (order && reorder_due(j, epoch)) reorder_points(j, order);
         if (cb) {
-            center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
-                        j->dpart.as<double>(), j->stream);
+            recenter(j);
             download_points(j, j->ycur(), snap.data());
             cb(epoch, snap.data(), len, user);
         }
@@ -663,8 +670,7 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
 
 void finish_run(bht_job *j) {
     if (!j->prepared) throw Error(ERR_STATE, "no prepared run");
-    center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
-                j->dpart.as<double>(), j->stream);
+    recenter(j);
     // stop-lying of the last epoch has not been applied lazily by a following epoch
     const uint64_t sl = j->params.stop_lying_epoch;
     if (sl != 0 && j->epoch == sl + 1)
@@ -1022,8 +1028,7 @@ bht_status bht_job_get_state(const bht_job *job, float *y, float *uy, float *gai
     return guarded([&] {
         if (!job->uy.p) throw Error(ERR_STATE, "no optimiser state");
         bht_job *j = const_cast<bht_job *>(job);
-        center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
-                    j->dpart.as<double>(), j->stream);
+        recenter(j);
         if (y) download_points(j, j->ycur(), y);
         if (uy) download_points(j, j->uy.as<float>(), uy);
         if (gains) download_points(j, j->gains.as<float>(), gains);
@@ -1037,8 +1042,9 @@ bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, cons
         if (y) upload_points(job, y, job->ycur());
         if (uy) upload_points(job, uy, job->uy.as<float>());
         if (gains) upload_points(job, gains, job->gains.as<float>());
+        // the state is taken as given: no recentring of an earlier update is pending
         EpochScalars *sc = job->scal.as<EpochScalars>();
-        BHT_HIP(hipMemsetAsync(sc->mean, 0, sizeof(sc->mean), job->stream));
+        BHT_HIP(hipMemsetAsync(&sc->mean_pending, 0, sizeof(int), job->stream));
         BHT_HIP(hipStreamSynchronize(job->stream));
     });
 }

@@ -1,22 +1,29 @@
 // Attractive forces over the CSR graph fused with the gains / momentum update and the partial
 // sums of the next recentring (src/tsne/arena.rs:398-427, src/tsne/mod.rs:470-516, 534-568).
 //
-// A group of 16 lanes owns one CSR row: lanes stride over the row's edges (coalesced column and
-// value loads, gathered y[col]), reduce with shuffles, and lane 0 applies the update.  The new
-// embedding goes to a second buffer (y_next) because other rows still read y this epoch.
+// One wave owns one CSR row at a time.  Each lane takes up to four edges of the row per step
+// (issuing their column, value and gathered y loads before using any), so a row of ~150 edges
+// needs one or two dependent memory round trips instead of ten.  The wave reduces with a fixed
+// butterfly and lane 0 applies the update.  The new embedding goes to a second buffer (y_next)
+// because other rows still read y this epoch.
 #include "internal.hpp"
 
 namespace bht {
 
 constexpr int UP_THREADS = 256;
-constexpr int GROUP = 16;
+constexpr int UNROLL = 4;
+
+int attract_blocks(int n) {
+    uint64_t b = ceil_div((uint64_t)(n > 0 ? n : 1), UP_THREADS / 64);
+    return (int)(b < 4096 ? b : 4096);
+}
 
 template <int D, bool UPDATE>
 __global__ void __launch_bounds__(UP_THREADS) k_attract(UpdateArgs a, float *pos_out) {
     __shared__ double red[UP_THREADS / 64];
-    const int sub = threadIdx.x & (GROUP - 1);
-    const int grp = (blockIdx.x * UP_THREADS + threadIdx.x) / GROUP;
-    const int ngrp = gridDim.x * UP_THREADS / GROUP;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * UP_THREADS + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * UP_THREADS) >> 6;
     int epoch = 0;
     float inv_z = 0.0f, momentum = a.momentum;
     bool stop_now = false;
@@ -30,43 +37,68 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(UpdateArgs a, float *pos
         momentum = epoch > a.momentum_switch_epoch ? a.final_momentum : a.momentum;
     }
     double m0 = 0.0, m1 = 0.0;
-    for (int i = grp; i < a.n; i += ngrp) {
-        float yi[2];
-        yi[0] = a.y[(size_t)i * D];
-        yi[1] = D == 2 ? a.y[(size_t)i * D + 1] : 0.0f;
+    for (int i = wave; i < a.n; i += nwaves) {
+        float yi0, yi1 = 0.0f;
+        if (D == 2) {
+            const float2 v = reinterpret_cast<const float2 *>(a.y)[i];
+            yi0 = v.x;
+            yi1 = v.y;
+        } else {
+            yi0 = a.y[i];
+        }
         float acc0 = 0.0f, acc1 = 0.0f;
-        const uint32_t e1 = a.rows[i + 1];
-        for (uint32_t e = a.rows[i] + sub; e < e1; e += GROUP) {
-            const uint32_t j = a.cols[e];
-            float pv = a.vals[e];
-            if (UPDATE && stop_now) {
-                pv = pv * a.inv_exag;
-                a.vals[e] = pv;
-            }
-            float d0 = yi[0] - a.y[(size_t)j * D];
-            float dist = 0.0f;
-            dist += d0 * d0;
-            float d1 = 0.0f;
-            if (D == 2) {
-                d1 = yi[1] - a.y[(size_t)j * D + 1];
-                dist += d1 * d1;
-            }
-            const float factor = pv / (dist + 1.0f);
-            acc0 += factor * d0;
-            if (D == 2) acc1 += factor * d1;
-        }
+        const uint32_t e0 = a.rows[i], e1 = a.rows[i + 1];
+        for (uint32_t base = e0; base < e1; base += 64 * UNROLL) {
+            uint32_t col[UNROLL];
+            float pv[UNROLL];
 #pragma unroll
-        for (int off = GROUP / 2; off > 0; off >>= 1) {
-            acc0 += __shfl_xor(acc0, off, GROUP);
-            if (D == 2) acc1 += __shfl_xor(acc1, off, GROUP);
+            for (int u = 0; u < UNROLL; u++) {
+                const uint32_t e = base + lane + 64 * u;
+                const bool ok = e < e1;
+                col[u] = ok ? __builtin_nontemporal_load(a.cols + e) : (uint32_t)i;
+                pv[u] = ok ? __builtin_nontemporal_load(a.vals + e) : 0.0f;
+                if (UPDATE && stop_now && ok) {
+                    pv[u] = pv[u] * a.inv_exag;
+                    a.vals[e] = pv[u];
+                }
+            }
+            float yj0[UNROLL], yj1[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                if (D == 2) {
+                    const float2 v = reinterpret_cast<const float2 *>(a.y)[col[u]];
+                    yj0[u] = v.x;
+                    yj1[u] = v.y;
+                } else {
+                    yj0[u] = a.y[col[u]];
+                    yj1[u] = 0.0f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                const float d0 = yi0 - yj0[u];
+                float dist = 0.0f;
+                dist += d0 * d0;
+                float d1 = 0.0f;
+                if (D == 2) {
+                    d1 = yi1 - yj1[u];
+                    dist += d1 * d1;
+                }
+                const float factor = pv[u] / (dist + 1.0f);
+                acc0 += factor * d0;
+                if (D == 2) acc1 += factor * d1;
+            }
         }
-        if (sub != 0) continue;
+        acc0 = wave_sum_f32(acc0);
+        if (D == 2) acc1 = wave_sum_f32(acc1);
+        if (lane != 0) continue;
         if (!UPDATE) {
             pos_out[(size_t)i * D] = acc0;
             if (D == 2) pos_out[(size_t)i * D + 1] = acc1;
             continue;
         }
-        float acc[2] = {acc0, acc1};
+        const float acc[2] = {acc0, acc1};
+        const float yi[2] = {yi0, yi1};
 #pragma unroll
         for (int d = 0; d < D; d++) {
             const size_t k = (size_t)i * D + d;
@@ -85,36 +117,21 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(UpdateArgs a, float *pos
         }
     }
     if (!UPDATE) return;
-    // partial sums of the new embedding for the next recentring
+    // per-block sums of the new embedding for the next recentring
     double s0 = block_sum_f64(m0, red);
     double s1 = D == 2 ? block_sum_f64(m1, red) : 0.0;
     if (threadIdx.x == 0) {
-        a.dpart[2 * blockIdx.x] = s0;
-        a.dpart[2 * blockIdx.x + 1] = s1;
+        a.mpart[2 * blockIdx.x] = s0;
+        a.mpart[2 * blockIdx.x + 1] = s1;
+        if (blockIdx.x == 0) {
+            a.scal->epoch_next = epoch + 1;
+            a.scal->mean_pending = 1;
+        }
     }
-    if (!arrive_last(&a.scal->ctr[CTR_UPDATE], gridDim.x)) return;
-    double t0 = 0.0, t1 = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += UP_THREADS) {
-        t0 += a.dpart[2 * b];
-        t1 += a.dpart[2 * b + 1];
-    }
-    t0 = block_sum_f64(t0, red);
-    t1 = block_sum_f64(t1, red);
-    if (threadIdx.x == 0) {
-        a.scal->mean[0] = (float)(t0 / (double)a.n);
-        a.scal->mean[1] = D == 2 ? (float)(t1 / (double)a.n) : 0.0f;
-        a.scal->epoch = epoch + 1;
-        a.scal->ctr[CTR_UPDATE] = 0;
-    }
-}
-
-static int update_blocks(int n) {
-    uint64_t b = ceil_div((uint64_t)(n > 0 ? n : 1) * GROUP, UP_THREADS);
-    return (int)(b < 2048 ? b : 2048);
 }
 
 void attract_update(const UpdateArgs &a, hipStream_t s) {
-    const int nb = update_blocks(a.n);
+    const int nb = attract_blocks(a.n);
     if (a.dim == 2)
         hipLaunchKernelGGL((k_attract<2, true>), dim3(nb), dim3(UP_THREADS), 0, s, a, nullptr);
     else
@@ -131,7 +148,7 @@ void attract_only(int n, int dim, const uint32_t *rows, const uint32_t *cols, co
     a.cols = cols;
     a.vals = const_cast<float *>(vals);
     a.y = y;
-    const int nb = update_blocks(n);
+    const int nb = attract_blocks(n);
     if (dim == 2)
         hipLaunchKernelGGL((k_attract<2, false>), dim3(nb), dim3(UP_THREADS), 0, s, a, pos);
     else

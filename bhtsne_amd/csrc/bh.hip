@@ -253,18 +253,6 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const float *y, in
     }
     zacc = wave_sum_f64(zacc);
     if (lane == 0) dpart[blockIdx.x] = zacc;
-    if (!arrive_last(&scal->ctr[slot], gridDim.x)) return;
-    double z = 0.0;
-    for (int b = lane; b < (int)gridDim.x; b += TRAV_THREADS) z += dpart[b];
-    z = wave_sum_f64(z);
-    if (lane == 0) {
-        red[0] = z;
-        scal->zsum = z;
-        const float zf = (float)z;
-        scal->z = zf;
-        scal->inv_z = 1.0f / zf;
-        scal->ctr[slot] = 0;
-    }
 }
 
 const uint32_t *bh_repulsion(const BhDev &d, float theta, hipStream_t s, int slot) {
@@ -282,6 +270,8 @@ const uint32_t *bh_repulsion(const BhDev &d, float theta, hipStream_t s, int slo
     hipLaunchKernelGGL(k_bh_traverse, dim3(tb), dim3(TRAV_THREADS), 0, s, d.y, n, si, d.nodes,
                        theta * theta, d.frep, d.scal, d.dpart, slot);
     BHT_LAUNCH_CHECK();
+    // Z is the plain sum of the per-point q sums (repulsion.rs:155)
+    finalize_z(d.scal, d.dpart, tb, n, false, s);
     return si;  // point indices in Morton order
 }
 

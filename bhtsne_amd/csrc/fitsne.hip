@@ -60,14 +60,41 @@ __device__ __forceinline__ int locate_box(float v, const GridPlan &p, float &loc
 }
 
 // ---- 1. recentring + bounds + plan -----------------------------------------------------------
+//
+// Three small kernels instead of one with a grid-wide "last block" reduction: a device-scope fence
+// in every block would write back the XCD's L2 each time.  Kernel boundaries give the ordering.
 
+// Mean of the last update's per-block sums (written by the attraction kernel), applied once.
+__global__ void __launch_bounds__(256) k_mean_finalize(EpochScalars *scal, const double *mpart,
+                                                       int nparts, int n) {
+    __shared__ double red[4];
+    const int pending = scal->mean_pending;
+    if (!pending) {
+        if (threadIdx.x == 0) {
+            scal->mean[0] = 0.0f;
+            scal->mean[1] = 0.0f;
+        }
+        return;
+    }
+    double t0 = 0.0, t1 = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += 256) {
+        t0 += mpart[2 * b];
+        t1 += mpart[2 * b + 1];
+    }
+    t0 = block_sum_f64(t0, red);
+    t1 = block_sum_f64(t1, red);
+    if (threadIdx.x == 0) {
+        scal->mean[0] = (float)(t0 / (double)n);
+        scal->mean[1] = (float)(t1 / (double)n);
+        scal->mean_pending = 0;
+        scal->epoch = scal->epoch_next;
+    }
+}
+
+// y -= mean; per-block, per-axis min / max.
 template <int D>
-__global__ void __launch_bounds__(PT_THREADS) k_center_plan(float *y, int n, EpochScalars *scal,
-                                                            float *fpart, int min_boxes,
-                                                            int max_boxes,
-                                                            const int *len_of_boxes,
-                                                            const int *off_of_boxes,
-                                                            int do_plan) {
+__global__ void __launch_bounds__(PT_THREADS) k_center(float *y, int n, const EpochScalars *scal,
+                                                       float *fpart) {
     __shared__ float red[4][PT_THREADS / 64];
     const float m0 = scal->mean[0], m1 = D == 2 ? scal->mean[1] : 0.0f;
     float mn0 = INFINITY, mn1 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY;
@@ -96,32 +123,35 @@ __global__ void __launch_bounds__(PT_THREADS) k_center_plan(float *y, int n, Epo
             mn0 = fminf(mn0, red[0][q]); mx0 = fmaxf(mx0, red[1][q]);
             mn1 = fminf(mn1, red[2][q]); mx1 = fmaxf(mx1, red[3][q]);
         }
-        float4 pv = make_float4(mn0, mx0, mn1, mx1);
-        reinterpret_cast<float4 *>(fpart)[blockIdx.x] = pv;
+        reinterpret_cast<float4 *>(fpart)[blockIdx.x] = make_float4(mn0, mx0, mn1, mx1);
     }
-    if (!arrive_last(&scal->ctr[CTR_CENTER], gridDim.x)) return;
-    // last block: combine the per-block bounds
-    mn0 = INFINITY; mx0 = -INFINITY; mn1 = INFINITY; mx1 = -INFINITY;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += PT_THREADS) {
+}
+
+// Combines the bounds; with do_plan, decides the FIt-SNE grid of this epoch.
+template <int D>
+__global__ void __launch_bounds__(256) k_plan(EpochScalars *scal, const float *fpart, int nparts,
+                                              int min_boxes, int max_boxes,
+                                              const int *len_of_boxes, const int *off_of_boxes,
+                                              int do_plan) {
+    __shared__ float red[4][4];
+    float mn0 = INFINITY, mx0 = -INFINITY, mn1 = INFINITY, mx1 = -INFINITY;
+    for (int b = threadIdx.x; b < nparts; b += 256) {
         float4 pv = reinterpret_cast<const float4 *>(fpart)[b];
         mn0 = fminf(mn0, pv.x); mx0 = fmaxf(mx0, pv.y);
         mn1 = fminf(mn1, pv.z); mx1 = fmaxf(mx1, pv.w);
     }
     mn0 = wave_min_f32(mn0); mx0 = wave_max_f32(mx0);
     mn1 = wave_min_f32(mn1); mx1 = wave_max_f32(mx1);
-    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) { red[0][w] = mn0; red[1][w] = mx0; red[2][w] = mn1; red[3][w] = mx1; }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int q = 1; q < PT_THREADS / 64; q++) {
+    for (int q = 1; q < 4; q++) {
         mn0 = fminf(mn0, red[0][q]); mx0 = fmaxf(mx0, red[1][q]);
         mn1 = fminf(mn1, red[2][q]); mx1 = fmaxf(mx1, red[3][q]);
     }
     scal->amin[0] = mn0; scal->amax[0] = mx0;
     scal->amin[1] = mn1; scal->amax[1] = mx1;
-    scal->mean[0] = 0.0f;
-    scal->mean[1] = 0.0f;
-    scal->ctr[CTR_CENTER] = 0;
     if (!do_plan) return;
     // coordinate_bounds + box_count + next_smooth (interpolation.rs:115-128, 337-378)
     float lo = D == 2 ? fminf(mn0, mn1) : mn0;
@@ -153,29 +183,55 @@ __global__ void __launch_bounds__(PT_THREADS) k_center_plan(float *y, int n, Epo
     scal->plan = p;
 }
 
-void fitsne_center_plan(const FitsneDev &d, hipStream_t s) {
-    int nb = point_blocks(d.n);
-    const int maxb = d.nodes_cap / IP;
-    if (d.dim == 2)
-        hipLaunchKernelGGL(k_center_plan<2>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
-                           d.fpart, d.min_boxes, maxb, d.tw->len_of_boxes, d.tw->off_of_boxes,
-                           1);
-    else
-        hipLaunchKernelGGL(k_center_plan<1>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
-                           d.fpart, d.min_boxes, maxb, d.tw->len_of_boxes, d.tw->off_of_boxes,
-                           1);
+static void center_launch(float *y, int n, int dim, EpochScalars *scal, float *fpart,
+                          const double *mpart, int min_boxes, int max_boxes,
+                          const TwiddleSet *tw, hipStream_t s) {
+    const int nb = point_blocks(n);
+    hipLaunchKernelGGL(k_mean_finalize, dim3(1), dim3(256), 0, s, scal, mpart, attract_blocks(n),
+                       n);
+    const int do_plan = tw != nullptr;
+    const int *lo = tw ? tw->len_of_boxes : nullptr, *of = tw ? tw->off_of_boxes : nullptr;
+    if (dim == 2) {
+        hipLaunchKernelGGL(k_center<2>, dim3(nb), dim3(PT_THREADS), 0, s, y, n, scal, fpart);
+        hipLaunchKernelGGL(k_plan<2>, dim3(1), dim3(256), 0, s, scal, fpart, nb, min_boxes,
+                           max_boxes, lo, of, do_plan);
+    } else {
+        hipLaunchKernelGGL(k_center<1>, dim3(nb), dim3(PT_THREADS), 0, s, y, n, scal, fpart);
+        hipLaunchKernelGGL(k_plan<1>, dim3(1), dim3(256), 0, s, scal, fpart, nb, min_boxes,
+                           max_boxes, lo, of, do_plan);
+    }
     BHT_LAUNCH_CHECK();
 }
 
-void center_only(float *y, int n, int dim, EpochScalars *scal, double *dpart, hipStream_t s) {
-    int nb = point_blocks(n);
-    float *fpart = reinterpret_cast<float *>(dpart);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_center_plan<2>, dim3(nb), dim3(PT_THREADS), 0, s, y, n, scal, fpart,
-                           10, MAX_BOXES, nullptr, nullptr, 0);
-    else
-        hipLaunchKernelGGL(k_center_plan<1>, dim3(nb), dim3(PT_THREADS), 0, s, y, n, scal, fpart,
-                           10, MAX_BOXES, nullptr, nullptr, 0);
+void fitsne_center_plan(const FitsneDev &d, hipStream_t s) {
+    center_launch(d.y, d.n, d.dim, d.scal, d.fpart, d.mpart, d.min_boxes, d.nodes_cap / IP, d.tw,
+                  s);
+}
+
+void center_only(float *y, int n, int dim, EpochScalars *scal, float *fpart, const double *mpart,
+                 hipStream_t s) {
+    center_launch(y, n, dim, scal, fpart, mpart, 10, MAX_BOXES, nullptr, s);
+}
+
+// Z = sum of the per-block sums (minus the n self-interactions for FIt-SNE, interpolation.rs:331).
+__global__ void __launch_bounds__(256) k_z_finalize(EpochScalars *scal, const double *dpart,
+                                                    int nparts, int n, int subtract_n) {
+    __shared__ double red[4];
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += 256) t += dpart[b];
+    t = block_sum_f64(t, red);
+    if (threadIdx.x == 0) {
+        scal->zsum = t;
+        const float z = subtract_n ? (float)(t - (double)n) : (float)t;
+        scal->z = z;
+        scal->inv_z = 1.0f / z;
+    }
+}
+
+void finalize_z(EpochScalars *scal, const double *dpart, int nparts, int n, bool subtract_n,
+                hipStream_t s) {
+    hipLaunchKernelGGL(k_z_finalize, dim3(1), dim3(256), 0, s, scal, dpart, nparts, n,
+                       subtract_n ? 1 : 0);
     BHT_LAUNCH_CHECK();
 }
 
@@ -440,18 +496,6 @@ __global__ void __launch_bounds__(PT_THREADS) k_gather(const float *y, int n,
     }
     double bs = block_sum_f64(zacc, red);
     if (threadIdx.x == 0) dpart[blockIdx.x] = bs;
-    if (!arrive_last(&scal->ctr[CTR_GATHER], gridDim.x)) return;
-    double t = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += PT_THREADS)
-        t += dpart[b];
-    t = block_sum_f64(t, red);
-    if (threadIdx.x == 0) {
-        scal->zsum = t;
-        float z = (float)(t - (double)n);
-        scal->z = z;
-        scal->inv_z = 1.0f / z;
-        scal->ctr[CTR_GATHER] = 0;
-    }
 }
 
 // ---- driver ----------------------------------------------------------------------------------
@@ -496,6 +540,7 @@ const uint32_t *fitsne_repulsion(FitsneDev &d, hipStream_t s) {
         hipLaunchKernelGGL(k_gather<1>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
                            d.grid, d.frep, d.dpart);
     BHT_LAUNCH_CHECK();
+    finalize_z(d.scal, d.dpart, nb, d.n, true, s);
     return sv;  // point indices in box order: a spatial order of the current embedding
 }
 

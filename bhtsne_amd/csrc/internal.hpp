@@ -31,6 +31,8 @@ struct EpochScalars {
     int pad0;
     double psum;           // FP64 sum of P (normalisation)
     double kl_ref, kl_true;// KL divergence: reference formula, true formula
+    int mean_pending;      // the last update left recentring partials to be applied
+    int epoch_next;        // epoch index after the last update
     unsigned ctr[8];       // arrival counters of the single-pass reductions
 };
 
@@ -62,7 +64,8 @@ struct FitsneDev {
     float *rker;       // kernel row transform, (len_cap/2+1)^2
     float *kspec;      // kernel spectrum quarter, (len_cap/2+1)^2
     float *part;       // spread partials, 2 x nchunks x 36
-    double *dpart;     // per-block FP64 partials
+    double *dpart;     // per-block FP64 partials (Z)
+    double *mpart;     // per-block FP64 partials of the update (recentring sums)
     float *fpart;      // per-block f32 partials (min/max)
     void *sort_temp;
     EpochScalars *scal;
@@ -80,9 +83,16 @@ bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, 
                       int begin_bit, int end_bit, void *temp, hipStream_t s);
 
 // ---- FIt-SNE epoch (fitsne.hip, fft.hip) ---------------------------------------------------
-int point_blocks(int n);  // grid size of the point-parallel kernels
+int point_blocks(int n);    // grid size of the point-parallel kernels
+int attract_blocks(int n);  // grid size of the attraction kernel (= its partial count)
+// Applies the pending recentring of the last update, then bounds and the grid plan.
 void fitsne_center_plan(const FitsneDev &d, hipStream_t s);
-void center_only(float *y, int n, int dim, EpochScalars *scal, double *dpart, hipStream_t s);
+// Applies the pending recentring and writes the per-axis bounds (no grid plan).
+void center_only(float *y, int n, int dim, EpochScalars *scal, float *fpart, const double *mpart,
+                 hipStream_t s);
+// Z and 1/Z from per-block partial sums (minus n for FIt-SNE).
+void finalize_z(EpochScalars *scal, const double *dpart, int nparts, int n, bool subtract_n,
+                hipStream_t s);
 // after center_plan: frep, Z; returns the point indices in box order (device)
 const uint32_t *fitsne_repulsion(FitsneDev &d, hipStream_t s);
 void fft_kernel_spectrum(const FitsneDev &d, hipStream_t s);
@@ -101,7 +111,7 @@ struct UpdateArgs {
     float learning_rate, momentum, final_momentum, inv_exag;
     int momentum_switch_epoch, stop_lying_epoch;
     EpochScalars *scal;
-    double *dpart;
+    double *mpart;  // per-block sums of the new embedding (2 per block)
 };
 void attract_update(const UpdateArgs &a, hipStream_t s);
 void attract_only(int n, int dim, const uint32_t *rows, const uint32_t *cols, const float *vals,
