@@ -62,6 +62,7 @@ def lib():
         "oracle_lagrange_weights": (None, [C.c_float, f32p]),
         "oracle_fitsne_repulsion": (None, [f32p, C.c_int64, C.c_int, C.c_int, f32p,
                                            C.POINTER(C.c_float), f64p]),
+        "oracle_set_fft_f32": (None, [C.c_int]),
         "oracle_fitsne_repulsion_f64": (None, [f32p, C.c_int64, C.c_int, C.c_int, f64p,
                                                C.POINTER(C.c_double)]),
         "oracle_morton_encode2": (C.c_uint64, [C.c_uint32, C.c_uint32]),
@@ -192,6 +193,17 @@ def fitsne_repulsion(y, min_boxes=10):
     info = np.zeros(4, np.float64)
     lib().oracle_fitsne_repulsion(y, n, D, int(min_boxes), neg, C.byref(z), info)
     return neg, z.value, dict(boxes=int(info[0]), fft_len=int(info[1]), min=info[2], max=info[3])
+
+
+class f32_fft:
+    """Context manager: the f32 pipeline's transforms run in f32 arithmetic (a representative
+    single-precision FFT, as the reference's rustfft computes) instead of exact double."""
+
+    def __enter__(self):
+        lib().oracle_set_fft_f32(1)
+
+    def __exit__(self, *exc):
+        lib().oracle_set_fft_f32(0)
 
 
 def fitsne_repulsion_f64(y, min_boxes=10):

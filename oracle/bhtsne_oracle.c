@@ -502,6 +502,67 @@ static void fft_exec(const fftplan *pl, cplx *out, const cplx *in, int sign) {
     fft_rec(pl, out, in, pl->n, 1, 0, sign);
 }
 
+/* The same decimation in f32 arithmetic with f32 twiddles: a representative single-precision
+ * FFT library, standing in for rustfft/realfft's f32 transforms when the f32 pipeline runs in
+ * "f32 FFT" mode (oracle_set_fft_f32).  Its rounding is not rustfft's; it shows the size of an
+ * f32 FFT's noise next to the double transform's. */
+typedef struct {
+    float re, im;
+} cplxf;
+
+static int g_fft_f32 = 0;
+
+EXPORT void oracle_set_fft_f32(int on) { g_fft_f32 = on; }
+
+static void fft_rec_f(const fftplan *pl, cplxf *out, const cplxf *in, int n, int s, int fi,
+                      int sign) {
+    if (n == 1) {
+        out[0] = in[0];
+        return;
+    }
+    int p = pl->factors[fi];
+    int m = n / p;
+    for (int r = 0; r < p; r++) fft_rec_f(pl, out + r * m, in + r * s, m, s * p, fi + 1, sign);
+    int step = pl->n / n;
+    cplxf tmp[64];
+    for (int k = 0; k < m; k++) {
+        for (int r = 0; r < p; r++) tmp[r] = out[r * m + k];
+        for (int q = 0; q < p; q++) {
+            float re = 0.0f, im = 0.0f;
+            int kk = k + q * m;
+            for (int r = 0; r < p; r++) {
+                long e = ((long)r * kk) % n;
+                float wr = (float)pl->tw[e * step].re, wi = (float)pl->tw[e * step].im;
+                if (sign > 0) wi = -wi;
+                re += tmp[r].re * wr - tmp[r].im * wi;
+                im += tmp[r].re * wi + tmp[r].im * wr;
+            }
+            out[q * m + k].re = re;
+            out[q * m + k].im = im;
+        }
+    }
+}
+
+static void fft_exec_any(const fftplan *pl, cplx *out, const cplx *in, int sign) {
+    if (!g_fft_f32) {
+        fft_exec(pl, out, in, sign);
+        return;
+    }
+    int n = pl->n;
+    cplxf *a = (cplxf *)malloc(sizeof(cplxf) * n), *b = (cplxf *)malloc(sizeof(cplxf) * n);
+    for (int i = 0; i < n; i++) {
+        a[i].re = (float)in[i].re;
+        a[i].im = (float)in[i].im;
+    }
+    fft_rec_f(pl, b, a, n, 1, 0, sign);
+    for (int i = 0; i < n; i++) {
+        out[i].re = b[i].re;
+        out[i].im = b[i].im;
+    }
+    free(a);
+    free(b);
+}
+
 /* Double-precision 2-D real forward transform of a len x len row-major grid into the half
  * spectrum len x (len/2+1) (interleaved re, im), fft.rs:101-111 semantics. */
 EXPORT void oracle_fft2_r2c_f64(const double *grid, double *spec, int len) {
@@ -650,7 +711,7 @@ static void fwd_f32(const fftplan *pl, const float *grid, float *spec, int len, 
 #pragma omp for schedule(static)
         for (int r = 0; r < rows; r++) {
             for (int c = 0; c < len; c++) { in[c].re = grid[(size_t)r * len + c]; in[c].im = 0.0; }
-            fft_exec(pl, out, in, -1);
+            fft_exec_any(pl, out, in,-1);
             for (int c = 0; c < h; c++) {
                 spec[2 * ((size_t)r * h + c)] = (float)out[c].re;
                 spec[2 * ((size_t)r * h + c) + 1] = (float)out[c].im;
@@ -663,7 +724,7 @@ static void fwd_f32(const fftplan *pl, const float *grid, float *spec, int len, 
                     in[r].re = spec[2 * ((size_t)r * h + c)];
                     in[r].im = spec[2 * ((size_t)r * h + c) + 1];
                 }
-                fft_exec(pl, out, in, -1);
+                fft_exec_any(pl, out, in,-1);
                 for (int r = 0; r < len; r++) {
                     spec[2 * ((size_t)r * h + c)] = (float)out[r].re;
                     spec[2 * ((size_t)r * h + c) + 1] = (float)out[r].im;
@@ -688,7 +749,7 @@ static void inv_f32(const fftplan *pl, float *spec, float *grid, int len, int D)
                     in[r].re = spec[2 * ((size_t)r * h + c)];
                     in[r].im = spec[2 * ((size_t)r * h + c) + 1];
                 }
-                fft_exec(pl, out, in, +1);
+                fft_exec_any(pl, out, in,+1);
                 for (int r = 0; r < len; r++) {
                     spec[2 * ((size_t)r * h + c)] = (float)out[r].re;
                     spec[2 * ((size_t)r * h + c) + 1] = (float)out[r].im;
@@ -704,7 +765,7 @@ static void inv_f32(const fftplan *pl, float *spec, float *grid, int len, int D)
                 if (c < h) { in[c].re = row[2 * c]; in[c].im = row[2 * c + 1]; }
                 else { in[c].re = row[2 * (len - c)]; in[c].im = -row[2 * (len - c) + 1]; }
             }
-            fft_exec(pl, out, in, +1);
+            fft_exec_any(pl, out, in,+1);
             for (int c = 0; c < len; c++) grid[(size_t)r * len + c] = (float)out[c].re;
         }
         free(in);
