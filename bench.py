@@ -46,6 +46,21 @@ def make_data(workload):
     return datagen.config_c1()[0]
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at the 1M-point workload, as recorded from the FETCH_SIZE /
+    WRITE_SIZE counter passes (scaled by the copy calibration) in the profile summary."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "k_attract_c3_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(aff, y_seed, state_late, min_boxes, n_epochs=3):
     """The oracle (a C restatement of the reference's f32 loop, OpenMP over rows as rayon is)
     timed on this host: n_epochs from the seed state and n_epochs from the GPU run's final state,
@@ -84,7 +99,19 @@ def main():
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-epochs", type=int, default=3)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (the "
+                         "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import torch
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs every job on a single MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
@@ -134,7 +161,9 @@ def main():
     upd_avg_s = upd_ms / 1e3 / max(timed, 1)
     achieved = alg_bytes / upd_avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic("k_attract") if args.workload == "c3" else None,
+                "algorithmic_bytes": alg_bytes,
                 "kernel": "k_attract<2,true> (attraction + gains/momentum update)",
                 "avg_ms": round(upd_avg_s * 1e3, 4),
                 "repulsion_avg_ms": round(rep_ms / max(timed, 1), 4)}
