@@ -30,7 +30,126 @@ WORKLOADS = {
                "N(0,16I), axis std 1/sqrt(i+1), seed 1), perplexity 30", min_boxes=10),
     "c1": dict(n=5_000, name="FIt-SNE 2D, N=5,000, D_in=50, 10-component GMM (seed 0), "
                "perplexity 30", min_boxes=10),
+    "c4": dict(name="batched: independent jobs seeded by job id, N log-uniform [500, 50,000], "
+               "D_in in {16,32,64}, 5-20 component GMM (centres N(0,25I), sigma 1), D_out 2, "
+               "FIt-SNE (even ids) / Barnes-Hut theta 0.5 (odd ids), perplexity 30"),
+    "c5": dict(name="serving: Poisson arrivals, requests seeded by id, N log-uniform "
+               "[1,000, 200,000], D_in in {20,50}, D_out in {1,2}, 10-component GMM, FIt-SNE, "
+               "perplexity 30"),
 }
+
+
+def bench_batched(args):
+    """Batched jobs/s: every job in one batch (affinity builds, then all epochs advanced by
+    shared launches), against plain one-job-at-a-time fits of every `stride`-th job."""
+    import bhtsne_amd as bt
+    from bhtsne_amd import datagen
+    njobs, epochs, stride = args.jobs, args.job_epochs, args.plain_stride
+    specs = [datagen.config_c4_job(j) for j in range(njobs)]
+    ns = [x.shape[0] for x, _ in specs]
+    log(f"{njobs} jobs, N {min(ns)}..{max(ns)} (sum {sum(ns)}), {epochs} epochs each")
+    ctx = bt.Context(0)
+
+    def make(j):
+        x, _ = specs[j]
+        # the batch does not renumber points; the plain fits do the same arithmetic without it
+        t = bt.TSNE(x, ctx=ctx).perplexity(30.0).epochs(epochs)._set(flags=1)
+        return t.initial_embedding(datagen.seed_embedding(x.shape[0], 2, j))
+
+    # warm the kernels (first launches load the code objects)
+    w = bt.Batch(ctx)
+    w.add(make(0), specs[0][1], 0.5)
+    w.add(make(1), specs[1][1], 0.5)
+    w.run()
+    w.close()
+
+    t0 = time.perf_counter()
+    b = bt.Batch(ctx)
+    jobs = []
+    for j in range(njobs):
+        t = make(j)
+        b.add(t, specs[j][1], 0.5)
+        jobs.append(t)
+    t_admit = time.perf_counter() - t0
+    done = b.run()
+    t_batch = time.perf_counter() - t0
+    assert len(done) == njobs
+    embs = {j: jobs[j].embedding() for j in range(0, njobs, stride)}
+    kls = {j: jobs[j].kl_divergence() for j in range(0, njobs, stride)}
+    b.close()
+    log(f"batched: {njobs} jobs in {t_batch:.2f}s (admission incl. affinities {t_admit:.2f}s, "
+        f"epochs {t_batch - t_admit:.2f}s) -> {njobs / t_batch:.2f} jobs/s")
+    del jobs, done
+
+    # plain: each sampled job fitted alone
+    plain_t, same, max_kl = 0.0, 0, 0.0
+    sample = list(range(0, njobs, stride))
+    for j in sample:
+        t = make(j)
+        s0 = time.perf_counter()
+        if specs[j][1] == "fitsne":
+            t.fit_sne()
+        else:
+            t.barnes_hut(0.5)
+        t.synchronize()
+        plain_t += time.perf_counter() - s0
+        same += int(np.array_equal(t.embedding(), embs[j]))
+        max_kl = max(max_kl, abs(t.kl_divergence() - kls[j]) / abs(kls[j]))
+    plain_rate = len(sample) / plain_t
+    log(f"plain: {len(sample)} sampled jobs in {plain_t:.2f}s -> {plain_rate:.2f} jobs/s; "
+        f"{same}/{len(sample)} embeddings bit-identical to the batch, max KL rel diff {max_kl:.1e}")
+    rate = njobs / t_batch
+    return {"metric": "batched jobs/s (256 mixed FIt-SNE / Barnes-Hut jobs, 1000 epochs each)",
+            "value": round(rate, 3), "unit": "jobs/s", "n_gpus": 1, "steps": njobs,
+            "warmup": 2, "ms_per_step": round(1e3 * t_batch / njobs, 3),
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": WORKLOADS["c4"]["name"], "jobs": njobs, "epochs": epochs,
+                       "sum_n": sum(ns), "parallelism": "single GPU"},
+            "batched": {"jobs_per_s": round(rate, 3), "seconds": round(t_batch, 3),
+                        "admission_s": round(t_admit, 3)},
+            "plain": {"jobs_per_s": round(plain_rate, 3), "sampled_jobs": len(sample),
+                      "seconds": round(plain_t, 3)},
+            "vs_plain": round(rate / plain_rate, 3),
+            "parity": {"bit_identical": f"{same}/{len(sample)}", "max_kl_rel_diff": max_kl}}
+
+
+def bench_serving(args):
+    """Serving: a Poisson stream of requests through the continuous-batching loop, then the same
+    stream served one plain fit at a time; jobs/s, p50 / p99 latency, per-job equality."""
+    import bhtsne_amd as bt
+    from bhtsne_amd import datagen, serving
+    count, rate, epochs = args.requests, args.rate, args.job_epochs
+    arrivals = datagen.poisson_arrivals(count, rate, 0)
+    reqs = []
+    for r in range(count):
+        x, d_out = datagen.config_c5_request(r)
+        reqs.append(serving.Request(r, x, d_out=d_out, epochs=epochs, arrival=float(arrivals[r]),
+                                    y0=datagen.seed_embedding(x.shape[0], d_out, r)))
+    ns = [q.x.shape[0] for q in reqs]
+    log(f"{count} requests at {rate}/s, N {min(ns)}..{max(ns)}, {epochs} epochs each")
+    ctx = bt.Context(0)
+    # warm-up request
+    serving.serve([serving.Request(-1, reqs[0].x[:2000], epochs=20)], ctx, want_kl=False)
+    res_b = serving.serve(reqs, ctx, step_epochs=args.step_epochs)
+    sb = serving.latency_summary(res_b)
+    log(f"batched: {sb}")
+    res_p = serving.serve_plain(reqs, ctx)
+    sp = serving.latency_summary(res_p)
+    log(f"plain: {sp}")
+    same = sum(int(np.array_equal(a.embedding, c.embedding)) for a, c in zip(res_b, res_p))
+    kl_diff = max(abs(a.kl - c.kl) / abs(c.kl) for a, c in zip(res_b, res_p))
+    return {"metric": "served jobs/s with p50/p99 latency (Poisson arrivals)",
+            "value": round(sb["jobs_per_s"], 3), "unit": "jobs/s", "n_gpus": 1,
+            "steps": count, "warmup": 1, "ms_per_step": round(1e3 * sb["span_s"] / count, 3),
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": WORKLOADS["c5"]["name"], "requests": count,
+                       "rate_per_s": rate, "epochs": epochs, "parallelism": "single GPU"},
+            "batched": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sb.items()},
+            "plain": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sp.items()},
+            "vs_plain": round(sb["jobs_per_s"] / sp["jobs_per_s"], 3),
+            "parity": {"bit_identical": f"{same}/{count}", "max_kl_rel_diff": kl_diff}}
 
 
 def log(msg):
@@ -99,6 +218,15 @@ def main():
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-epochs", type=int, default=3)
+    ap.add_argument("--jobs", type=int, default=256, help="batched workload: number of jobs")
+    ap.add_argument("--job-epochs", type=int, default=1000,
+                    help="batched / serving workloads: epochs per job")
+    ap.add_argument("--plain-stride", type=int, default=4,
+                    help="batched workload: fit every k-th job alone for the plain rate")
+    ap.add_argument("--requests", type=int, default=48, help="serving workload: requests")
+    ap.add_argument("--rate", type=float, default=4.0, help="serving workload: arrivals per s")
+    ap.add_argument("--step-epochs", type=int, default=25,
+                    help="serving workload: epochs between admissions")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -116,6 +244,11 @@ def main():
         print(f"bench.py measures one GPU: this build runs every job on a single MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload in ("c4", "c5"):
+        out = bench_batched(args) if args.workload == "c4" else bench_serving(args)
+        print(json.dumps(out), flush=True)
+        return
 
     import bhtsne_amd as bt
     from bhtsne_amd import datagen

@@ -17,8 +17,8 @@ import numpy as np
 from . import _lib
 from ._lib import BhtError, InvalidInput, check, ptr
 
-__all__ = ["Context", "Neighbor", "SparseAffinities", "TSNE", "BhtError", "InvalidInput",
-           "knn", "search_beta", "symmetrize", "default_context"]
+__all__ = ["Context", "Neighbor", "SparseAffinities", "TSNE", "Batch", "BhtError",
+           "InvalidInput", "knn", "search_beta", "symmetrize", "default_context", "job_footprint"]
 
 
 class Context:
@@ -320,6 +320,90 @@ class TSNE:
         check(self._L.bht_job_grid_info(self._h, C.byref(b), C.byref(L_), C.byref(lo),
                                         C.byref(hi)))
         return dict(boxes=b.value, fft_len=L_.value, min=lo.value, max=hi.value)
+
+
+# ---- batched jobs ---------------------------------------------------------------------------------
+
+class Batch:
+    """Many independent TSNE jobs advanced together: every epoch kernel is one launch for all
+    members of a method / d_out, replayed from a captured graph.  Each member's result equals its
+    fit alone (``TSNE.fit_sne`` / ``barnes_hut``), with the point renumbering of large single jobs
+    left out.  Jobs join with ``add`` (which builds their affinities unless cached) and leave when
+    their ``epochs`` are done."""
+
+    def __init__(self, ctx: Optional[Context] = None, graph: bool = True):
+        self._ctx = ctx or default_context()
+        self._L = _lib.load()
+        h = C.c_void_p()
+        check(self._L.bht_batch_create(self._ctx.handle, C.byref(h)))
+        self._h = h
+        self._members = {}  # handle value -> TSNE (kept alive while a member)
+        if not graph:
+            check(self._L.bht_batch_set_graph(self._h, 0))
+
+    def add(self, job: TSNE, method: str = "fitsne", theta: float = 0.5):
+        m = _lib.METHOD_FITSNE if method == "fitsne" else _lib.METHOD_BARNES_HUT
+        check(self._L.bht_batch_add(self._h, job._h, m, float(theta)))
+        self._members[job._h.value] = job
+        return self
+
+    def __len__(self):
+        return int(self._L.bht_batch_size(self._h))
+
+    def step(self, max_epochs: int) -> int:
+        k = C.c_uint64(0)
+        check(self._L.bht_batch_step(self._h, int(max_epochs), C.byref(k)))
+        return int(k.value)
+
+    def take_finished(self) -> list:
+        out = []
+        buf = (C.c_void_p * 64)()
+        while True:
+            cnt = C.c_uint64(0)
+            check(self._L.bht_batch_take_finished(self._h, buf, 64, C.byref(cnt)))
+            for i in range(cnt.value):
+                job = self._members.pop(buf[i])
+                job._fitted = True
+                out.append(job)
+            if cnt.value < 64:
+                return out
+
+    def run(self, chunk: int = 1 << 30) -> list:
+        """Steps until every member has finished; returns the jobs in completion order."""
+        done = []
+        while len(self):
+            self.step(chunk)
+            done += self.take_finished()
+        return done + self.take_finished()
+
+    def synchronize(self):
+        check(self._L.bht_batch_synchronize(self._h))
+
+    def close(self):
+        if self._h:
+            self._L.bht_batch_destroy(self._h)
+            self._h = None
+            self._members.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_footprint(n: int, d_in: int, d_out: int = 2, method: str = "fitsne", **params) -> int:
+    """Upper bound of the device bytes of one job (HBM admission of a request loop)."""
+    L = _lib.load()
+    p = _lib.Params()
+    L.bht_params_default(C.byref(p))
+    p.d_out = int(d_out)
+    for k, v in params.items():
+        setattr(p, k, v)
+    m = _lib.METHOD_FITSNE if method == "fitsne" else _lib.METHOD_BARNES_HUT
+    out = C.c_uint64(0)
+    check(L.bht_job_footprint(int(n), int(d_in), C.byref(p), m, C.byref(out)))
+    return int(out.value)
 
 
 # ---- operators (the reference's internal steps) -------------------------------------------------

@@ -90,6 +90,17 @@ SIGNATURES = {
                                         C.c_float, C.c_uint32, C.c_void_p]),
     "bht_symmetrize": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "bht_batch_create": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "bht_batch_destroy": (None, [C.c_void_p]),
+    "bht_batch_add": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float]),
+    "bht_batch_step": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "bht_batch_take_finished": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint64,
+                                            C.POINTER(C.c_uint64)]),
+    "bht_batch_size": (C.c_uint64, [C.c_void_p]),
+    "bht_batch_set_graph": (C.c_int32, [C.c_void_p, C.c_int]),
+    "bht_batch_synchronize": (C.c_int32, [C.c_void_p]),
+    "bht_job_footprint": (C.c_int32, [C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32,
+                                      C.POINTER(C.c_uint64)]),
 }
 
 _lib = None

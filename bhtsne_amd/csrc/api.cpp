@@ -11,13 +11,13 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
 
 #include "../../include/bhtsne_hip.h"
 #include "affinity.hpp"
-#include "bh.hpp"
 #include "internal.hpp"
 #include "reorder.hpp"
 
@@ -61,28 +61,44 @@ int64_t next_smooth(int64_t n) {  // interpolation.rs:364-378
 
 // ---- context ---------------------------------------------------------------------------------
 
+// Allocation accounting is shared by the jobs and batches of a context, which a request loop may
+// drive from two host threads (one preparing arriving jobs, one stepping the batch).
 struct bht_ctx {
     int device = 0;
     uint64_t budget = 0;
     uint64_t in_use = 0;
     TwiddleSet tw;
     hipStream_t stream = nullptr;
+    std::mutex mu;
 
     void *alloc(size_t bytes) {
         if (bytes == 0) bytes = 16;
-        if (budget && in_use + bytes > budget)
-            throw Error(ERR_OOM, "HBM budget exceeded: " + std::to_string(in_use + bytes) +
-                                     " bytes requested of " + std::to_string(budget));
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (budget && in_use + bytes > budget)
+                throw Error(ERR_OOM, "HBM budget exceeded: " + std::to_string(in_use + bytes) +
+                                         " bytes requested of " + std::to_string(budget));
+            in_use += bytes;
+        }
         void *p = nullptr;
-        BHT_HIP(hipSetDevice(device));
-        BHT_HIP(hipMalloc(&p, bytes));
-        in_use += bytes;
+        hipError_t e = hipSetDevice(device);
+        if (e == hipSuccess) e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            std::lock_guard<std::mutex> g(mu);
+            in_use -= bytes;
+            BHT_HIP(e);
+        }
         return p;
     }
     void release(void *p, size_t bytes) {
         if (!p) return;
         hipFree(p);
+        std::lock_guard<std::mutex> g(mu);
         in_use -= bytes ? bytes : 16;
+    }
+    uint64_t used() {
+        std::lock_guard<std::mutex> g(mu);
+        return in_use;
     }
 };
 
@@ -178,7 +194,8 @@ struct bht_job {
     float cached_perplexity = 0.0f;
     bool stop_lying_fired = false;
 
-    // optimiser state
+    // optimiser state; y[cur] is the embedding of the next epoch (cur = epoch parity, which the
+    // kernels read from the device epoch counter)
     DevBuf y[2], uy, gains, frep, pos;
     int cur = 0;
     bool have_init = false;
@@ -189,10 +206,12 @@ struct bht_job {
     DevBuf keys, svals, keys_alt, svals_alt, grid, spec, rker, kspec, part, dpart, mpart, fpart,
         sort_temp, scal;
     int nodes_cap = 0, len_cap = 0;
-    FitsneDev fd{};
     // Barnes-Hut workspace
     DevBuf codes, codes_alt, bidx, bidx_alt, bsort_temp, bnode, bmeta, brange, bhw2, bcount;
-    BhDev bd{};
+    // the job's entry of the device job table (host copy + device copy)
+    JobDev hd{};
+    DevBuf jd;
+    bool in_batch = false;
 
     // point renumbering: perm[i] = sample index of the engine's point i (identity until the
     // first reorder); qv[s] = P value at position s of the reference's CSR layout (s < n)
@@ -284,27 +303,16 @@ void ensure_fitsne_workspace(bht_job *j) {
     j->kspec.ensure(c, h * h * sizeof(float));
     const size_t nchunks = (n + 63) / 64;
     j->part.ensure(c, 2 * nchunks * 36 * sizeof(float));
-    FitsneDev &d = j->fd;
-    d.n = (int)n;
-    d.dim = (int)j->dim();
+    JobDev &d = j->hd;
     d.min_boxes = minb > MAX_BOXES ? MAX_BOXES : minb;
+    d.max_boxes = maxb;
     d.nodes_cap = j->nodes_cap;
     d.len_cap = j->len_cap;
-    d.keys = j->keys.as<uint32_t>();
-    d.vals = j->svals.as<uint32_t>();
-    d.keys_alt = j->keys_alt.as<uint32_t>();
-    d.vals_alt = j->svals_alt.as<uint32_t>();
     d.grid = j->grid.as<float4>();
     d.spec = j->spec.as<float2>();
     d.rker = j->rker.as<float>();
     d.kspec = j->kspec.as<float>();
     d.part = j->part.as<float>();
-    d.dpart = j->dpart.as<double>();
-    d.mpart = j->mpart.as<double>();
-    d.fpart = j->fpart.as<float>();
-    d.sort_temp = j->sort_temp.p;
-    d.scal = j->scal.as<EpochScalars>();
-    d.tw = &c->tw;
 }
 
 void ensure_bh_workspace(bht_job *j) {
@@ -321,28 +329,103 @@ void ensure_bh_workspace(bht_job *j) {
     j->brange.ensure(c, 4 * n * 4);
     j->bhw2.ensure(c, 64 * 4);
     j->bcount.ensure(c, 16);
-    BhDev &d = j->bd;
-    d.n = (int)j->n;
-    d.codes = j->codes.as<uint64_t>();
-    d.codes_alt = j->codes_alt.as<uint64_t>();
-    d.idx = j->bidx.as<uint32_t>();
-    d.idx_alt = j->bidx_alt.as<uint32_t>();
-    d.sort_temp = j->bsort_temp.p;
+    JobDev &d = j->hd;
     d.nodes.node = j->bnode.as<float4>();
     d.nodes.meta = j->bmeta.as<uint32_t>();
     d.nodes.range = j->brange.as<uint32_t>();
     d.nodes.hw2 = j->bhw2.as<float>();
     d.nodes.counts = j->bcount.as<uint32_t>();
+}
+
+float resolve_learning_rate(const bht_job *j) {  // lib.rs:652-659
+    if (j->params.learning_rate > 0.0f) return j->params.learning_rate;
+    float a = (float)j->n / j->params.early_exaggeration / 4.0f;
+    return std::max(a, 50.0f);
+}
+
+// Fills the job's table entry from its buffers and parameters.  A job alone sorts in its own
+// buffers; a batch points the sort fields at its arena afterwards.
+void fill_jobdev(bht_job *j) {
+    JobDev &d = j->hd;
+    d.n = (int)j->n;
+    d.dim = (int)j->dim();
+    d.method = j->method == FIT_BH ? M_BH : M_FITSNE;
+    d.theta_sq = j->theta * j->theta;
+    d.ybuf[0] = j->y[0].as<float>();
+    d.ybuf[1] = j->y[1].as<float>();
+    d.uy = j->uy.as<float>();
+    d.gains = j->gains.as<float>();
+    d.frep = j->frep.as<float>();
+    d.rows = j->rows.as<uint32_t>();
+    d.cols = j->cols.as<uint32_t>();
+    d.vals = j->vals.as<float>();
+    d.qv = j->qv.as<float>();
+    d.qv_n = (int)std::min<uint64_t>(j->n, j->nnz);
     d.scal = j->scal.as<EpochScalars>();
     d.dpart = j->dpart.as<double>();
+    d.mpart = j->mpart.as<double>();
+    d.fpart = j->fpart.as<float>();
+    d.learning_rate = resolve_learning_rate(j);
+    d.momentum = j->run_momentum;
+    d.final_momentum = j->params.final_momentum;
+    d.inv_exag = 1.0f / j->params.early_exaggeration;
+    d.momentum_switch_epoch =
+        (int)std::min<uint64_t>(j->params.momentum_switch_epoch, (uint64_t)INT32_MAX - 2);
+    d.stop_lying_epoch =
+        (int)std::min<uint64_t>(j->params.stop_lying_epoch, (uint64_t)INT32_MAX - 2);
+    if (d.method == M_FITSNE) {
+        const bool alt = sort_result_alt(box_key_bits(d.dim));
+        d.keys = j->keys.as<uint32_t>();
+        d.kidx = j->svals.as<uint32_t>();
+        d.skeys = alt ? j->keys_alt.as<uint32_t>() : j->keys.as<uint32_t>();
+        d.sidx = alt ? j->svals_alt.as<uint32_t>() : j->svals.as<uint32_t>();
+    } else {
+        const bool alt = sort_result_alt(MORTON_KEY_BITS);
+        d.codes = j->codes.as<uint64_t>();
+        d.cidx = j->bidx.as<uint32_t>();
+        d.scodes = alt ? j->codes_alt.as<uint64_t>() : j->codes.as<uint64_t>();
+        d.scidx = alt ? j->bidx_alt.as<uint32_t>() : j->bidx.as<uint32_t>();
+    }
+    job_block_counts(d, false);
+}
+
+// Refreshes the job's own device table entry (single-job launches).
+void sync_jobdev(bht_job *j) {
+    fill_jobdev(j);
+    j->jd.ensure(j->ctx, sizeof(JobDev));
+    BHT_HIP(hipMemcpyAsync(j->jd.p, &j->hd, sizeof(JobDev), hipMemcpyHostToDevice, j->stream));
+    BHT_HIP(hipStreamSynchronize(j->stream));
+}
+
+// Launch description of the job alone.
+LaunchSet single_set(bht_job *j) {
+    LaunchSet L;
+    L.jobs = j->jd.as<JobDev>();
+    for (int f = 0; f < NFAM; f++) L.nb[f] = j->hd.nblk[f];
+    L.dim = (int)j->dim();
+    L.njobs = 1;
+    L.tw = &j->ctx->tw;
+    L.fft_lds = (size_t)j->len_cap * sizeof(float2);
+    if (j->method == FIT_BH) {
+        L.keys = j->codes.p;
+        L.keys_alt = j->codes_alt.p;
+        L.vals = j->bidx.as<uint32_t>();
+        L.vals_alt = j->bidx_alt.as<uint32_t>();
+        L.sort_temp = j->bsort_temp.p;
+    } else {
+        L.keys = j->keys.p;
+        L.keys_alt = j->keys_alt.p;
+        L.vals = j->svals.as<uint32_t>();
+        L.vals_alt = j->svals_alt.as<uint32_t>();
+        L.sort_temp = j->sort_temp.p;
+    }
+    L.sort_n = j->n;
+    return L;
 }
 
 // Applies the recentring left pending by the last update (zero_mean, mod.rs:534-568) and writes
 // the per-axis bounds.
-void recenter(bht_job *j) {
-    center_only(j->ycur(), (int)j->n, (int)j->dim(), j->scal.as<EpochScalars>(),
-                j->fpart.as<float>(), j->mpart.as<double>(), j->stream);
-}
+void recenter(bht_job *j) { center_plan(single_set(j), false, j->stream); }
 
 // ---- point order -----------------------------------------------------------------------------
 
@@ -453,9 +536,11 @@ void reorder_points(bht_job *j, const uint32_t *order) {
     std::swap(j->cols.bytes, j->cols2.bytes);
     std::swap(j->vals.p, j->vals2.p);
     std::swap(j->vals.bytes, j->vals2.bytes);
-    // per-point state: embedding into the idle ping-pong buffer, uy and gains through ptmp
+    // per-point state: embedding through the idle ping-pong buffer (the buffers then trade
+    // places, so the epoch parity still selects the current one), uy and gains through ptmp
     gather_rows(j->ycur(), order, n, D, j->y[1 - j->cur].as<float>(), s);
-    j->cur = 1 - j->cur;
+    std::swap(j->y[0].p, j->y[1].p);
+    std::swap(j->y[0].bytes, j->y[1].bytes);
     gather_rows(j->uy.as<float>(), order, n, D, j->ptmp.as<float>(), s);
     std::swap(j->uy.p, j->ptmp.p);
     std::swap(j->uy.bytes, j->ptmp.bytes);
@@ -468,6 +553,7 @@ void reorder_points(bht_job *j, const uint32_t *order) {
     std::swap(j->perm.bytes, j->perm_tmp.bytes);
     j->permuted = true;
     j->reorders++;
+    sync_jobdev(j);
 }
 
 // Epochs after which the points are renumbered (large jobs only, where y outgrows an L2).
@@ -592,73 +678,44 @@ void prepare_run(bht_job *j, int method, float theta, bool cached) {
     j->epoch = 0;
     j->run_momentum = j->params.momentum;
     j->prepared = true;
-}
-
-float resolve_learning_rate(const bht_job *j) {  // lib.rs:652-659
-    if (j->params.learning_rate > 0.0f) return j->params.learning_rate;
-    float a = (float)j->n / j->params.early_exaggeration / 4.0f;
-    return std::max(a, 50.0f);
+    sync_jobdev(j);
 }
 
 // Repulsive forces and Z of the current embedding; returns the point indices in the spatial
 // order the repulsion sorted them into (device), or null.
 const uint32_t *repulsion(bht_job *j) {
-    FitsneDev &d = j->fd;
-    d.y = j->ycur();
-    d.frep = j->frep.as<float>();
+    const LaunchSet L = single_set(j);
     if (j->method == FIT_FITSNE) {
-        fitsne_center_plan(d, j->stream);
-        return fitsne_repulsion(d, j->stream);
+        center_plan(L, true, j->stream);
+        return fitsne_repulsion(L, j->stream);
     }
     // recentre + per-axis bounds (arena.rs:65-94), then the arena and its traversal
-    recenter(j);
-    BhDev &b = j->bd;
-    b.y = j->ycur();
-    b.frep = j->frep.as<float>();
-    if (j->n > 0) return bh_repulsion(b, j->theta, j->stream, CTR_BH);
+    center_plan(L, false, j->stream);
+    if (j->n > 0) return bh_repulsion(L, j->stream);
     return nullptr;
+}
+
+// Host bookkeeping after epoch `epoch` of a job (lib.rs:1162-1179); the device side of the
+// schedule (stop-lying rescale, momentum switch) follows the epoch counter by itself.
+void epoch_tail(bht_job *j, uint64_t epoch) {
+    if (epoch == j->params.stop_lying_epoch && epoch != 0) j->stop_lying_fired = true;
+    if (epoch == j->params.momentum_switch_epoch) j->params.momentum = j->params.final_momentum;
 }
 
 void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     if (!j->prepared) throw Error(ERR_STATE, "bht_job_prepare must precede bht_job_step");
+    if (j->in_batch) throw Error(ERR_STATE, "the job is advanced by its batch");
     const uint64_t len = j->n * j->dim();
     std::vector<float> snap(cb ? len : 0);
-    UpdateArgs a{};
-    a.n = (int)j->n;
-    a.dim = (int)j->dim();
-    a.frep = j->frep.as<float>();
-    a.learning_rate = resolve_learning_rate(j);
-    a.momentum = j->run_momentum;
-    a.final_momentum = j->params.final_momentum;
-    a.inv_exag = 1.0f / j->params.early_exaggeration;
-    a.momentum_switch_epoch =
-        (int)std::min<uint64_t>(j->params.momentum_switch_epoch, (uint64_t)INT32_MAX - 2);
-    a.stop_lying_epoch =
-        (int)std::min<uint64_t>(j->params.stop_lying_epoch, (uint64_t)INT32_MAX - 2);
-    a.scal = j->scal.as<EpochScalars>();
-    a.mpart = j->mpart.as<double>();
     for (uint64_t e = 0; e < count; e++) {
         j->mark();
         const uint32_t *order = repulsion(j);
         j->mark();
-        a.rows = j->rows.as<uint32_t>();
-        a.cols = j->cols.as<uint32_t>();
-        a.vals = j->vals.as<float>();
-        a.uy = j->uy.as<float>();
-        a.gains = j->gains.as<float>();
-        a.y = j->ycur();
-        a.y_next = j->y[1 - j->cur].as<float>();
-        attract_update(a, j->stream);
+        attract_update(single_set(j), j->stream);
         j->mark();
         j->cur = 1 - j->cur;
         const uint64_t epoch = j->epoch++;
-        if (epoch == j->params.stop_lying_epoch && epoch != 0) {
-            j->stop_lying_fired = true;
-            // the values at the reference's positions [0, n) follow the same rescale
-            scale_values(j->qv.as<float>(), std::min<uint64_t>(j->n, j->nnz),
-                         1.0f / j->params.early_exaggeration, j->stream);
-        }
-        if (epoch == j->params.momentum_switch_epoch) j->params.momentum = j->params.final_momentum;
+        epoch_tail(j, epoch);
         if (order && reorder_due(j, epoch)) reorder_points(j, order);
         if (cb) {
             recenter(j);
@@ -673,8 +730,11 @@ void finish_run(bht_job *j) {
     recenter(j);
     // stop-lying of the last epoch has not been applied lazily by a following epoch
     const uint64_t sl = j->params.stop_lying_epoch;
-    if (sl != 0 && j->epoch == sl + 1)
-        scale_values(j->vals.as<float>(), j->nnz, 1.0f / j->params.early_exaggeration, j->stream);
+    if (sl != 0 && j->epoch == sl + 1) {
+        const float inv = 1.0f / j->params.early_exaggeration;
+        scale_values(j->vals.as<float>(), j->nnz, inv, j->stream);
+        scale_values(j->qv.as<float>(), std::min<uint64_t>(j->n, j->nnz), inv, j->stream);
+    }
     BHT_HIP(hipStreamSynchronize(j->stream));
     j->fit_kind = j->method;
     j->fit_theta = j->theta;
@@ -756,6 +816,273 @@ void fit_with_neighbors(bht_job *j, int method, float theta, const uint64_t *idx
 
 }  // namespace
 
+// ---- batch: many jobs advanced by the same launches -------------------------------------------
+//
+// The members' table entries live in one device array.  Jobs of one method and output dimension
+// form a group; every epoch kernel of a group is one launch whose blocks are dealt out to the
+// group's jobs by a block map (job index << 16 | block of the job).  The per-epoch sorts of a
+// group run as one segmented sort over an arena holding every member's keys at a tile-aligned
+// offset.  The epoch's launch sequence depends only on the membership, so it is captured once
+// as a graph and replayed.
+
+namespace {
+
+struct BatchGroup {
+    int method = M_FITSNE, dim = 2;
+    std::vector<bht_job *> jobs;
+    std::vector<int> slots;  // table index of each member
+    LaunchSet L;
+    DevBuf keys, keys_alt, vals, vals_alt, sort_temp, segs, tile_seg, maps;
+};
+
+}  // namespace
+
+struct bht_batch {
+    bht_ctx *ctx;
+    hipStream_t stream = nullptr;
+    std::vector<bht_job *> members;
+    std::vector<bht_job *> finished;
+    std::vector<std::unique_ptr<BatchGroup>> groups;
+    DevBuf table;
+    bool dirty = true;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t epochs_launched = 0;
+
+    explicit bht_batch(bht_ctx *c) : ctx(c) {
+        BHT_HIP(hipSetDevice(ctx->device));
+        BHT_HIP(hipStreamCreate(&stream));
+    }
+    ~bht_batch() {
+        drop_graph();
+        if (stream) hipStreamDestroy(stream);
+    }
+    void drop_graph() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+namespace {
+
+uint64_t pad_tile(uint64_t n) {
+    return (n + RS_TILE_KEYS - 1) / RS_TILE_KEYS * RS_TILE_KEYS;
+}
+
+// Rebuilds the table, the groups' arenas and block maps after a membership change.
+void batch_rebuild(bht_batch *b) {
+    bht_ctx *c = b->ctx;
+    b->drop_graph();
+    b->groups.clear();
+    std::vector<JobDev> table(b->members.size());
+    for (size_t m = 0; m < b->members.size(); m++) {
+        bht_job *j = b->members[m];
+        const int method = j->method == FIT_BH ? M_BH : M_FITSNE;
+        BatchGroup *g = nullptr;
+        for (auto &q : b->groups)
+            if (q->method == method && q->dim == (int)j->dim()) g = q.get();
+        if (!g) {
+            b->groups.emplace_back(new BatchGroup());
+            g = b->groups.back().get();
+            g->method = method;
+            g->dim = (int)j->dim();
+        }
+        g->jobs.push_back(j);
+        g->slots.push_back((int)m);
+    }
+    for (auto &gp : b->groups) {
+        BatchGroup &g = *gp;
+        const bool bh = g.method == M_BH;
+        const size_t ksize = bh ? 8 : 4;
+        const int key_bits = bh ? MORTON_KEY_BITS : box_key_bits(g.dim);
+        const bool alt = sort_result_alt(key_bits);
+        // arena layout: one tile-aligned segment per member
+        std::vector<SortSeg> segs(g.jobs.size());
+        std::vector<uint32_t> tile_seg;
+        uint64_t base = 0, dense = 0;
+        for (size_t q = 0; q < g.jobs.size(); q++) {
+            const uint64_t n = g.jobs[q]->n;
+            const uint32_t nt = (uint32_t)(pad_tile(n) / RS_TILE_KEYS);
+            segs[q] = SortSeg{base, dense, (uint32_t)n, (uint32_t)tile_seg.size(), nt, 0};
+            for (uint32_t t = 0; t < nt; t++) tile_seg.push_back((uint32_t)q);
+            base += pad_tile(n);
+            dense += n;
+        }
+        const uint64_t arena = std::max<uint64_t>(base, RS_TILE_KEYS);
+        g.keys.ensure(c, arena * ksize);
+        g.keys_alt.ensure(c, arena * ksize);
+        g.vals.ensure(c, arena * 4);
+        g.vals_alt.ensure(c, arena * 4);
+        g.sort_temp.ensure(c, radix_sort_temp_bytes(arena));
+        g.segs.ensure(c, segs.size() * sizeof(SortSeg));
+        g.tile_seg.ensure(c, std::max<size_t>(tile_seg.size(), 1) * 4);
+        BHT_HIP(hipMemcpy(g.segs.p, segs.data(), segs.size() * sizeof(SortSeg),
+                          hipMemcpyHostToDevice));
+        if (!tile_seg.empty())
+            BHT_HIP(hipMemcpy(g.tile_seg.p, tile_seg.data(), tile_seg.size() * 4,
+                              hipMemcpyHostToDevice));
+        // table entries: batch grid sizes, sort fields into the arena
+        int len_cap = 1;
+        for (size_t q = 0; q < g.jobs.size(); q++) {
+            bht_job *j = g.jobs[q];
+            fill_jobdev(j);
+            JobDev d = j->hd;
+            job_block_counts(d, true);
+            const uint64_t o = segs[q].base;
+            if (bh) {
+                d.codes = g.keys.as<uint64_t>() + o;
+                d.cidx = g.vals.as<uint32_t>() + o;
+                d.scodes = (alt ? g.keys_alt.as<uint64_t>() : g.keys.as<uint64_t>()) + o;
+                d.scidx = (alt ? g.vals_alt.as<uint32_t>() : g.vals.as<uint32_t>()) + o;
+            } else {
+                d.keys = g.keys.as<uint32_t>() + o;
+                d.kidx = g.vals.as<uint32_t>() + o;
+                d.skeys = (alt ? g.keys_alt.as<uint32_t>() : g.keys.as<uint32_t>()) + o;
+                d.sidx = (alt ? g.vals_alt.as<uint32_t>() : g.vals.as<uint32_t>()) + o;
+            }
+            table[g.slots[q]] = d;
+            len_cap = std::max(len_cap, j->len_cap);
+        }
+        // block maps, family by family
+        std::vector<uint32_t> maps;
+        size_t off[NFAM];
+        int count[NFAM];
+        for (int f = 0; f < NFAM; f++) {
+            off[f] = maps.size();
+            for (size_t q = 0; q < g.jobs.size(); q++) {
+                const int nb = table[g.slots[q]].nblk[f];
+                for (int lb = 0; lb < nb; lb++)
+                    maps.push_back(((uint32_t)g.slots[q] << 16) | (uint32_t)lb);
+            }
+            count[f] = (int)(maps.size() - off[f]);
+        }
+        g.maps.ensure(c, std::max<size_t>(maps.size(), 1) * 4);
+        BHT_HIP(hipMemcpy(g.maps.p, maps.data(), maps.size() * 4, hipMemcpyHostToDevice));
+        LaunchSet &L = g.L;
+        for (int f = 0; f < NFAM; f++) {
+            L.map[f] = g.maps.as<uint32_t>() + off[f];
+            L.nb[f] = count[f];
+        }
+        L.dim = g.dim;
+        L.njobs = (int)g.jobs.size();
+        L.tw = &c->tw;
+        L.fft_lds = (size_t)len_cap * sizeof(float2);
+        L.keys = g.keys.p;
+        L.keys_alt = g.keys_alt.p;
+        L.vals = g.vals.as<uint32_t>();
+        L.vals_alt = g.vals_alt.as<uint32_t>();
+        L.sort_n = arena;
+        L.sort_temp = g.sort_temp.p;
+        L.segs = g.segs.as<SortSeg>();
+        L.tile_seg = g.tile_seg.as<uint32_t>();
+        L.sort_tiles = (uint32_t)tile_seg.size();
+    }
+    b->table.ensure(c, std::max<size_t>(table.size(), 1) * sizeof(JobDev));
+    if (!table.empty())
+        BHT_HIP(hipMemcpy(b->table.p, table.data(), table.size() * sizeof(JobDev),
+                          hipMemcpyHostToDevice));
+    for (auto &g : b->groups) g->L.jobs = b->table.as<JobDev>();
+    b->dirty = false;
+}
+
+// One epoch of every member: each group's repulsion, then its attraction + update.
+void batch_epoch_launches(bht_batch *b) {
+    for (auto &gp : b->groups) {
+        const LaunchSet &L = gp->L;
+        if (L.njobs == 0) continue;
+        if (gp->method == M_FITSNE) {
+            center_plan(L, true, b->stream);
+            fitsne_repulsion(L, b->stream);
+        } else {
+            center_plan(L, false, b->stream);
+            bh_repulsion(L, b->stream);
+        }
+        attract_update(L, b->stream);
+    }
+}
+
+void batch_launch_epoch(bht_batch *b) {
+    if (!b->use_graph) {
+        batch_epoch_launches(b);
+        return;
+    }
+    if (!b->exec) {
+        BHT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        batch_epoch_launches(b);
+        BHT_HIP(hipStreamEndCapture(b->stream, &b->graph));
+        BHT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    BHT_HIP(hipGraphLaunch(b->exec, b->stream));
+}
+
+// Validates and prepares a job for batched stepping (the fit's preamble, lib.rs:652-685).  A job
+// already prepared for this method at epoch 0 (bht_job_prepare, e.g. on another host thread
+// while the batch runs) joins as it is.
+void batch_admit(bht_batch *b, bht_job *j, int method, float theta) {
+    if (j->in_batch) throw Error(ERR_STATE, "the job already belongs to a batch");
+    if (j->ctx != b->ctx) throw Error(ERR_INVALID, "the job belongs to another context");
+    if (method == FIT_BH && !(theta > 0.0f)) throw Error(ERR_INVALID, MSG_THETA);
+    check_dims(j, method);
+    check_perplexity(j);
+    if (j->n >= (1u << 31) / 2) throw Error(ERR_INVALID, "batched jobs must have n < 2^30");
+    const bool ready = j->prepared && j->method == method && j->epoch == 0 &&
+                       (method != FIT_BH || j->theta == theta);
+    if (!ready) {
+        const bool cached = has_cached_affinities(j);
+        if (!cached) build_affinities_from_data(j);
+        validate_init(j);
+        prepare_run(j, method, theta, cached);
+    }
+    j->in_batch = true;
+    b->members.push_back(j);
+    b->dirty = true;
+}
+
+// Advances every member by up to max_epochs (never past the member with the fewest epochs
+// left), then retires the members whose runs are complete.  Returns the epochs advanced.
+uint64_t batch_step(bht_batch *b, uint64_t max_epochs) {
+    if (b->members.empty()) return 0;
+    uint64_t k = max_epochs;
+    for (bht_job *j : b->members) {
+        const uint64_t left = j->params.epochs > j->epoch ? j->params.epochs - j->epoch : 0;
+        k = std::min(k, left);
+    }
+    if (k > 0) {
+        if (b->dirty) batch_rebuild(b);
+        for (uint64_t e = 0; e < k; e++) batch_launch_epoch(b);
+        b->epochs_launched += k;
+        for (bht_job *j : b->members)
+            for (uint64_t e = 0; e < k; e++) {
+                j->cur = 1 - j->cur;
+                epoch_tail(j, j->epoch++);
+            }
+    }
+    // retire completed runs
+    std::vector<bht_job *> keep;
+    bool synced = false;
+    for (bht_job *j : b->members) {
+        if (j->epoch < j->params.epochs) {
+            keep.push_back(j);
+            continue;
+        }
+        if (!synced) {
+            BHT_HIP(hipStreamSynchronize(b->stream));
+            synced = true;
+        }
+        j->in_batch = false;
+        finish_run(j);
+        b->finished.push_back(j);
+        b->dirty = true;
+    }
+    b->members.swap(keep);
+    return k;
+}
+
+}  // namespace
+
 // ---- exported functions ----------------------------------------------------------------------
 
 extern "C" {
@@ -806,7 +1133,9 @@ void bht_ctx_destroy(bht_ctx *ctx) {
     delete ctx;
 }
 
-uint64_t bht_ctx_bytes_in_use(const bht_ctx *ctx) { return ctx ? ctx->in_use : 0; }
+uint64_t bht_ctx_bytes_in_use(const bht_ctx *ctx) {
+    return ctx ? const_cast<bht_ctx *>(ctx)->used() : 0;
+}
 
 bht_status bht_job_create(bht_ctx *ctx, uint64_t n_samples, const bht_params *params,
                           bht_job **out) {
@@ -1010,10 +1339,9 @@ bht_status bht_job_forces(bht_job *job, float *attractive, float *repulsive, flo
     if (!job) return fail(BHT_ERR_INVALID, "null argument");
     return guarded([&] {
         if (!job->prepared) throw Error(ERR_STATE, "bht_job_prepare must precede bht_job_forces");
+        if (job->in_batch) throw Error(ERR_STATE, "the job is advanced by its batch");
         repulsion(job);
-        attract_only((int)job->n, (int)job->dim(), job->rows.as<uint32_t>(),
-                     job->cols.as<uint32_t>(), job->vals.as<float>(), job->ycur(),
-                     job->pos.as<float>(), job->stream);
+        attract_only(single_set(job), job->pos.as<float>(), job->stream);
         EpochScalars sc;
         BHT_HIP(hipMemcpyAsync(&sc, job->scal.p, sizeof(sc), hipMemcpyDeviceToHost, job->stream));
         if (attractive) download_points(job, job->pos.as<float>(), attractive);
@@ -1225,6 +1553,100 @@ bht_status bht_symmetrize(bht_ctx *ctx, const uint32_t *cols, const float *vals,
         if (vals_out) BHT_HIP(hipMemcpy(vals_out, job.vals.p, m * 4, hipMemcpyDeviceToHost));
         *nnz = m;
     });
+}
+
+// ---- batches ---------------------------------------------------------------------------------
+
+bht_status bht_batch_create(bht_ctx *ctx, bht_batch **out) {
+    if (!ctx || !out) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] { *out = new bht_batch(ctx); });
+}
+
+void bht_batch_destroy(bht_batch *batch) {
+    if (!batch) return;
+    hipSetDevice(batch->ctx->device);
+    hipStreamSynchronize(batch->stream);
+    for (bht_job *j : batch->members) {
+        j->in_batch = false;
+        j->prepared = false;
+    }
+    delete batch;
+}
+
+bht_status bht_batch_add(bht_batch *batch, bht_job *job, uint32_t method, float theta) {
+    if (!batch || !job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (method != BHT_METHOD_FITSNE && method != BHT_METHOD_BARNES_HUT)
+            throw Error(ERR_INVALID, "unknown method");
+        batch_admit(batch, job, method == BHT_METHOD_FITSNE ? FIT_FITSNE : FIT_BH, theta);
+    });
+}
+
+bht_status bht_batch_step(bht_batch *batch, uint64_t max_epochs, uint64_t *epochs_advanced) {
+    if (!batch) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const uint64_t k = batch_step(batch, max_epochs);
+        if (epochs_advanced) *epochs_advanced = k;
+    });
+}
+
+bht_status bht_batch_take_finished(bht_batch *batch, bht_job **out, uint64_t capacity,
+                                   uint64_t *count) {
+    if (!batch || !count || (capacity && !out)) return fail(BHT_ERR_INVALID, "null argument");
+    const uint64_t m = std::min<uint64_t>(capacity, batch->finished.size());
+    for (uint64_t i = 0; i < m; i++) out[i] = batch->finished[i];
+    batch->finished.erase(batch->finished.begin(), batch->finished.begin() + (ptrdiff_t)m);
+    *count = m;
+    return BHT_OK;
+}
+
+uint64_t bht_batch_size(const bht_batch *batch) { return batch ? batch->members.size() : 0; }
+
+bht_status bht_batch_set_graph(bht_batch *batch, int enable) {
+    if (!batch) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        BHT_HIP(hipStreamSynchronize(batch->stream));
+        batch->drop_graph();
+        batch->use_graph = enable != 0;
+    });
+}
+
+bht_status bht_batch_synchronize(bht_batch *batch) {
+    if (!batch) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] { BHT_HIP(hipStreamSynchronize(batch->stream)); });
+}
+
+bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params, uint32_t method,
+                             uint64_t *bytes) {
+    if (!bytes) return fail(BHT_ERR_INVALID, "null argument");
+    bht_params p;
+    if (params) p = *params;
+    else bht_params_default(&p);
+    const uint64_t D = p.d_out ? p.d_out : 2;
+    const uint64_t k = p.perplexity > 0.0f ? (uint64_t)(3.0f * p.perplexity) : 0;
+    const uint64_t ne = n * k, nnz = 2 * ne;
+    uint64_t b = n * d_in * 4;                      // samples
+    b += (n + 1) * 4 + nnz * 8 + n * 4;             // CSR + KL weights
+    b += 6 * n * D * 4 + 4096 * 8 * 3 + 4096 * 16;  // optimiser state + partials
+    b += ne * 12 * 4 + radix_sort_temp_bytes(ne);   // affinity-build peak (transient)
+    if (method == BHT_METHOD_BARNES_HUT) {
+        b += n * (2 * 8 + 2 * 4) + radix_sort_temp_bytes(n);  // Morton sort
+        b += 2 * n * 16 + 2 * n * 4 + 4 * n * 4;               // tree
+        b += pad_tile(n) * 24 + radix_sort_temp_bytes(pad_tile(n));  // batch arena
+    } else {
+        uint32_t maxb = std::min<uint32_t>(std::max<uint32_t>(p.max_boxes, 1), MAX_BOXES);
+        maxb = std::max(maxb, std::min<uint32_t>(std::max<uint32_t>(p.min_boxes, 1), MAX_BOXES));
+        const uint64_t nodes = 3ull * maxb, len = (uint64_t)next_smooth(6 * (int64_t)maxb - 1);
+        const uint64_t h = len / 2 + 1;
+        b += n * 16 + radix_sort_temp_bytes(n);                   // box sort
+        b += (D == 2 ? nodes * nodes : nodes) * 16;               // node grid
+        if (D == 2) b += 2 * nodes * len * 8;                     // packed spectra
+        b += 2 * h * h * 4;                                       // kernel transforms
+        b += 2 * ((n + 63) / 64) * 36 * 4;                        // spread partials
+        b += pad_tile(n) * 16 + radix_sort_temp_bytes(pad_tile(n));  // batch arena
+    }
+    *bytes = b;
+    return BHT_OK;
 }
 
 }  // extern "C"

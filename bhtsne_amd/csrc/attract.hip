@@ -19,35 +19,45 @@ int attract_blocks(int n) {
 }
 
 template <int D, bool UPDATE>
-__global__ void __launch_bounds__(UP_THREADS) k_attract(UpdateArgs a, float *pos_out) {
+__global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, const uint32_t *map,
+                                                        float *pos_out) {
     __shared__ double red[UP_THREADS / 64];
+    const BlockCtx bc = block_ctx(map, jobs, F_ATT);
+    const JobDev &J = jobs[bc.job];
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * UP_THREADS + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * UP_THREADS) >> 6;
-    int epoch = 0;
-    float inv_z = 0.0f, momentum = a.momentum;
+    const int wave = (bc.lb * UP_THREADS + threadIdx.x) >> 6;
+    const int nwaves = (bc.nb * UP_THREADS) >> 6;
+    const int epoch = J.scal->epoch;
+    const int n = J.n;
+    const float *y = J.ybuf[epoch & 1];
+    float *y_next = J.ybuf[(epoch + 1) & 1];
+    const uint32_t *rows = J.rows, *cols = J.cols;
+    float *vals = J.vals;
+    const float inv_exag = J.inv_exag;
+    float inv_z = 0.0f, momentum = J.momentum;
     bool stop_now = false;
     if (UPDATE) {
-        epoch = a.scal->epoch;
-        inv_z = a.scal->inv_z;
+        inv_z = J.scal->inv_z;
         // stop_lying fires after epoch == stop_lying_epoch (lib.rs:1168-1171): the next epoch is
-        // the first to read the rescaled values, and rescales them in place as it reads.
-        stop_now = a.stop_lying_epoch != 0 && epoch == a.stop_lying_epoch + 1;
+        // the first to read the rescaled values, and rescales them in place as it reads (with
+        // the copies of the reference-position values that KL weights by).
+        stop_now = J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
         // the momentum switch likewise takes effect after epoch == momentum_switch_epoch
-        momentum = epoch > a.momentum_switch_epoch ? a.final_momentum : a.momentum;
+        momentum = epoch > J.momentum_switch_epoch ? J.final_momentum : J.momentum;
     }
     double m0 = 0.0, m1 = 0.0;
-    for (int i = wave; i < a.n; i += nwaves) {
+    for (int i = wave; i < n; i += nwaves) {
         float yi0, yi1 = 0.0f;
         if (D == 2) {
-            const float2 v = reinterpret_cast<const float2 *>(a.y)[i];
+            const float2 v = reinterpret_cast<const float2 *>(y)[i];
             yi0 = v.x;
             yi1 = v.y;
         } else {
-            yi0 = a.y[i];
+            yi0 = y[i];
         }
+        if (UPDATE && stop_now && lane == 0 && i < J.qv_n) J.qv[i] = J.qv[i] * inv_exag;
         float acc0 = 0.0f, acc1 = 0.0f;
-        const uint32_t e0 = a.rows[i], e1 = a.rows[i + 1];
+        const uint32_t e0 = rows[i], e1 = rows[i + 1];
         for (uint32_t base = e0; base < e1; base += 64 * UNROLL) {
             uint32_t col[UNROLL];
             float pv[UNROLL];
@@ -55,22 +65,22 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(UpdateArgs a, float *pos
             for (int u = 0; u < UNROLL; u++) {
                 const uint32_t e = base + lane + 64 * u;
                 const bool ok = e < e1;
-                col[u] = ok ? __builtin_nontemporal_load(a.cols + e) : (uint32_t)i;
-                pv[u] = ok ? __builtin_nontemporal_load(a.vals + e) : 0.0f;
+                col[u] = ok ? __builtin_nontemporal_load(cols + e) : (uint32_t)i;
+                pv[u] = ok ? __builtin_nontemporal_load(vals + e) : 0.0f;
                 if (UPDATE && stop_now && ok) {
-                    pv[u] = pv[u] * a.inv_exag;
-                    a.vals[e] = pv[u];
+                    pv[u] = pv[u] * inv_exag;
+                    vals[e] = pv[u];
                 }
             }
             float yj0[UNROLL], yj1[UNROLL];
 #pragma unroll
             for (int u = 0; u < UNROLL; u++) {
                 if (D == 2) {
-                    const float2 v = reinterpret_cast<const float2 *>(a.y)[col[u]];
+                    const float2 v = reinterpret_cast<const float2 *>(y)[col[u]];
                     yj0[u] = v.x;
                     yj1[u] = v.y;
                 } else {
-                    yj0[u] = a.y[col[u]];
+                    yj0[u] = y[col[u]];
                     yj1[u] = 0.0f;
                 }
             }
@@ -102,16 +112,16 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(UpdateArgs a, float *pos
 #pragma unroll
         for (int d = 0; d < D; d++) {
             const size_t k = (size_t)i * D + d;
-            const float gradient = acc[d] - a.frep[k] * inv_z;
-            float uy = a.uy[k];
-            float gain = a.gains[k];
+            const float gradient = acc[d] - J.frep[k] * inv_z;
+            float uy = J.uy[k];
+            float gain = J.gains[k];
             gain = rust_signum(gradient) != rust_signum(uy) ? gain + 0.2f : gain * 0.8f;
             if (gain < 0.01f) gain = 0.01f;
-            uy = momentum * uy - a.learning_rate * gain * gradient;
+            uy = momentum * uy - J.learning_rate * gain * gradient;
             const float yn = yi[d] + uy;
-            a.gains[k] = gain;
-            a.uy[k] = uy;
-            a.y_next[k] = yn;
+            J.gains[k] = gain;
+            J.uy[k] = uy;
+            y_next[k] = yn;
             if (d == 0) m0 += (double)yn;
             else m1 += (double)yn;
         }
@@ -121,38 +131,34 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(UpdateArgs a, float *pos
     double s0 = block_sum_f64(m0, red);
     double s1 = D == 2 ? block_sum_f64(m1, red) : 0.0;
     if (threadIdx.x == 0) {
-        a.mpart[2 * blockIdx.x] = s0;
-        a.mpart[2 * blockIdx.x + 1] = s1;
-        if (blockIdx.x == 0) {
-            a.scal->epoch_next = epoch + 1;
-            a.scal->mean_pending = 1;
+        J.mpart[2 * bc.lb] = s0;
+        J.mpart[2 * bc.lb + 1] = s1;
+        if (bc.lb == 0) {
+            J.scal->epoch_next = epoch + 1;
+            J.scal->mean_pending = 1;
         }
     }
 }
 
-void attract_update(const UpdateArgs &a, hipStream_t s) {
-    const int nb = attract_blocks(a.n);
-    if (a.dim == 2)
-        hipLaunchKernelGGL((k_attract<2, true>), dim3(nb), dim3(UP_THREADS), 0, s, a, nullptr);
+void attract_update(const LaunchSet &L, hipStream_t s) {
+    const dim3 g(L.nb[F_ATT]);
+    if (L.dim == 2)
+        hipLaunchKernelGGL((k_attract<2, true>), g, dim3(UP_THREADS), 0, s, L.jobs, L.map[F_ATT],
+                           nullptr);
     else
-        hipLaunchKernelGGL((k_attract<1, true>), dim3(nb), dim3(UP_THREADS), 0, s, a, nullptr);
+        hipLaunchKernelGGL((k_attract<1, true>), g, dim3(UP_THREADS), 0, s, L.jobs, L.map[F_ATT],
+                           nullptr);
     BHT_LAUNCH_CHECK();
 }
 
-void attract_only(int n, int dim, const uint32_t *rows, const uint32_t *cols, const float *vals,
-                  const float *y, float *pos, hipStream_t s) {
-    UpdateArgs a{};
-    a.n = n;
-    a.dim = dim;
-    a.rows = rows;
-    a.cols = cols;
-    a.vals = const_cast<float *>(vals);
-    a.y = y;
-    const int nb = attract_blocks(n);
-    if (dim == 2)
-        hipLaunchKernelGGL((k_attract<2, false>), dim3(nb), dim3(UP_THREADS), 0, s, a, pos);
+void attract_only(const LaunchSet &L, float *pos, hipStream_t s) {
+    const dim3 g(L.nb[F_ATT]);
+    if (L.dim == 2)
+        hipLaunchKernelGGL((k_attract<2, false>), g, dim3(UP_THREADS), 0, s, L.jobs, L.map[F_ATT],
+                           pos);
     else
-        hipLaunchKernelGGL((k_attract<1, false>), dim3(nb), dim3(UP_THREADS), 0, s, a, pos);
+        hipLaunchKernelGGL((k_attract<1, false>), g, dim3(UP_THREADS), 0, s, L.jobs, L.map[F_ATT],
+                           pos);
     BHT_LAUNCH_CHECK();
 }
 

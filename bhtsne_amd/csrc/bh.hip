@@ -13,7 +13,6 @@
 #include <cfloat>
 
 #include "internal.hpp"
-#include "bh.hpp"
 
 namespace bht {
 
@@ -43,8 +42,14 @@ __device__ __forceinline__ uint32_t quantize(float p, float mn, float inv_scale)
     return 0u;
 }
 
-__global__ void k_bh_codes(const float *y, int n, const EpochScalars *scal, uint64_t *codes,
-                           uint32_t *idx) {
+__global__ void __launch_bounds__(256) k_bh_codes(const JobDev *jobs, const uint32_t *map) {
+    const BlockCtx bc = block_ctx(map, jobs, F_PT);
+    const JobDev &J = jobs[bc.job];
+    const EpochScalars *scal = J.scal;
+    const float *y = job_y(J);
+    uint64_t *codes = J.codes;
+    uint32_t *idx = J.cidx;
+    const int n = J.n;
     const float scale = 4294967296.0f;  // 2^32 as f32
     float mn[2], inv[2];
 #pragma unroll
@@ -53,7 +58,7 @@ __global__ void k_bh_codes(const float *y, int n, const EpochScalars *scal, uint
         float extent = scal->amax[a] - scal->amin[a];
         inv[a] = extent > 0.0f ? scale / extent : 0.0f;
     }
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = bc.lb * 256 + threadIdx.x; i < n; i += bc.nb * 256) {
         float2 p = reinterpret_cast<const float2 *>(y)[i];
         uint32_t q0 = quantize(p.x, mn[0], inv[0]);
         uint32_t q1 = quantize(p.y, mn[1], inv[1]);
@@ -89,14 +94,20 @@ __device__ __forceinline__ uint32_t lower_bound_digit(const uint64_t *codes, uin
     return lo;
 }
 
-__global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const float *y, int n,
-                                                            const uint64_t *codes,
-                                                            const uint32_t *sidx,
-                                                            EpochScalars *scal, BhNodes t) {
+__global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
+                                                            const uint32_t *map) {
     __shared__ uint32_t sbuf[BUILD_THREADS];
     __shared__ uint32_t round_start[MAX_ROUNDS + 2];
     __shared__ uint32_t s_nn;
     __shared__ int s_rounds;
+    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
+    const JobDev &J = jobs[bc.job];
+    const float *y = job_y(J);
+    const int n = J.n;
+    const uint64_t *codes = J.scodes;
+    const uint32_t *sidx = J.scidx;
+    EpochScalars *scal = J.scal;
+    const BhNodes t = J.nodes;
     const int tid = threadIdx.x;
     if (tid == 0) {
         // level half-widths from the largest axis extent (arena.rs:167-174)
@@ -207,16 +218,20 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const float *y, int 
     if (tid == 0) t.counts[0] = s_nn;
 }
 
-__global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const float *y, int n,
-                                                              const uint32_t *sidx, BhNodes t,
-                                                              float theta_sq, float *frep,
-                                                              EpochScalars *scal,
-                                                              double *dpart, int slot) {
+__global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const JobDev *jobs,
+                                                              const uint32_t *map) {
     __shared__ uint32_t stack[STACK][TRAV_THREADS];
-    __shared__ double red[1];
+    const BlockCtx bc = block_ctx(map, jobs, F_TRAV);
+    const JobDev &J = jobs[bc.job];
+    const float *y = job_y(J);
+    const int n = J.n;
+    const uint32_t *sidx = J.scidx;
+    const BhNodes t = J.nodes;
+    const float theta_sq = J.theta_sq;
+    float *frep = J.frep;
     const int lane = threadIdx.x;
     double zacc = 0.0;
-    for (int s = blockIdx.x * TRAV_THREADS + lane; s < n; s += gridDim.x * TRAV_THREADS) {
+    for (int s = bc.lb * TRAV_THREADS + lane; s < n; s += bc.nb * TRAV_THREADS) {
         const uint32_t i = sidx[s];
         const float2 qp = reinterpret_cast<const float2 *>(y)[i];
         float fx = 0.0f, fy = 0.0f, q_sum = 0.0f;
@@ -252,27 +267,25 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const float *y, in
         zacc += (double)q_sum;
     }
     zacc = wave_sum_f64(zacc);
-    if (lane == 0) dpart[blockIdx.x] = zacc;
+    if (lane == 0) J.dpart[bc.lb] = zacc;
 }
 
-const uint32_t *bh_repulsion(const BhDev &d, float theta, hipStream_t s, int slot) {
-    const int n = d.n;
-    int nb = (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), 256), 1024);
-    hipLaunchKernelGGL(k_bh_codes, dim3(nb), dim3(256), 0, s, d.y, n, d.scal, d.codes, d.idx);
+int traverse_blocks(int n) {
+    return (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), TRAV_THREADS), 4096);
+}
+
+const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s) {
+    hipLaunchKernelGGL(k_bh_codes, dim3(L.nb[F_PT]), dim3(256), 0, s, L.jobs, L.map[F_PT]);
     BHT_LAUNCH_CHECK();
-    bool alt = radix_sort_pairs<uint64_t>(d.codes, d.idx, d.codes_alt, d.idx_alt, (uint64_t)n, 0,
-                                          64, d.sort_temp, s);
-    const uint64_t *sc = alt ? d.codes_alt : d.codes;
-    const uint32_t *si = alt ? d.idx_alt : d.idx;
-    hipLaunchKernelGGL(k_bh_build, dim3(1), dim3(BUILD_THREADS), 0, s, d.y, n, sc, si, d.scal,
-                       d.nodes);
-    const int tb = (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), TRAV_THREADS), 4096);
-    hipLaunchKernelGGL(k_bh_traverse, dim3(tb), dim3(TRAV_THREADS), 0, s, d.y, n, si, d.nodes,
-                       theta * theta, d.frep, d.scal, d.dpart, slot);
+    const bool alt = sort_group<uint64_t>(L, MORTON_KEY_BITS, s);
+    hipLaunchKernelGGL(k_bh_build, dim3(L.nb[F_ONE]), dim3(BUILD_THREADS), 0, s, L.jobs,
+                       L.map[F_ONE]);
+    hipLaunchKernelGGL(k_bh_traverse, dim3(L.nb[F_TRAV]), dim3(TRAV_THREADS), 0, s, L.jobs,
+                       L.map[F_TRAV]);
     BHT_LAUNCH_CHECK();
     // Z is the plain sum of the per-point q sums (repulsion.rs:155)
-    finalize_z(d.scal, d.dpart, tb, n, false, s);
-    return si;  // point indices in Morton order
+    finalize_z(L, F_TRAV, s);
+    return alt ? L.vals_alt : L.vals;  // point indices of job 0 in Morton order
 }
 
 }  // namespace bht

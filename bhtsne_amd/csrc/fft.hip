@@ -151,15 +151,19 @@ __device__ __forceinline__ int signed_lag(int c, int L) { return c <= L / 2 ? c 
 
 // Rows r1 = 2q, r2 = 2q + 1 of the squared Cauchy kernel (interpolation.rs:198-217), transformed
 // together; both transforms are real (even rows), so real and imaginary parts separate directly.
-__global__ void __launch_bounds__(FFT_THREADS) k_kernel_rows(const EpochScalars *scal,
-                                                             const float2 *tw_all, float *rker,
-                                                             int dim) {
+__global__ void __launch_bounds__(FFT_THREADS) k_kernel_rows(const JobDev *jobs,
+                                                             const uint32_t *map,
+                                                             const float2 *tw_all) {
     extern __shared__ float2 buf[];
-    const GridPlan p = scal->plan;
+    const BlockCtx bc = block_ctx(map, jobs, F_KSPEC);
+    const JobDev &J = jobs[bc.job];
+    const int dim = J.dim;
+    float *rker = dim == 2 ? J.rker : J.kspec;
+    const GridPlan p = J.scal->plan;
     const int L = p.len, h = L / 2 + 1;
     const float2 *tw = tw_all + p.tw_off;
     const int npairs = dim == 2 ? (h + 1) / 2 : 1;
-    for (int q = blockIdx.x; q < npairs; q += gridDim.x) {
+    for (int q = bc.lb; q < npairs; q += bc.nb) {
         const int r1 = 2 * q, r2 = 2 * q + 1;
         for (int c = threadIdx.x; c < L; c += FFT_THREADS) {
             float dc = p.spacing * (float)signed_lag(c, L);
@@ -194,16 +198,20 @@ __global__ void __launch_bounds__(FFT_THREADS) k_kernel_rows(const EpochScalars 
 
 // Columns v1 = 2q, v2 = 2q + 1 of the row transforms, extended evenly over r, transformed
 // together; the quarter spectrum is stored pre-scaled by 1 / len^2.
-__global__ void __launch_bounds__(FFT_THREADS) k_kernel_cols(const EpochScalars *scal,
-                                                             const float2 *tw_all,
-                                                             const float *rker, float *kspec) {
+__global__ void __launch_bounds__(FFT_THREADS) k_kernel_cols(const JobDev *jobs,
+                                                             const uint32_t *map,
+                                                             const float2 *tw_all) {
     extern __shared__ float2 buf[];
-    const GridPlan p = scal->plan;
+    const BlockCtx bc = block_ctx(map, jobs, F_KSPEC);
+    const JobDev &J = jobs[bc.job];
+    const float *rker = J.rker;
+    float *kspec = J.kspec;
+    const GridPlan p = J.scal->plan;
     const int L = p.len, h = L / 2 + 1;
     const float2 *tw = tw_all + p.tw_off;
     const float inv_total = 1.0f / (float)(L * L);
     const int npairs = (h + 1) / 2;
-    for (int q = blockIdx.x; q < npairs; q += gridDim.x) {
+    for (int q = bc.lb; q < npairs; q += bc.nb) {
         const int v1 = 2 * q, v2 = 2 * q + 1;
         for (int r = threadIdx.x; r < L; r += FFT_THREADS) {
             int rr = r <= L / 2 ? r : L - r;
@@ -224,17 +232,19 @@ __global__ void __launch_bounds__(FFT_THREADS) k_kernel_cols(const EpochScalars 
 // ---- data passes -----------------------------------------------------------------------------
 
 // Forward row transforms of the packed grids: line (g, r), r < nodes.
-__global__ void __launch_bounds__(FFT_THREADS) k_rows_fwd(const EpochScalars *scal,
-                                                          const float2 *tw_all,
-                                                          const float4 *grid, float2 *spec,
-                                                          size_t gstride) {
+__global__ void __launch_bounds__(FFT_THREADS) k_rows_fwd(const JobDev *jobs, const uint32_t *map,
+                                                          const float2 *tw_all) {
     extern __shared__ float2 buf[];
-    const GridPlan p = scal->plan;
+    const BlockCtx bc = block_ctx(map, jobs, F_ROWS);
+    const JobDev &J = jobs[bc.job];
+    float2 *spec = J.spec;
+    const size_t gstride = (size_t)J.nodes_cap * J.len_cap;
+    const GridPlan p = J.scal->plan;
     const int L = p.len, nodes = p.nodes;
     const float2 *tw = tw_all + p.tw_off;
-    const float2 *g2 = reinterpret_cast<const float2 *>(grid);
+    const float2 *g2 = reinterpret_cast<const float2 *>(J.grid);
     const float inv_q = 1.0f / p.qscale;
-    for (int line = blockIdx.x; line < 2 * nodes; line += gridDim.x) {
+    for (int line = bc.lb; line < 2 * nodes; line += bc.nb) {
         const int g = line / nodes, r = line % nodes;
         // (t0, t1 / q) and (t2 / q, t3 / q^2)
         const float sa = g == 0 ? 1.0f : inv_q, sb = g == 0 ? inv_q : inv_q * inv_q;
@@ -251,15 +261,18 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rows_fwd(const EpochScalars *sc
 }
 
 // Column pass: forward transform, multiply by the real kernel spectrum, inverse transform.
-__global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const EpochScalars *scal,
-                                                           const float2 *tw_all,
-                                                           const float *kspec, float2 *spec,
-                                                           size_t gstride) {
+__global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const JobDev *jobs, const uint32_t *map,
+                                                           const float2 *tw_all) {
     extern __shared__ float2 buf[];
-    const GridPlan p = scal->plan;
+    const BlockCtx bc = block_ctx(map, jobs, F_COLS);
+    const JobDev &J = jobs[bc.job];
+    const float *kspec = J.kspec;
+    float2 *spec = J.spec;
+    const size_t gstride = (size_t)J.nodes_cap * J.len_cap;
+    const GridPlan p = J.scal->plan;
     const int L = p.len, nodes = p.nodes, h = L / 2 + 1;
     const float2 *tw = tw_all + p.tw_off;
-    for (int line = blockIdx.x; line < 2 * L; line += gridDim.x) {
+    for (int line = bc.lb; line < 2 * L; line += bc.nb) {
         const int g = line / L, c = line % L;
         float2 *col = spec + g * gstride + c;
         for (int r = threadIdx.x; r < L; r += FFT_THREADS)
@@ -282,17 +295,19 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const EpochScalars *s
 
 // Inverse row transforms; the potentials of terms (t0, t1) / (t2, t3) land in the two halves of
 // the float4 node grid.
-__global__ void __launch_bounds__(FFT_THREADS) k_rows_inv(const EpochScalars *scal,
-                                                          const float2 *tw_all,
-                                                          const float2 *spec, float4 *grid,
-                                                          size_t gstride) {
+__global__ void __launch_bounds__(FFT_THREADS) k_rows_inv(const JobDev *jobs, const uint32_t *map,
+                                                          const float2 *tw_all) {
     extern __shared__ float2 buf[];
-    const GridPlan p = scal->plan;
+    const BlockCtx bc = block_ctx(map, jobs, F_ROWS);
+    const JobDev &J = jobs[bc.job];
+    const float2 *spec = J.spec;
+    const size_t gstride = (size_t)J.nodes_cap * J.len_cap;
+    const GridPlan p = J.scal->plan;
     const int L = p.len, nodes = p.nodes;
     const float2 *tw = tw_all + p.tw_off;
-    float2 *g2 = reinterpret_cast<float2 *>(grid);
+    float2 *g2 = reinterpret_cast<float2 *>(J.grid);
     const float q = p.qscale;
-    for (int line = blockIdx.x; line < 2 * nodes; line += gridDim.x) {
+    for (int line = bc.lb; line < 2 * nodes; line += bc.nb) {
         const int g = line / nodes, r = line % nodes;
         const float sa = g == 0 ? 1.0f : q, sb = g == 0 ? q : q * q;
         const float2 *in = spec + g * gstride + (size_t)r * L;
@@ -306,14 +321,18 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rows_inv(const EpochScalars *sc
 }
 
 // D = 1: one line, both packed signals (t0 + i t1, t2 + i 0) transformed, multiplied, inverted.
-__global__ void __launch_bounds__(FFT_THREADS) k_line_conv_1d(const EpochScalars *scal,
-                                                              const float2 *tw_all,
-                                                              const float *kspec, float4 *grid) {
+__global__ void __launch_bounds__(FFT_THREADS) k_line_conv_1d(const JobDev *jobs,
+                                                              const uint32_t *map,
+                                                              const float2 *tw_all) {
     extern __shared__ float2 buf[];
-    const GridPlan p = scal->plan;
+    const BlockCtx bc = block_ctx(map, jobs, F_LINE1);
+    const JobDev &J = jobs[bc.job];
+    const float *kspec = J.kspec;
+    float4 *grid = J.grid;
+    const GridPlan p = J.scal->plan;
     const int L = p.len, nodes = p.nodes;
     const float2 *tw = tw_all + p.tw_off;
-    const int g = blockIdx.x;  // 0: (t0, t1 / q), 1: (t2 / q^2, 0)
+    const int g = bc.lb;  // 0: (t0, t1 / q), 1: (t2 / q^2, 0)
     if (g > 1) return;
     const float q = p.qscale, inv_q = 1.0f / q;
     for (int c = threadIdx.x; c < L; c += FFT_THREADS) {
@@ -343,35 +362,30 @@ __global__ void __launch_bounds__(FFT_THREADS) k_line_conv_1d(const EpochScalars
     }
 }
 
-void fft_kernel_spectrum(const FitsneDev &d, hipStream_t s) {
-    const size_t lds = (size_t)d.len_cap * sizeof(float2);
-    const int hcap = d.len_cap / 2 + 1;
-    const int rows_blocks = std::max(1, std::min(1024, (hcap + 1) / 2));
-    hipLaunchKernelGGL(k_kernel_rows, dim3(rows_blocks), dim3(FFT_THREADS), lds, s, d.scal,
-                       d.tw->table, d.dim == 2 ? d.rker : d.kspec, d.dim);
-    if (d.dim == 2)
-        hipLaunchKernelGGL(k_kernel_cols, dim3(rows_blocks), dim3(FFT_THREADS), lds, s, d.scal,
-                           d.tw->table, d.rker, d.kspec);
+void fft_kernel_spectrum(const LaunchSet &L, hipStream_t s) {
+    const size_t lds = L.fft_lds;
+    hipLaunchKernelGGL(k_kernel_rows, dim3(L.nb[F_KSPEC]), dim3(FFT_THREADS), lds, s, L.jobs,
+                       L.map[F_KSPEC], L.tw->table);
+    if (L.dim == 2)
+        hipLaunchKernelGGL(k_kernel_cols, dim3(L.nb[F_KSPEC]), dim3(FFT_THREADS), lds, s, L.jobs,
+                           L.map[F_KSPEC], L.tw->table);
     BHT_LAUNCH_CHECK();
 }
 
-void fft_convolve(const FitsneDev &d, hipStream_t s) {
-    const size_t lds = (size_t)d.len_cap * sizeof(float2);
-    if (d.dim == 1) {
-        hipLaunchKernelGGL(k_line_conv_1d, dim3(2), dim3(FFT_THREADS), lds, s, d.scal,
-                           d.tw->table, d.kspec, d.grid);
+void fft_convolve(const LaunchSet &L, hipStream_t s) {
+    const size_t lds = L.fft_lds;
+    if (L.dim == 1) {
+        hipLaunchKernelGGL(k_line_conv_1d, dim3(L.nb[F_LINE1]), dim3(FFT_THREADS), lds, s, L.jobs,
+                           L.map[F_LINE1], L.tw->table);
         BHT_LAUNCH_CHECK();
         return;
     }
-    const size_t gstride = (size_t)d.nodes_cap * d.len_cap;
-    const int row_blocks = std::min(2048, 2 * d.nodes_cap);
-    const int col_blocks = std::min(2048, 2 * d.len_cap);
-    hipLaunchKernelGGL(k_rows_fwd, dim3(row_blocks), dim3(FFT_THREADS), lds, s, d.scal,
-                       d.tw->table, d.grid, d.spec, gstride);
-    hipLaunchKernelGGL(k_cols_conv, dim3(col_blocks), dim3(FFT_THREADS), lds, s, d.scal,
-                       d.tw->table, d.kspec, d.spec, gstride);
-    hipLaunchKernelGGL(k_rows_inv, dim3(row_blocks), dim3(FFT_THREADS), lds, s, d.scal,
-                       d.tw->table, d.spec, d.grid, gstride);
+    hipLaunchKernelGGL(k_rows_fwd, dim3(L.nb[F_ROWS]), dim3(FFT_THREADS), lds, s, L.jobs,
+                       L.map[F_ROWS], L.tw->table);
+    hipLaunchKernelGGL(k_cols_conv, dim3(L.nb[F_COLS]), dim3(FFT_THREADS), lds, s, L.jobs,
+                       L.map[F_COLS], L.tw->table);
+    hipLaunchKernelGGL(k_rows_inv, dim3(L.nb[F_ROWS]), dim3(FFT_THREADS), lds, s, L.jobs,
+                       L.map[F_ROWS], L.tw->table);
     BHT_LAUNCH_CHECK();
 }
 

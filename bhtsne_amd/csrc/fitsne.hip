@@ -18,6 +18,33 @@ int point_blocks(int n) {
     return b < 1024 ? b : 1024;
 }
 
+int spread_blocks(int n) {
+    const uint64_t nchunks = ceil_div((uint64_t)(n > 0 ? n : 1), 64);
+    return (int)std::min<uint64_t>(ceil_div(nchunks, PT_THREADS / 64), 2048);
+}
+
+// Grid sizes of every kernel family for one job.  A job alone fills the device by itself; in a
+// batch the jobs share it, so the grid-bound families (clearing, transforms) take fewer blocks
+// per job and stride further.
+void job_block_counts(JobDev &J, bool batched) {
+    const int n = J.n;
+    J.nblk[F_PT] = point_blocks(n);
+    J.nblk[F_ONE] = 1;
+    J.nblk[F_SPR] = spread_blocks(n);
+    J.nblk[F_ATT] = attract_blocks(n);
+    J.nblk[F_TRAV] = traverse_blocks(n);
+    J.nblk[F_LINE1] = 2;
+    const int cap = batched ? 128 : 2048;
+    const int64_t nodes_total =
+        J.dim == 2 ? (int64_t)J.nodes_cap * J.nodes_cap : (int64_t)J.nodes_cap;
+    J.nblk[F_ZERO] = (int)std::max<int64_t>(1, std::min<int64_t>(batched ? 64 : 1024,
+                                                                 (nodes_total + 255) / 256));
+    const int hcap = J.len_cap / 2 + 1;
+    J.nblk[F_KSPEC] = std::max(1, std::min(batched ? cap : 1024, (hcap + 1) / 2));
+    J.nblk[F_ROWS] = std::max(1, std::min(cap, 2 * J.nodes_cap));
+    J.nblk[F_COLS] = std::max(1, std::min(cap, 2 * J.len_cap));
+}
+
 // ---- per-point helpers (interpolation.rs:162-187, 400-456) ---------------------------------
 
 struct Lagrange3 {
@@ -64,10 +91,15 @@ __device__ __forceinline__ int locate_box(float v, const GridPlan &p, float &loc
 // Three small kernels instead of one with a grid-wide "last block" reduction: a device-scope fence
 // in every block would write back the XCD's L2 each time.  Kernel boundaries give the ordering.
 
-// Mean of the last update's per-block sums (written by the attraction kernel), applied once.
-__global__ void __launch_bounds__(256) k_mean_finalize(EpochScalars *scal, const double *mpart,
-                                                       int nparts, int n) {
+// Mean of the last update's per-block sums (written by the attraction kernel), applied once;
+// advances the job's epoch index (which selects its current embedding buffer).
+__global__ void __launch_bounds__(256) k_mean_finalize(const JobDev *jobs, const uint32_t *map) {
     __shared__ double red[4];
+    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
+    const JobDev &J = jobs[bc.job];
+    EpochScalars *scal = J.scal;
+    const double *mpart = J.mpart;
+    const int nparts = J.nblk[F_ATT], n = J.n;
     const int pending = scal->mean_pending;
     if (!pending) {
         if (threadIdx.x == 0) {
@@ -93,13 +125,17 @@ __global__ void __launch_bounds__(256) k_mean_finalize(EpochScalars *scal, const
 
 // y -= mean; per-block, per-axis min / max.
 template <int D>
-__global__ void __launch_bounds__(PT_THREADS) k_center(float *y, int n, const EpochScalars *scal,
-                                                       float *fpart) {
+__global__ void __launch_bounds__(PT_THREADS) k_center(const JobDev *jobs, const uint32_t *map) {
     __shared__ float red[4][PT_THREADS / 64];
+    const BlockCtx bc = block_ctx(map, jobs, F_PT);
+    const JobDev &J = jobs[bc.job];
+    const EpochScalars *scal = J.scal;
+    float *y = job_y(J);
+    const int n = J.n;
     const float m0 = scal->mean[0], m1 = D == 2 ? scal->mean[1] : 0.0f;
     float mn0 = INFINITY, mn1 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY;
-    const int stride = gridDim.x * PT_THREADS;
-    for (int i = blockIdx.x * PT_THREADS + threadIdx.x; i < n; i += stride) {
+    const int stride = bc.nb * PT_THREADS;
+    for (int i = bc.lb * PT_THREADS + threadIdx.x; i < n; i += stride) {
         if (D == 2) {
             float2 v = reinterpret_cast<float2 *>(y)[i];
             v.x = v.x - m0;
@@ -123,17 +159,21 @@ __global__ void __launch_bounds__(PT_THREADS) k_center(float *y, int n, const Ep
             mn0 = fminf(mn0, red[0][q]); mx0 = fmaxf(mx0, red[1][q]);
             mn1 = fminf(mn1, red[2][q]); mx1 = fmaxf(mx1, red[3][q]);
         }
-        reinterpret_cast<float4 *>(fpart)[blockIdx.x] = make_float4(mn0, mx0, mn1, mx1);
+        reinterpret_cast<float4 *>(J.fpart)[bc.lb] = make_float4(mn0, mx0, mn1, mx1);
     }
 }
 
 // Combines the bounds; with do_plan, decides the FIt-SNE grid of this epoch.
 template <int D>
-__global__ void __launch_bounds__(256) k_plan(EpochScalars *scal, const float *fpart, int nparts,
-                                              int min_boxes, int max_boxes,
+__global__ void __launch_bounds__(256) k_plan(const JobDev *jobs, const uint32_t *map,
                                               const int *len_of_boxes, const int *off_of_boxes,
                                               int do_plan) {
     __shared__ float red[4][4];
+    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
+    const JobDev &J = jobs[bc.job];
+    EpochScalars *scal = J.scal;
+    const float *fpart = J.fpart;
+    const int nparts = J.nblk[F_PT], min_boxes = J.min_boxes, max_boxes = J.max_boxes;
     float mn0 = INFINITY, mx0 = -INFINITY, mn1 = INFINITY, mx1 = -INFINITY;
     for (int b = threadIdx.x; b < nparts; b += 256) {
         float4 pv = reinterpret_cast<const float4 *>(fpart)[b];
@@ -152,7 +192,7 @@ __global__ void __launch_bounds__(256) k_plan(EpochScalars *scal, const float *f
     }
     scal->amin[0] = mn0; scal->amax[0] = mx0;
     scal->amin[1] = mn1; scal->amax[1] = mx1;
-    if (!do_plan) return;
+    if (!do_plan || J.method != M_FITSNE) return;
     // coordinate_bounds + box_count + next_smooth (interpolation.rs:115-128, 337-378)
     float lo = D == 2 ? fminf(mn0, mn1) : mn0;
     float hi = D == 2 ? fmaxf(mx0, mx1) : mx0;
@@ -183,67 +223,63 @@ __global__ void __launch_bounds__(256) k_plan(EpochScalars *scal, const float *f
     scal->plan = p;
 }
 
-static void center_launch(float *y, int n, int dim, EpochScalars *scal, float *fpart,
-                          const double *mpart, int min_boxes, int max_boxes,
-                          const TwiddleSet *tw, hipStream_t s) {
-    const int nb = point_blocks(n);
-    hipLaunchKernelGGL(k_mean_finalize, dim3(1), dim3(256), 0, s, scal, mpart, attract_blocks(n),
-                       n);
-    const int do_plan = tw != nullptr;
-    const int *lo = tw ? tw->len_of_boxes : nullptr, *of = tw ? tw->off_of_boxes : nullptr;
-    if (dim == 2) {
-        hipLaunchKernelGGL(k_center<2>, dim3(nb), dim3(PT_THREADS), 0, s, y, n, scal, fpart);
-        hipLaunchKernelGGL(k_plan<2>, dim3(1), dim3(256), 0, s, scal, fpart, nb, min_boxes,
-                           max_boxes, lo, of, do_plan);
+void center_plan(const LaunchSet &L, bool do_plan, hipStream_t s) {
+    const int *lo = L.tw ? L.tw->len_of_boxes : nullptr, *of = L.tw ? L.tw->off_of_boxes : nullptr;
+    const int plan = do_plan && L.tw != nullptr;
+    hipLaunchKernelGGL(k_mean_finalize, dim3(L.nb[F_ONE]), dim3(256), 0, s, L.jobs, L.map[F_ONE]);
+    if (L.dim == 2) {
+        hipLaunchKernelGGL(k_center<2>, dim3(L.nb[F_PT]), dim3(PT_THREADS), 0, s, L.jobs,
+                           L.map[F_PT]);
+        hipLaunchKernelGGL(k_plan<2>, dim3(L.nb[F_ONE]), dim3(256), 0, s, L.jobs, L.map[F_ONE], lo,
+                           of, plan);
     } else {
-        hipLaunchKernelGGL(k_center<1>, dim3(nb), dim3(PT_THREADS), 0, s, y, n, scal, fpart);
-        hipLaunchKernelGGL(k_plan<1>, dim3(1), dim3(256), 0, s, scal, fpart, nb, min_boxes,
-                           max_boxes, lo, of, do_plan);
+        hipLaunchKernelGGL(k_center<1>, dim3(L.nb[F_PT]), dim3(PT_THREADS), 0, s, L.jobs,
+                           L.map[F_PT]);
+        hipLaunchKernelGGL(k_plan<1>, dim3(L.nb[F_ONE]), dim3(256), 0, s, L.jobs, L.map[F_ONE], lo,
+                           of, plan);
     }
     BHT_LAUNCH_CHECK();
 }
 
-void fitsne_center_plan(const FitsneDev &d, hipStream_t s) {
-    center_launch(d.y, d.n, d.dim, d.scal, d.fpart, d.mpart, d.min_boxes, d.nodes_cap / IP, d.tw,
-                  s);
-}
-
-void center_only(float *y, int n, int dim, EpochScalars *scal, float *fpart, const double *mpart,
-                 hipStream_t s) {
-    center_launch(y, n, dim, scal, fpart, mpart, 10, MAX_BOXES, nullptr, s);
-}
-
-// Z = sum of the per-block sums (minus the n self-interactions for FIt-SNE, interpolation.rs:331).
-__global__ void __launch_bounds__(256) k_z_finalize(EpochScalars *scal, const double *dpart,
-                                                    int nparts, int n, int subtract_n) {
+// Z = sum of the per-block sums (minus the n self-interactions for FIt-SNE, interpolation.rs:331;
+// the plain sum of the q sums for Barnes-Hut, repulsion.rs:155).
+__global__ void __launch_bounds__(256) k_z_finalize(const JobDev *jobs, const uint32_t *map,
+                                                    int part_family) {
     __shared__ double red[4];
+    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
+    const JobDev &J = jobs[bc.job];
+    EpochScalars *scal = J.scal;
+    const int nparts = J.nblk[part_family];
     double t = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += 256) t += dpart[b];
+    for (int b = threadIdx.x; b < nparts; b += 256) t += J.dpart[b];
     t = block_sum_f64(t, red);
     if (threadIdx.x == 0) {
         scal->zsum = t;
-        const float z = subtract_n ? (float)(t - (double)n) : (float)t;
+        const float z = J.method == M_FITSNE ? (float)(t - (double)J.n) : (float)t;
         scal->z = z;
         scal->inv_z = 1.0f / z;
     }
 }
 
-void finalize_z(EpochScalars *scal, const double *dpart, int nparts, int n, bool subtract_n,
-                hipStream_t s) {
-    hipLaunchKernelGGL(k_z_finalize, dim3(1), dim3(256), 0, s, scal, dpart, nparts, n,
-                       subtract_n ? 1 : 0);
+void finalize_z(const LaunchSet &L, int part_family, hipStream_t s) {
+    hipLaunchKernelGGL(k_z_finalize, dim3(L.nb[F_ONE]), dim3(256), 0, s, L.jobs, L.map[F_ONE],
+                       part_family);
     BHT_LAUNCH_CHECK();
 }
 
 // ---- 2. box keys ---------------------------------------------------------------------------
 
 template <int D>
-__global__ void __launch_bounds__(PT_THREADS) k_box_keys(const float *y, int n,
-                                                         const EpochScalars *scal,
-                                                         uint32_t *keys, uint32_t *vals) {
-    const GridPlan p = scal->plan;
-    const int stride = gridDim.x * PT_THREADS;
-    for (int i = blockIdx.x * PT_THREADS + threadIdx.x; i < n; i += stride) {
+__global__ void __launch_bounds__(PT_THREADS) k_box_keys(const JobDev *jobs,
+                                                         const uint32_t *map) {
+    const BlockCtx bc = block_ctx(map, jobs, F_PT);
+    const JobDev &J = jobs[bc.job];
+    const GridPlan p = J.scal->plan;
+    const float *y = job_y(J);
+    uint32_t *keys = J.keys, *vals = J.kidx;
+    const int n = J.n;
+    const int stride = bc.nb * PT_THREADS;
+    for (int i = bc.lb * PT_THREADS + threadIdx.x; i < n; i += stride) {
         float l;
         uint32_t key;
         if (D == 2) {
@@ -259,10 +295,13 @@ __global__ void __launch_bounds__(PT_THREADS) k_box_keys(const float *y, int n,
     }
 }
 
-__global__ void k_zero_grid(float4 *grid, const EpochScalars *scal, int dim) {
-    const GridPlan p = scal->plan;
-    int total = dim == 2 ? p.nodes * p.nodes : p.nodes;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+__global__ void __launch_bounds__(256) k_zero_grid(const JobDev *jobs, const uint32_t *map) {
+    const BlockCtx bc = block_ctx(map, jobs, F_ZERO);
+    const JobDev &J = jobs[bc.job];
+    const GridPlan p = J.scal->plan;
+    float4 *grid = J.grid;
+    const int total = J.dim == 2 ? p.nodes * p.nodes : p.nodes;
+    for (int i = bc.lb * 256 + threadIdx.x; i < total; i += bc.nb * 256)
         grid[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -332,16 +371,21 @@ __device__ __forceinline__ void write_box(float4 *grid, const GridPlan &p, uint3
 }
 
 template <int D>
-__global__ void __launch_bounds__(PT_THREADS) k_spread(const float *y, const uint32_t *keys,
-                                                       const uint32_t *vals, int n,
-                                                       const EpochScalars *scal, float4 *grid,
-                                                       float *part, int nchunks) {
+__global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const uint32_t *map) {
     constexpr int NV = D == 2 ? NV2 : NV1;
-    const GridPlan p = scal->plan;
+    const BlockCtx bc = block_ctx(map, jobs, F_SPR);
+    const JobDev &J = jobs[bc.job];
+    const GridPlan p = J.scal->plan;
+    const float *y = job_y(J);
+    const uint32_t *keys = J.skeys, *vals = J.sidx;
+    float4 *grid = J.grid;
+    float *part = J.part;
+    const int n = J.n;
+    const int nchunks = (n + 63) / 64;
     const Lagrange3 lb;
     const int lane = threadIdx.x & 63;
-    const int wave0 = (blockIdx.x * PT_THREADS + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * PT_THREADS) >> 6;
+    const int wave0 = (bc.lb * PT_THREADS + threadIdx.x) >> 6;
+    const int nwaves = (bc.nb * PT_THREADS) >> 6;
     float *part_first = part;
     float *part_last = part + (size_t)nchunks * NV;
     for (int c = wave0; c < nchunks; c += nwaves) {
@@ -390,17 +434,21 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const float *y, const uin
 // by binary search of the sorted keys and sums the chunk partials with the lanes (strided, then a
 // fixed butterfly), so a run of any length costs O(length / 64 / 64) steps.
 template <int D>
-__global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const uint32_t *keys, int n,
-                                                             const EpochScalars *scal,
-                                                             float4 *grid, const float *part,
-                                                             int nchunks) {
+__global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const JobDev *jobs,
+                                                             const uint32_t *map) {
     constexpr int NV = D == 2 ? NV2 : NV1;
-    const GridPlan p = scal->plan;
-    const float *part_first = part;
-    const float *part_last = part + (size_t)nchunks * NV;
+    const BlockCtx bc = block_ctx(map, jobs, F_SPR);
+    const JobDev &J = jobs[bc.job];
+    const GridPlan p = J.scal->plan;
+    const uint32_t *keys = J.skeys;
+    float4 *grid = J.grid;
+    const int n = J.n;
+    const int nchunks = (n + 63) / 64;
+    const float *part_first = J.part;
+    const float *part_last = J.part + (size_t)nchunks * NV;
     const int lane = threadIdx.x & 63;
-    const int wave0 = (blockIdx.x * PT_THREADS + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * PT_THREADS) >> 6;
+    const int wave0 = (bc.lb * PT_THREADS + threadIdx.x) >> 6;
+    const int nwaves = (bc.nb * PT_THREADS) >> 6;
     for (int c = wave0; c < nchunks; c += nwaves) {
         const int last_s = min(c * 64 + 63, n - 1);
         const uint32_t key = keys[last_s];
@@ -435,15 +483,19 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const uint32_t *key
 // ---- 5. gather + reconstruct (interpolation.rs:282-332) -------------------------------------
 
 template <int D>
-__global__ void __launch_bounds__(PT_THREADS) k_gather(const float *y, int n,
-                                                       EpochScalars *scal, const float4 *grid,
-                                                       float *frep, double *dpart) {
+__global__ void __launch_bounds__(PT_THREADS) k_gather(const JobDev *jobs, const uint32_t *map) {
     __shared__ double red[PT_THREADS / 64];
-    const GridPlan p = scal->plan;
+    const BlockCtx bc = block_ctx(map, jobs, F_PT);
+    const JobDev &J = jobs[bc.job];
+    const GridPlan p = J.scal->plan;
+    const float *y = job_y(J);
+    const float4 *grid = J.grid;
+    float *frep = J.frep;
+    const int n = J.n;
     const Lagrange3 lb;
     double zacc = 0.0;
-    const int stride = gridDim.x * PT_THREADS;
-    for (int i = blockIdx.x * PT_THREADS + threadIdx.x; i < n; i += stride) {
+    const int stride = bc.nb * PT_THREADS;
+    for (int i = bc.lb * PT_THREADS + threadIdx.x; i < n; i += stride) {
         if (D == 2) {
             float2 pt = reinterpret_cast<const float2 *>(y)[i];
             float l0, l1, w0[3], w1[3];
@@ -495,53 +547,38 @@ __global__ void __launch_bounds__(PT_THREADS) k_gather(const float *y, int n,
         }
     }
     double bs = block_sum_f64(zacc, red);
-    if (threadIdx.x == 0) dpart[blockIdx.x] = bs;
+    if (threadIdx.x == 0) J.dpart[bc.lb] = bs;
 }
 
 // ---- driver ----------------------------------------------------------------------------------
 
-const uint32_t *fitsne_repulsion(FitsneDev &d, hipStream_t s) {
-    const int nb = point_blocks(d.n);
-    const int nchunks = (int)ceil_div(d.n, 64);
-    const int spread_blocks = (int)std::min<uint64_t>(ceil_div(nchunks, PT_THREADS / 64), 2048);
-    const int fix_blocks = spread_blocks;
-    const int zero_blocks = 1024;
-    if (d.dim == 2)
-        hipLaunchKernelGGL(k_box_keys<2>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
-                           d.keys, d.vals);
+const uint32_t *fitsne_repulsion(const LaunchSet &L, hipStream_t s) {
+    const dim3 pt(L.nb[F_PT]), spr(L.nb[F_SPR]), zero(L.nb[F_ZERO]);
+    if (L.dim == 2)
+        hipLaunchKernelGGL(k_box_keys<2>, pt, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_PT]);
     else
-        hipLaunchKernelGGL(k_box_keys<1>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
-                           d.keys, d.vals);
+        hipLaunchKernelGGL(k_box_keys<1>, pt, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_PT]);
     BHT_LAUNCH_CHECK();
-    // boxes^D <= 10^6 < 2^20: three 8-bit passes
-    bool alt = radix_sort_pairs<uint32_t>(d.keys, d.vals, d.keys_alt, d.vals_alt, (uint64_t)d.n,
-                                          0, d.dim == 2 ? 24 : 16, d.sort_temp, s);
-    const uint32_t *sk = alt ? d.keys_alt : d.keys;
-    const uint32_t *sv = alt ? d.vals_alt : d.vals;
-    hipLaunchKernelGGL(k_zero_grid, dim3(zero_blocks), dim3(256), 0, s, d.grid, d.scal, d.dim);
-    if (d.dim == 2) {
-        hipLaunchKernelGGL(k_spread<2>, dim3(spread_blocks), dim3(PT_THREADS), 0, s, d.y, sk, sv,
-                           d.n, d.scal, d.grid, d.part, nchunks);
-        hipLaunchKernelGGL(k_spread_fixup<2>, dim3(fix_blocks), dim3(PT_THREADS), 0, s, sk, d.n,
-                           d.scal, d.grid, d.part, nchunks);
+    const bool alt = sort_group<uint32_t>(L, box_key_bits(L.dim), s);
+    hipLaunchKernelGGL(k_zero_grid, zero, dim3(256), 0, s, L.jobs, L.map[F_ZERO]);
+    if (L.dim == 2) {
+        hipLaunchKernelGGL(k_spread<2>, spr, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_SPR]);
+        hipLaunchKernelGGL(k_spread_fixup<2>, spr, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_SPR]);
     } else {
-        hipLaunchKernelGGL(k_spread<1>, dim3(spread_blocks), dim3(PT_THREADS), 0, s, d.y, sk, sv,
-                           d.n, d.scal, d.grid, d.part, nchunks);
-        hipLaunchKernelGGL(k_spread_fixup<1>, dim3(fix_blocks), dim3(PT_THREADS), 0, s, sk, d.n,
-                           d.scal, d.grid, d.part, nchunks);
+        hipLaunchKernelGGL(k_spread<1>, spr, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_SPR]);
+        hipLaunchKernelGGL(k_spread_fixup<1>, spr, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_SPR]);
     }
     BHT_LAUNCH_CHECK();
-    fft_kernel_spectrum(d, s);
-    fft_convolve(d, s);
-    if (d.dim == 2)
-        hipLaunchKernelGGL(k_gather<2>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
-                           d.grid, d.frep, d.dpart);
+    fft_kernel_spectrum(L, s);
+    fft_convolve(L, s);
+    if (L.dim == 2)
+        hipLaunchKernelGGL(k_gather<2>, pt, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_PT]);
     else
-        hipLaunchKernelGGL(k_gather<1>, dim3(nb), dim3(PT_THREADS), 0, s, d.y, d.n, d.scal,
-                           d.grid, d.frep, d.dpart);
+        hipLaunchKernelGGL(k_gather<1>, pt, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_PT]);
     BHT_LAUNCH_CHECK();
-    finalize_z(d.scal, d.dpart, nb, d.n, true, s);
-    return sv;  // point indices in box order: a spatial order of the current embedding
+    finalize_z(L, F_PT, s);
+    // point indices of job 0 in box order: a spatial order of the current embedding
+    return alt ? L.vals_alt : L.vals;
 }
 
 }  // namespace bht

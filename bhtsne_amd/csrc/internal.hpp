@@ -1,4 +1,4 @@
-// Internal interfaces between the C ABI (api.cpp) and the HIP kernels (*.hip).
+// Internal interfaces between the C ABI (api.cpp, batch.cpp) and the HIP kernels (*.hip).
 #pragma once
 
 #include "common.hpp"
@@ -49,27 +49,122 @@ struct TwiddleSet {
     size_t total = 0;
 };
 
-// Device view of one FIt-SNE job's epoch workspace.
-struct FitsneDev {
-    int n;
-    int dim;           // 1 or 2
-    int min_boxes;     // 10 = reference grid; larger = finer
-    int nodes_cap;     // capacity of the node grids (3 * max boxes)
-    int len_cap;       // capacity of the FFT length
-    float *y;          // current embedding (n * dim)
-    float *frep;       // repulsive forces (n * dim)
-    uint32_t *keys, *vals, *keys_alt, *vals_alt;  // box sort
-    float4 *grid;      // spread charges / potentials, dense nodes x nodes (D = 2)
-    float2 *spec;      // 2 x nodes_cap x len_cap complex
-    float *rker;       // kernel row transform, (len_cap/2+1)^2
-    float *kspec;      // kernel spectrum quarter, (len_cap/2+1)^2
-    float *part;       // spread partials, 2 x nchunks x 36
-    double *dpart;     // per-block FP64 partials (Z)
-    double *mpart;     // per-block FP64 partials of the update (recentring sums)
-    float *fpart;      // per-block f32 partials (min/max)
-    void *sort_temp;
+// Barnes-Hut tree in breadth-first order, as the reference's arena (src/tsne/arena.rs:36-62).
+struct BhNodes {
+    float4 *node;     // [2n] com.x, com.y, count (u32 bits), first_child (u32 bits, ~0 = leaf)
+    uint32_t *meta;   // [2n] child_count | level << 8
+    uint32_t *range;  // [4n] start, end in the sorted order
+    float *hw2;       // [33] squared half-width per level
+    uint32_t *counts; // [1] node count
+};
+
+// ---- job table -------------------------------------------------------------------------------
+//
+// Every epoch kernel reads its job from a device-resident table.  A launch covers one job or
+// many: block b of a launch belongs to job map[b] >> 16 as its block map[b] & 0xffff, out of the
+// job's nblk[family] blocks of that kernel family.  Without a map the launch is job 0 alone and
+// its grid is the job's block count.
+
+enum Family {
+    F_PT = 0,   // point-parallel passes (centring, box keys, gather, Morton codes)
+    F_ONE,      // one block per job (finalisers, grid plan, tree build)
+    F_SPR,      // box-segmented spreading and its fix-up
+    F_ZERO,     // grid clearing
+    F_KSPEC,    // kernel-spectrum row / column transforms
+    F_ROWS,     // forward / inverse row transforms of the charge grids
+    F_COLS,     // column transforms with the spectral product
+    F_LINE1,    // D = 1 line convolution (two blocks)
+    F_ATT,      // attraction + update
+    F_TRAV,     // Barnes-Hut traversal
+    NFAM
+};
+
+enum Method { M_BH = 0, M_FITSNE = 1 };
+
+struct JobDev {
+    int n, dim, method;
+    int min_boxes, max_boxes, nodes_cap, len_cap;
+    int nblk[NFAM];
+    float theta_sq;
+    // optimiser state; the embedding of epoch e is ybuf[e & 1]
+    float *ybuf[2];
+    float *uy, *gains, *frep;
+    const uint32_t *rows, *cols;
+    float *vals;
+    float *qv;          // values at the reference's CSR positions [0, qv_n) (KL weights)
+    int qv_n;
     EpochScalars *scal;
-    const TwiddleSet *tw;
+    double *dpart;      // per-block FP64 partials (Z)
+    double *mpart;      // per-block FP64 partials of the update (recentring sums)
+    float *fpart;       // per-block f32 partials (min / max)
+    // gradient step (src/tsne/mod.rs:470-516, src/lib.rs:1162-1179)
+    float learning_rate, momentum, final_momentum, inv_exag;
+    int momentum_switch_epoch, stop_lying_epoch;
+    // FIt-SNE: box sort input and output, grids
+    uint32_t *keys, *kidx;
+    const uint32_t *skeys, *sidx;
+    float4 *grid;       // spread charges / potentials, dense nodes x nodes (D = 2)
+    float2 *spec;       // 2 x nodes_cap x len_cap complex
+    float *rker;        // kernel row transform, (len_cap/2+1)^2
+    float *kspec;       // kernel spectrum quarter, (len_cap/2+1)^2
+    float *part;        // spread partials, 2 x nchunks x 36
+    // Barnes-Hut: Morton sort input and output, tree
+    uint64_t *codes;
+    uint32_t *cidx;
+    const uint64_t *scodes;
+    const uint32_t *scidx;
+    BhNodes nodes;
+};
+
+struct BlockCtx {
+    int job, lb, nb;
+};
+
+__device__ __forceinline__ BlockCtx block_ctx(const uint32_t *map, const JobDev *jobs, int fam) {
+    if (map == nullptr) return {0, (int)blockIdx.x, (int)gridDim.x};
+    const uint32_t m = map[blockIdx.x];
+    const int j = (int)(m >> 16);
+    return {j, (int)(m & 0xffffu), jobs[j].nblk[fam]};
+}
+
+__device__ __forceinline__ int job_epoch(const JobDev &J) { return J.scal->epoch; }
+__device__ __forceinline__ float *job_y(const JobDev &J) { return J.ybuf[J.scal->epoch & 1]; }
+
+// Block counts of one job (the same formulas size the single-job grids and the batch maps).
+int point_blocks(int n);
+int attract_blocks(int n);
+int spread_blocks(int n);
+int traverse_blocks(int n);
+void job_block_counts(JobDev &J, bool batched);
+
+// Segment of a segmented sort: the keys of one job at [base, base + n) of the arena, sorted in
+// place of the same range; its tiles are [tile0, tile0 + ntiles) of the launch.
+struct SortSeg {
+    uint64_t base;     // offset of the segment in the arena (a multiple of the tile size)
+    uint64_t dense0;   // keys of all earlier segments
+    uint32_t n;
+    uint32_t tile0;
+    uint32_t ntiles;
+    uint32_t pad;
+};
+
+// One launch group: jobs of one method and one output dimension, advanced together.
+struct LaunchSet {
+    const JobDev *jobs = nullptr;       // device table
+    const uint32_t *map[NFAM] = {};     // device block maps (null: job 0 alone)
+    int nb[NFAM] = {};                  // grid size of each family
+    int dim = 2;
+    int njobs = 1;
+    size_t fft_lds = 0;                 // dynamic LDS of the FFT kernels (max len_cap)
+    const TwiddleSet *tw = nullptr;
+    // sort arena of the group (u32 box keys for FIt-SNE, u64 Morton codes for Barnes-Hut)
+    void *keys = nullptr, *keys_alt = nullptr;
+    uint32_t *vals = nullptr, *vals_alt = nullptr;
+    uint64_t sort_n = 0;                // arena length (padded segments)
+    void *sort_temp = nullptr;
+    const SortSeg *segs = nullptr;      // null: one unpadded segment of sort_n keys
+    const uint32_t *tile_seg = nullptr; // segment of each tile
+    uint32_t sort_tiles = 0;            // tiles of the segmented sort
 };
 
 // ---- scan / sort (sort.hip) ----------------------------------------------------------------
@@ -77,45 +172,41 @@ size_t scan_temp_bytes(uint64_t n);
 template <typename T>
 void scan_exclusive(const uint32_t *in, T *out, uint64_t n, void *temp, T *total_out,
                     hipStream_t s);
+constexpr int RS_TILE_KEYS = 2048;      // keys per radix-sort tile (segment alignment)
 size_t radix_sort_temp_bytes(uint64_t n);
 template <typename K>
 bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
                       int begin_bit, int end_bit, void *temp, hipStream_t s);
+// Segmented form: each segment sorted independently in place of its own range; ntiles tiles.
+template <typename K>
+bool radix_sort_pairs_seg(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt,
+                          const SortSeg *segs, const uint32_t *tile_seg, uint32_t ntiles,
+                          int begin_bit, int end_bit, void *temp, hipStream_t s);
+// Sorts the group's arena: the plain sort for one job, the segmented one for a batch.
+template <typename K>
+bool sort_group(const LaunchSet &L, int end_bit, hipStream_t s);
+// Key bits of the per-epoch sorts and whether their result lands in the alternate buffers.
+inline int box_key_bits(int dim) { return dim == 2 ? 24 : 16; }  // boxes^D <= 10^6 < 2^20
+constexpr int MORTON_KEY_BITS = 64;
+inline bool sort_result_alt(int end_bit) { return (((end_bit + 7) / 8) & 1) != 0; }
 
 // ---- FIt-SNE epoch (fitsne.hip, fft.hip) ---------------------------------------------------
-int point_blocks(int n);    // grid size of the point-parallel kernels
-int attract_blocks(int n);  // grid size of the attraction kernel (= its partial count)
-// Applies the pending recentring of the last update, then bounds and the grid plan.
-void fitsne_center_plan(const FitsneDev &d, hipStream_t s);
-// Applies the pending recentring and writes the per-axis bounds (no grid plan).
-void center_only(float *y, int n, int dim, EpochScalars *scal, float *fpart, const double *mpart,
-                 hipStream_t s);
-// Z and 1/Z from per-block partial sums (minus n for FIt-SNE).
-void finalize_z(EpochScalars *scal, const double *dpart, int nparts, int n, bool subtract_n,
-                hipStream_t s);
-// after center_plan: frep, Z; returns the point indices in box order (device)
-const uint32_t *fitsne_repulsion(FitsneDev &d, hipStream_t s);
-void fft_kernel_spectrum(const FitsneDev &d, hipStream_t s);
-void fft_convolve(const FitsneDev &d, hipStream_t s);
+// Applies the pending recentring of the last update, then bounds and (do_plan) the grid plan.
+void center_plan(const LaunchSet &L, bool do_plan, hipStream_t s);
+// Z and 1/Z from the per-block partial sums (minus n for FIt-SNE).
+void finalize_z(const LaunchSet &L, int part_family, hipStream_t s);
+// after center_plan: frep, Z; returns the sorted point indices of job 0 (a spatial order)
+const uint32_t *fitsne_repulsion(const LaunchSet &L, hipStream_t s);
+void fft_kernel_spectrum(const LaunchSet &L, hipStream_t s);
+void fft_convolve(const LaunchSet &L, hipStream_t s);
+
+// Barnes-Hut (bh.hip): after center_plan (no plan), frep and Z; returns job 0's sorted indices.
+const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s);
 
 // Attraction + fused update (attract.hip).  rows are u32 CSR offsets (nnz < 2^32).
-struct UpdateArgs {
-    int n, dim;
-    const uint32_t *rows;
-    const uint32_t *cols;
-    float *vals;
-    const float *y;
-    float *y_next;
-    float *uy, *gains;
-    const float *frep;
-    float learning_rate, momentum, final_momentum, inv_exag;
-    int momentum_switch_epoch, stop_lying_epoch;
-    EpochScalars *scal;
-    double *mpart;  // per-block sums of the new embedding (2 per block)
-};
-void attract_update(const UpdateArgs &a, hipStream_t s);
-void attract_only(int n, int dim, const uint32_t *rows, const uint32_t *cols, const float *vals,
-                  const float *y, float *pos, hipStream_t s);
+void attract_update(const LaunchSet &L, hipStream_t s);
+// Attraction alone of job 0 into pos.
+void attract_only(const LaunchSet &L, float *pos, hipStream_t s);
 void scale_values(float *vals, uint64_t nnz, float scale, hipStream_t s);
 // Sparse KL (kl.hip): writes scal->kl_ref / kl_true using scal->inv_z.
 void sparse_kl(const float *y, int n, int dim, const uint32_t *rows, const uint32_t *cols,

@@ -124,19 +124,57 @@ constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 2048 keys per tile
 constexpr int RS_BINS = 256;
 
+static_assert(RS_TILE == RS_TILE_KEYS, "segment alignment must match the tile size");
+
+// Where a tile's keys live and where its digit counts go.  Unsegmented: tile t covers
+// [t * RS_TILE, ...) of n keys and its count of digit d sits at d * ntiles + t.  Segmented: the
+// tile belongs to one segment, covers a slice of that segment's range, and the counts of a
+// segment form a block (digit-major) of their own, in segment order, so that one exclusive scan
+// over all counts gives every segment its dense offsets; the scatter moves them back to the
+// segment's own (padded) range.
+struct TileCtx {
+    uint64_t first, end;   // key range of the tile, clipped to the segment
+    size_t h0;             // count index of digit 0
+    unsigned hstride;      // distance between consecutive digits
+    uint64_t out_adj;      // added to the scanned offset: segment base - keys before it
+};
+
+__device__ __forceinline__ TileCtx tile_ctx(uint64_t n, unsigned ntiles, const SortSeg *segs,
+                                            const uint32_t *tile_seg) {
+    TileCtx c;
+    if (segs == nullptr) {
+        c.first = (uint64_t)blockIdx.x * RS_TILE;
+        c.end = n;
+        c.h0 = blockIdx.x;
+        c.hstride = ntiles;
+        c.out_adj = 0;
+        return c;
+    }
+    const SortSeg g = segs[tile_seg[blockIdx.x]];
+    const unsigned lt = blockIdx.x - g.tile0;
+    c.first = g.base + (uint64_t)lt * RS_TILE;
+    c.end = g.base + g.n;
+    c.h0 = (size_t)RS_BINS * g.tile0 + lt;
+    c.hstride = g.ntiles;
+    c.out_adj = g.base - g.dense0;
+    return c;
+}
+
 template <typename K>
 __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K *keys, uint64_t n, int shift,
-                                                        uint32_t *hist, unsigned ntiles) {
+                                                        uint32_t *hist, unsigned ntiles,
+                                                        const SortSeg *segs,
+                                                        const uint32_t *tile_seg) {
     __shared__ uint32_t h[RS_BINS];
+    const TileCtx tc = tile_ctx(n, ntiles, segs, tile_seg);
     h[threadIdx.x] = 0;
     __syncthreads();
-    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
     for (int r = 0; r < RS_ITEMS; r++) {
-        uint64_t idx = base + (uint64_t)r * RS_THREADS + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(unsigned)(keys[idx] >> shift) & 0xffu], 1u);
+        uint64_t idx = tc.first + (uint64_t)r * RS_THREADS + threadIdx.x;
+        if (idx < tc.end) atomicAdd(&h[(unsigned)(keys[idx] >> shift) & 0xffu], 1u);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    hist[tc.h0 + (size_t)threadIdx.x * tc.hstride] = h[threadIdx.x];
 }
 
 template <typename K>
@@ -144,13 +182,17 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *keys, const 
                                                            K *keys_out, uint32_t *vals_out,
                                                            uint64_t n, int shift,
                                                            const uint32_t *offs,
-                                                           unsigned ntiles) {
+                                                           unsigned ntiles, const SortSeg *segs,
+                                                           const uint32_t *tile_seg) {
     __shared__ uint32_t base[RS_BINS];
     __shared__ uint32_t wcnt[RS_THREADS / 64][RS_BINS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    base[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
+    const TileCtx tc = tile_ctx(n, ntiles, segs, tile_seg);
+    base[threadIdx.x] =
+        offs[tc.h0 + (size_t)threadIdx.x * tc.hstride] + (uint32_t)tc.out_adj;
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint64_t tile0 = (uint64_t)blockIdx.x * RS_TILE;
+    const uint64_t tile0 = tc.first;
+    n = tc.end;
     for (int r = 0; r < RS_ITEMS; r++) {
 #pragma unroll
         for (int q = 0; q < RS_THREADS / 64; q++) wcnt[q][threadIdx.x] = 0;
@@ -194,10 +236,9 @@ size_t radix_sort_temp_bytes(uint64_t n) {
 // Sorts (keys, vals) by bits [begin_bit, end_bit) of the key, stable.  Ping-pongs between the
 // two buffer pairs; returns true when the result is in (keys_alt, vals_alt).
 template <typename K>
-bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
-                      int begin_bit, int end_bit, void *temp, hipStream_t s) {
-    if (n == 0) return false;
-    unsigned ntiles = ceil_div(n, RS_TILE);
+static bool radix_passes(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
+                         unsigned ntiles, const SortSeg *segs, const uint32_t *tile_seg,
+                         int begin_bit, int end_bit, void *temp, hipStream_t s) {
     uint64_t nh = (uint64_t)ntiles * RS_BINS;
     uint32_t *hist = (uint32_t *)temp;
     uint32_t *offs = hist + nh;
@@ -207,10 +248,10 @@ bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, 
     bool alt = false;
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist<K>, dim3(ntiles), dim3(RS_THREADS), 0, s, ki, n, shift, hist,
-                           ntiles);
+                           ntiles, segs, tile_seg);
         scan_exclusive<uint32_t>(hist, offs, nh, stemp, nullptr, s);
         hipLaunchKernelGGL(k_rs_scatter<K>, dim3(ntiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo,
-                           n, shift, offs, ntiles);
+                           n, shift, offs, ntiles, segs, tile_seg);
         BHT_LAUNCH_CHECK();
         K *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;
@@ -219,9 +260,44 @@ bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, 
     return alt;
 }
 
+template <typename K>
+bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
+                      int begin_bit, int end_bit, void *temp, hipStream_t s) {
+    if (n == 0) return false;
+    return radix_passes<K>(keys, vals, keys_alt, vals_alt, n, ceil_div(n, RS_TILE), nullptr,
+                           nullptr, begin_bit, end_bit, temp, s);
+}
+
+// The temp space of a segmented sort is radix_sort_temp_bytes(ntiles * RS_TILE).
+template <typename K>
+bool radix_sort_pairs_seg(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt,
+                          const SortSeg *segs, const uint32_t *tile_seg, uint32_t ntiles,
+                          int begin_bit, int end_bit, void *temp, hipStream_t s) {
+    if (ntiles == 0) return false;
+    return radix_passes<K>(keys, vals, keys_alt, vals_alt, (uint64_t)ntiles * RS_TILE, ntiles,
+                           segs, tile_seg, begin_bit, end_bit, temp, s);
+}
+
+template <typename K>
+bool sort_group(const LaunchSet &L, int end_bit, hipStream_t s) {
+    if (L.segs)
+        return radix_sort_pairs_seg<K>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt, L.segs,
+                                       L.tile_seg, L.sort_tiles, 0, end_bit, L.sort_temp, s);
+    return radix_sort_pairs<K>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt, L.sort_n, 0,
+                               end_bit, L.sort_temp, s);
+}
+
 template bool radix_sort_pairs<uint32_t>(uint32_t *, uint32_t *, uint32_t *, uint32_t *, uint64_t,
                                          int, int, void *, hipStream_t);
 template bool radix_sort_pairs<uint64_t>(uint64_t *, uint32_t *, uint64_t *, uint32_t *, uint64_t,
                                          int, int, void *, hipStream_t);
+template bool radix_sort_pairs_seg<uint32_t>(uint32_t *, uint32_t *, uint32_t *, uint32_t *,
+                                             const SortSeg *, const uint32_t *, uint32_t, int, int,
+                                             void *, hipStream_t);
+template bool radix_sort_pairs_seg<uint64_t>(uint64_t *, uint32_t *, uint64_t *, uint32_t *,
+                                             const SortSeg *, const uint32_t *, uint32_t, int, int,
+                                             void *, hipStream_t);
+template bool sort_group<uint32_t>(const LaunchSet &, int, hipStream_t);
+template bool sort_group<uint64_t>(const LaunchSet &, int, hipStream_t);
 
 }  // namespace bht

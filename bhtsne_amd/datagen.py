@@ -40,6 +40,40 @@ def config_c3(n=1_000_000):
     return gmm(n, 50, 50, 36.0, 2, weights="dirichlet")
 
 
+def log_uniform_int(rng, lo, hi):
+    return int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+
+def config_c4_job(j):
+    """Job j of the batched workload (seeded by j): N log-uniform in [500, 50,000], D_in from
+    {16, 32, 64}, 5-20 components, centres N(0, 25 I), sigma 1; even jobs FIt-SNE, odd jobs
+    Barnes-Hut (theta 0.5).  Returns (x, method)."""
+    rng = np.random.Generator(np.random.PCG64(10_000 + j))
+    n = log_uniform_int(rng, 500, 50_000)
+    d_in = int(rng.choice([16, 32, 64]))
+    comps = int(rng.integers(5, 21))
+    x, _ = gmm(n, d_in, comps, 25.0, 20_000 + j)
+    return x, ("fitsne" if j % 2 == 0 else "bh")
+
+
+def config_c5_request(r):
+    """Request r of the serving workload (seeded by r): N log-uniform in [1,000, 200,000], D_in
+    from {20, 50}, D_out from {1, 2}, 10 components, centres N(0, 25 I), sigma 1.
+    Returns (x, d_out)."""
+    rng = np.random.Generator(np.random.PCG64(30_000 + r))
+    n = log_uniform_int(rng, 1_000, 200_000)
+    d_in = int(rng.choice([20, 50]))
+    d_out = int(rng.choice([1, 2]))
+    x, _ = gmm(n, d_in, 10, 25.0, 40_000 + r)
+    return x, d_out
+
+
+def poisson_arrivals(count, rate, seed):
+    """Arrival times (seconds) of a Poisson stream of `count` requests at `rate` per second."""
+    rng = np.random.Generator(np.random.PCG64(50_000 + seed))
+    return np.cumsum(rng.exponential(1.0 / rate, count))
+
+
 def seed_embedding(n, d_out, seed):
     """Seed embedding ~ N(0, 1e-4) (the law of src/tsne/mod.rs:142-147), seeded."""
     rng = np.random.Generator(np.random.PCG64(seed + 1000))

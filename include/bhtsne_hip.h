@@ -165,6 +165,39 @@ bht_status bht_symmetrize(bht_ctx *ctx, const uint32_t *cols, const float *vals,
                           uint32_t k, uint64_t *rows_out, uint32_t *cols_out, float *vals_out,
                           uint64_t *nnz);
 
+/* ---- batched and served jobs ------------------------------------------------------------------
+ *
+ * No counterpart in the reference, which runs one tSNE per call on the CPU.  A batch advances
+ * many independent jobs together: every epoch kernel is one launch for all members (of the same
+ * method and d_out), and the epoch is replayed from a captured graph.  Each member computes
+ * exactly what bht_job_fit_sne / bht_job_barnes_hut computes for it alone (same kernels, same
+ * block partition, same summation order); only the point renumbering of large single jobs is not
+ * applied.  A request loop admits jobs with bht_batch_add as others finish, so a batch serves a
+ * stream of requests. */
+typedef struct bht_batch bht_batch;
+
+bht_status bht_batch_create(bht_ctx *ctx, bht_batch **out);
+/* Members still running are dropped (their runs are left unfinished). */
+void bht_batch_destroy(bht_batch *batch);
+/* The fit preamble of the job (affinities from its data unless cached, P normalisation, the
+ * initial embedding; the same validation and messages as the fit calls), then membership.
+ * method: BHT_METHOD_FITSNE or BHT_METHOD_BARNES_HUT (theta > 0).  Each job runs params.epochs. */
+bht_status bht_batch_add(bht_batch *batch, bht_job *job, uint32_t method, float theta);
+/* Advances every member by up to max_epochs, never past the member with the fewest epochs left;
+ * members whose runs complete are finished (as the fit calls finish them) and queued for
+ * bht_batch_take_finished.  Jobs must not be destroyed while they are members. */
+bht_status bht_batch_step(bht_batch *batch, uint64_t max_epochs, uint64_t *epochs_advanced);
+bht_status bht_batch_take_finished(bht_batch *batch, bht_job **out, uint64_t capacity,
+                                   uint64_t *count);
+uint64_t bht_batch_size(const bht_batch *batch);
+/* Graph replay of the epoch (default on) or plain launches. */
+bht_status bht_batch_set_graph(bht_batch *batch, int enable);
+bht_status bht_batch_synchronize(bht_batch *batch);
+/* Upper bound of the device bytes one job of n samples (d_in features) needs: what a request loop
+ * admits against the context's HBM budget. */
+bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params, uint32_t method,
+                             uint64_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif
