@@ -850,9 +850,11 @@ struct bht_batch {
     hipGraphExec_t exec = nullptr;
     uint64_t epochs_launched = 0;
 
+    // non-blocking: synchronous copies on the null stream by another host thread (a request
+    // loop preparing jobs) neither wait for the batch nor disturb a capture in progress
     explicit bht_batch(bht_ctx *c) : ctx(c) {
         BHT_HIP(hipSetDevice(ctx->device));
-        BHT_HIP(hipStreamCreate(&stream));
+        BHT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     }
     ~bht_batch() {
         drop_graph();
@@ -875,6 +877,8 @@ uint64_t pad_tile(uint64_t n) {
 // Rebuilds the table, the groups' arenas and block maps after a membership change.
 void batch_rebuild(bht_batch *b) {
     bht_ctx *c = b->ctx;
+    // epochs launched earlier may still read the old table, maps and arenas
+    BHT_HIP(hipStreamSynchronize(b->stream));
     b->drop_graph();
     b->groups.clear();
     std::vector<JobDev> table(b->members.size());
