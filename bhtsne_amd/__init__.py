@@ -408,16 +408,21 @@ def job_footprint(n: int, d_in: int, d_out: int = 2, method: str = "fitsne", **p
 
 # ---- operators (the reference's internal steps) -------------------------------------------------
 
-def knn(x, k: int, ctx: Optional[Context] = None):
-    """Exact kNN, self excluded, ascending (distance, index) (src/tsne/vptree.rs:274-319)."""
+def knn(x, k: int, ctx: Optional[Context] = None, mode: str = "auto", stats: bool = False):
+    """Exact kNN, self excluded, ascending (distance, index) (src/tsne/vptree.rs:274-319).
+    mode "auto": matrix-core candidates, exact re-rank, per-row certificate (uncertified rows
+    recomputed exactly); "exact": the exact kernel for every row.  Both give the same result;
+    with stats=True the number of rows the exact kernel computed is returned as well."""
     ctx = ctx or default_context()
     L = _lib.load()
     x = np.ascontiguousarray(x, np.float32)
     n, d = x.shape
     idx = np.zeros((n, k), np.uint32)
     dist = np.zeros((n, k), np.float32)
-    check(L.bht_knn_f32(ctx.handle, ptr(x), n, d, k, ptr(idx), ptr(dist)))
-    return idx, dist
+    rows = C.c_uint64(0)
+    m = _lib.KNN_EXACT if mode == "exact" else _lib.KNN_AUTO
+    check(L.bht_knn_f32_ex(ctx.handle, ptr(x), n, d, k, m, ptr(idx), ptr(dist), C.byref(rows)))
+    return (idx, dist, int(rows.value)) if stats else (idx, dist)
 
 
 def search_beta(dist, perplexity: float, mode: int = _lib.BANDWIDTH_FP64,

@@ -20,6 +20,8 @@ BHT_ERR_STATE = 4
 
 METHOD_BARNES_HUT = 0
 METHOD_FITSNE = 1
+KNN_AUTO = 0
+KNN_EXACT = 1
 BANDWIDTH_FP64 = 0
 BANDWIDTH_F32_SEQ = 1
 
@@ -90,6 +92,8 @@ SIGNATURES = {
                                         C.c_float, C.c_uint32, C.c_void_p]),
     "bht_symmetrize": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "bht_knn_f32_ex": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     "bht_batch_create": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "bht_batch_destroy": (None, [C.c_void_p]),
     "bht_batch_add": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float]),

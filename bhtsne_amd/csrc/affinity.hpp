@@ -35,5 +35,12 @@ uint64_t symmetrize_device(uint32_t *cols, float *vals, int64_t n, int k, SymWor
 // ascending (distance, index).
 void knn_exact(const float *x, int64_t n, int d, int k, uint32_t *idx, float *dist,
                hipStream_t s);
+// The same result through a matrix-core candidate pass, an exact re-rank and a certificate per
+// row (uncertified rows go through knn_exact).  temp: knn_temp_bytes(n, d, k) bytes; 0 means the
+// shape is outside the fast path (d > 64, k > 128 or n <= 128) and knn_exact runs instead.
+// Synchronises the stream.  exact_rows (optional) receives the rows recomputed exactly.
+size_t knn_temp_bytes(int64_t n, int d, int k);
+bool knn_fast(const float *x, int64_t n, int d, int k, uint32_t *idx, float *dist, void *temp,
+              hipStream_t s, uint64_t *exact_rows);
 
 }  // namespace bht

@@ -219,6 +219,8 @@ struct bht_job {
     bool permuted = false;
     uint64_t reorders = 0;
 
+    uint64_t knn_exact_rows = 0;  // rows of the last kNN recomputed by the exact kernel
+
     // run state
     int method = -1;
     float theta = 0.5f;
@@ -623,11 +625,14 @@ void build_affinities_from_data(bht_job *j) {
     if (!j->have_data) throw Error(ERR_STATE, "no sample vectors: call bht_job_set_data_f32");
     uint32_t k = n_neighbors_of(j);
     bht_ctx *c = j->ctx;
-    DevBuf idx(c), dist(c);
+    DevBuf idx(c), dist(c), temp(c);
     idx.ensure(c, (size_t)j->n * k * 4);
     dist.ensure(c, (size_t)j->n * k * 4);
-    knn_exact(j->x.as<float>(), (int64_t)j->n, (int)j->d_in, (int)k, idx.as<uint32_t>(),
-              dist.as<float>(), j->stream);
+    const size_t tb = knn_temp_bytes((int64_t)j->n, (int)j->d_in, (int)k);
+    if (tb) temp.ensure(c, tb);
+    knn_fast(j->x.as<float>(), (int64_t)j->n, (int)j->d_in, (int)k, idx.as<uint32_t>(),
+             dist.as<float>(), tb ? temp.p : nullptr, j->stream, &j->knn_exact_rows);
+    temp.reset();
     affinities_from_knn(j, idx.as<uint32_t>(), dist.as<float>(), k);
 }
 
@@ -1008,15 +1013,32 @@ void batch_epoch_launches(bht_batch *b) {
     }
 }
 
-void batch_launch_epoch(bht_batch *b) {
-    if (!b->use_graph) {
+// One epoch as a graph replay once the membership has held for a whole step (a request loop
+// that admits jobs every step runs plain launches instead of re-capturing each time).  A capture
+// that fails (another host thread made a call the runtime will not record beside a capture) is
+// discarded and the epoch launched plainly.
+void batch_launch_epoch(bht_batch *b, bool may_capture) {
+    if (!b->use_graph || (!b->exec && !may_capture)) {
         batch_epoch_launches(b);
         return;
     }
     if (!b->exec) {
-        BHT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        batch_epoch_launches(b);
-        BHT_HIP(hipStreamEndCapture(b->stream, &b->graph));
+        hipGraph_t g = nullptr;
+        BHT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+        bool recorded = true;
+        try {
+            batch_epoch_launches(b);
+        } catch (const Error &) {
+            recorded = false;  // launches refused by an invalidated capture
+        }
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (!recorded || e != hipSuccess || g == nullptr) {
+            if (g) hipGraphDestroy(g);
+            (void)hipGetLastError();
+            batch_epoch_launches(b);
+            return;
+        }
+        b->graph = g;
         BHT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     BHT_HIP(hipGraphLaunch(b->exec, b->stream));
@@ -1055,8 +1077,9 @@ uint64_t batch_step(bht_batch *b, uint64_t max_epochs) {
         k = std::min(k, left);
     }
     if (k > 0) {
+        const bool rebuilt = b->dirty;
         if (b->dirty) batch_rebuild(b);
-        for (uint64_t e = 0; e < k; e++) batch_launch_epoch(b);
+        for (uint64_t e = 0; e < k; e++) batch_launch_epoch(b, !rebuilt || k >= 64);
         b->epochs_launched += k;
         for (bht_job *j : b->members)
             for (uint64_t e = 0; e < k; e++) {
@@ -1454,22 +1477,37 @@ bht_status bht_job_kl_divergence(bht_job *job, double *reference_kl, double *tru
     });
 }
 
-bht_status bht_knn_f32(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,
-                       uint32_t *idx_out, float *dist_out) {
+bht_status bht_knn_f32_ex(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,
+                          uint32_t mode, uint32_t *idx_out, float *dist_out,
+                          uint64_t *exact_rows) {
     if (!ctx || !x || !idx_out || !dist_out) return fail(BHT_ERR_INVALID, "null argument");
     return guarded([&] {
         if (k == 0 || k >= n) throw Error(ERR_INVALID, "k must be in [1, n - 1]");
-        DevBuf dx(ctx), di(ctx), dd(ctx);
+        if (mode != BHT_KNN_AUTO && mode != BHT_KNN_EXACT) throw Error(ERR_INVALID, "unknown mode");
+        DevBuf dx(ctx), di(ctx), dd(ctx), temp(ctx);
         dx.ensure(ctx, (size_t)n * d_in * 4);
         di.ensure(ctx, (size_t)n * k * 4);
         dd.ensure(ctx, (size_t)n * k * 4);
         BHT_HIP(hipMemcpy(dx.p, x, (size_t)n * d_in * 4, hipMemcpyHostToDevice));
-        knn_exact(dx.as<float>(), (int64_t)n, (int)d_in, (int)k, di.as<uint32_t>(),
-                  dd.as<float>(), ctx->stream);
+        const size_t tb = mode == BHT_KNN_AUTO ? knn_temp_bytes((int64_t)n, (int)d_in, (int)k) : 0;
+        if (tb) temp.ensure(ctx, tb);
+        uint64_t rows = n;
+        if (tb)
+            knn_fast(dx.as<float>(), (int64_t)n, (int)d_in, (int)k, di.as<uint32_t>(),
+                     dd.as<float>(), temp.p, ctx->stream, &rows);
+        else
+            knn_exact(dx.as<float>(), (int64_t)n, (int)d_in, (int)k, di.as<uint32_t>(),
+                      dd.as<float>(), ctx->stream);
         BHT_HIP(hipStreamSynchronize(ctx->stream));
         BHT_HIP(hipMemcpy(idx_out, di.p, (size_t)n * k * 4, hipMemcpyDeviceToHost));
         BHT_HIP(hipMemcpy(dist_out, dd.p, (size_t)n * k * 4, hipMemcpyDeviceToHost));
+        if (exact_rows) *exact_rows = rows;
     });
+}
+
+bht_status bht_knn_f32(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,
+                       uint32_t *idx_out, float *dist_out) {
+    return bht_knn_f32_ex(ctx, x, n, d_in, k, BHT_KNN_AUTO, idx_out, dist_out, nullptr);
 }
 
 bht_status bht_search_beta_f32(bht_ctx *ctx, const float *dist, uint64_t n, uint32_t k,

@@ -154,9 +154,19 @@ bht_status bht_job_set_timing(bht_job *job, int enable);
 bht_status bht_job_get_timing(bht_job *job, double *out3);
 
 /* ---- operators on host buffers (the reference's internal steps, for parity checks) ----------- */
-/* k nearest neighbours of every row (self excluded), ascending (distance, index). */
+/* k nearest neighbours of every row (self excluded), ascending (distance, index), with the
+ * reference metric's f32 distances.  The fits use the same routine. */
 bht_status bht_knn_f32(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,
                        uint32_t *idx_out, float *dist_out);
+/* The same with the path chosen: BHT_KNN_AUTO = matrix-core candidates + exact re-rank +
+ * per-row certificate (exact kernel for uncertified rows; the result is identical), or
+ * BHT_KNN_EXACT = the exact kernel for every row.  exact_rows (optional) receives the number of
+ * rows the exact kernel computed. */
+#define BHT_KNN_AUTO 0u
+#define BHT_KNN_EXACT 1u
+bht_status bht_knn_f32_ex(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,
+                          uint32_t mode, uint32_t *idx_out, float *dist_out,
+                          uint64_t *exact_rows);
 /* Per-row bandwidth search: conditional P (n * k) from neighbour distances (n * k). */
 bht_status bht_search_beta_f32(bht_ctx *ctx, const float *dist, uint64_t n, uint32_t k,
                                float perplexity, uint32_t mode, float *p_out);

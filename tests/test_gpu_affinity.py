@@ -26,6 +26,31 @@ def test_knn_bit_exact(bt, n, d, k):
     assert np.array_equal(dist, rdist)
 
 
+@pytest.mark.parametrize("n,d,k,seed", [(3000, 50, 90, 1), (2500, 64, 90, 2), (4000, 16, 45, 3),
+                                        (1999, 33, 100, 4), (400, 50, 90, 5)])
+def test_knn_matrix_core_path_equals_exact(bt, n, d, k, seed):
+    # matrix-core candidates + exact re-rank + certificate: the same bits as the exact kernel
+    x, _ = datagen.gmm(n, d, 8, 25.0, seed)
+    idx, dist, rows = bt.knn(x, k, stats=True)
+    eidx, edist, erows = bt.knn(x, k, mode="exact", stats=True)
+    print(f"n={n} d={d} k={k}: rows recomputed exactly {rows} of {n}", flush=True)
+    assert erows == n
+    assert np.array_equal(idx, eidx) and np.array_equal(dist, edist)
+    assert rows < n // 10  # the certificate holds for almost every row of such data
+
+
+def test_knn_uncertified_rows_fall_back(bt):
+    # an integer lattice: many exactly equal distances at the k-th place, so the certificate
+    # fails there and those rows go through the exact kernel; the result is still exact
+    rng = np.random.default_rng(8)
+    x = rng.integers(0, 3, size=(1500, 6)).astype(np.float32)
+    idx, dist, rows = bt.knn(x, 40, stats=True)
+    ridx, rdist = oracle.knn(x, 40)
+    print("lattice rows recomputed exactly", rows, flush=True)
+    assert rows > 0
+    assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
+
+
 def test_knn_ties_break_by_index(bt):
     x = np.zeros((300, 4), np.float32)
     x[150:] = 1.0
