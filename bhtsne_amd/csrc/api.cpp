@@ -32,16 +32,21 @@ bht_status fail(int code, const std::string &msg) {
     return code;
 }
 
+// Runs an entry point's body; a failure becomes a status and a message, and the runtime's
+// last-error slot is cleared so that it is not reported again by a later, unrelated call.
 template <typename F>
 bht_status guarded(F &&f) {
     try {
         f();
         return BHT_OK;
     } catch (const Error &e) {
+        (void)hipGetLastError();
         return fail(e.code, e.what());
     } catch (const std::bad_alloc &) {
+        (void)hipGetLastError();
         return fail(BHT_ERR_OOM, "host allocation failed");
     } catch (const std::exception &e) {
+        (void)hipGetLastError();
         return fail(BHT_ERR_HIP, e.what());
     }
 }

@@ -350,11 +350,22 @@ __global__ void k_list_rows(const uint32_t *flag, const uint32_t *pos, int n, ui
     if (i < n && flag[i]) list[pos[i]] = (uint32_t)i;
 }
 
+// Workspace pieces, each starting on a 256-byte boundary: norms, candidates, tau, flags,
+// positions, row list, misc words, scan temp.
+static size_t knn_layout(size_t nn, size_t *off) {
+    const size_t piece[8] = {nn * 4, nn * KCAND * 4, nn * 4, nn * 4, nn * 4, nn * 4, 256,
+                             scan_temp_bytes(nn)};
+    size_t at = 0;
+    for (int i = 0; i < 8; i++) {
+        if (off) off[i] = at;
+        at += (piece[i] + 255) & ~(size_t)255;
+    }
+    return at;
+}
+
 size_t knn_temp_bytes(int64_t n, int d, int k) {
     if (d > 4 * MKS || k > KCAND || n <= KCAND) return 0;
-    const size_t nn = (size_t)n;
-    return nn * 4 /* nrm */ + nn * KCAND * 4 /* cand */ + nn * 4 /* tau */ + nn * 4 /* flags */ +
-           nn * 4 /* positions */ + nn * 4 /* list */ + 256 + scan_temp_bytes(nn) + 1024;
+    return knn_layout((size_t)n, nullptr);
 }
 
 bool knn_fast(const float *x, int64_t n, int d, int k, uint32_t *idx, float *dist, void *temp,
@@ -367,19 +378,16 @@ bool knn_fast(const float *x, int64_t n, int d, int k, uint32_t *idx, float *dis
     }
     const size_t nn = (size_t)n;
     char *p = (char *)temp;
-    auto take = [&](size_t bytes) {
-        char *r = p;
-        p += (bytes + 255) & ~(size_t)255;
-        return (void *)r;
-    };
-    float *nrm = (float *)take(nn * 4);
-    uint32_t *cand = (uint32_t *)take(nn * KCAND * 4);
-    float *tau = (float *)take(nn * 4);
-    uint32_t *flag = (uint32_t *)take(nn * 4);
-    uint32_t *pos = (uint32_t *)take(nn * 4);
-    uint32_t *list = (uint32_t *)take(nn * 4);
-    uint32_t *misc = (uint32_t *)take(256);  // [0] max norm bits, [1] uncertain rows
-    void *stemp = take(scan_temp_bytes(nn));
+    size_t off[8];
+    knn_layout(nn, off);
+    float *nrm = (float *)(p + off[0]);
+    uint32_t *cand = (uint32_t *)(p + off[1]);
+    float *tau = (float *)(p + off[2]);
+    uint32_t *flag = (uint32_t *)(p + off[3]);
+    uint32_t *pos = (uint32_t *)(p + off[4]);
+    uint32_t *list = (uint32_t *)(p + off[5]);
+    uint32_t *misc = (uint32_t *)(p + off[6]);  // [0] max norm bits, [1] uncertain rows
+    void *stemp = p + off[7];
     BHT_HIP(hipMemsetAsync(misc, 0, 256, s));
     const unsigned nb = ceil_div(nn, 256);
     hipLaunchKernelGGL(k_sqnorm, dim3(nb), dim3(256), 0, s, x, (int)n, d, nrm, misc);
