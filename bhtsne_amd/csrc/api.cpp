@@ -191,8 +191,8 @@ struct bht_job {
     uint32_t d_in = 0;
     bool have_data = false;
 
-    // affinities: CSR with u32 offsets on the device
-    DevBuf rows, cols, vals;
+    // affinities: CSR with u32 offsets on the device, and its row groups (attraction work units)
+    DevBuf rows, cols, vals, groups;
     uint64_t nnz = 0;
     bool have_p = false;
     bool have_cached_perplexity = false;
@@ -284,7 +284,7 @@ void ensure_state_buffers(bht_job *j) {
     j->pos.ensure(j->ctx, b);
     j->scal.ensure(j->ctx, sizeof(EpochScalars));
     j->dpart.ensure(j->ctx, 4096 * sizeof(double));
-    j->mpart.ensure(j->ctx, 2 * 4096 * sizeof(double));
+    j->mpart.ensure(j->ctx, 2 * (size_t)MAX_ATTRACT_BLOCKS * sizeof(double));
     j->fpart.ensure(j->ctx, 4 * 1024 * sizeof(float));
 }
 
@@ -366,6 +366,8 @@ void fill_jobdev(bht_job *j) {
     d.rows = j->rows.as<uint32_t>();
     d.cols = j->cols.as<uint32_t>();
     d.vals = j->vals.as<float>();
+    d.groups = j->groups.as<uint32_t>();
+    d.ngroups = (int)row_groups_count(j->nnz);
     d.qv = j->qv.as<float>();
     d.qv_n = (int)std::min<uint64_t>(j->n, j->nnz);
     d.scal = j->scal.as<EpochScalars>();
@@ -396,8 +398,17 @@ void fill_jobdev(bht_job *j) {
     job_block_counts(d, false);
 }
 
+// Row groups of the current CSR layout (after any change of rows: build, injection, renumbering).
+void refresh_row_groups(bht_job *j) {
+    const uint32_t ng = row_groups_count(j->nnz);
+    j->groups.ensure(j->ctx, ((size_t)ng + 1) * 4);
+    build_row_groups(j->rows.as<uint32_t>(), (int)j->n, j->nnz, j->groups.as<uint32_t>(),
+                     j->stream);
+}
+
 // Refreshes the job's own device table entry (single-job launches).
 void sync_jobdev(bht_job *j) {
+    refresh_row_groups(j);
     fill_jobdev(j);
     j->jd.ensure(j->ctx, sizeof(JobDev));
     BHT_HIP(hipMemcpyAsync(j->jd.p, &j->hd, sizeof(JobDev), hipMemcpyHostToDevice, j->stream));
@@ -1674,7 +1685,9 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
     const uint64_t ne = n * k, nnz = 2 * ne;
     uint64_t b = n * d_in * 4;                      // samples
     b += (n + 1) * 4 + nnz * 8 + n * 4;             // CSR + KL weights
-    b += 6 * n * D * 4 + 4096 * 8 * 3 + 4096 * 16;  // optimiser state + partials
+    b += ((uint64_t)row_groups_count(nnz) + 1) * 4; // row groups
+    b += 6 * n * D * 4 + 4096 * 8 + 4096 * 16;      // optimiser state + partials
+    b += 2 * (uint64_t)MAX_ATTRACT_BLOCKS * 8;      // update partials
     b += ne * 12 * 4 + radix_sort_temp_bytes(ne);   // affinity-build peak (transient)
     if (method == BHT_METHOD_BARNES_HUT) {
         b += n * (2 * 8 + 2 * 4) + radix_sort_temp_bytes(n);  // Morton sort

@@ -1,21 +1,95 @@
 // Attractive forces over the CSR graph fused with the gains / momentum update and the partial
 // sums of the next recentring (src/tsne/arena.rs:398-427, src/tsne/mod.rs:470-516, 534-568).
 //
-// One wave owns one CSR row at a time.  Each lane takes up to four edges of the row per step
-// (issuing their column, value and gathered y loads before using any), so a row of ~150 edges
-// needs one or two dependent memory round trips instead of ten.  The wave reduces with a fixed
-// butterfly and lane 0 applies the update.  The new embedding goes to a second buffer (y_next)
-// because other rows still read y this epoch.
+// Work unit: a row group, the rows whose first edge lies in one GROUP_EDGES-edge window of the
+// CSR (built once per CSR layout by k_row_groups; at most a few rows, since a symmetrised row has
+// at least k entries).  One wave streams the group's whole edge span, lane l taking edges
+// l, l + 64, .. of each pass of 64 * GROUP_UNROLL edges: every column, value and gathered y load of
+// a pass is independent of the others and of the row boundaries, so a wave keeps eight gathers per
+// lane in flight with no idle lanes, where a wave per row would wait on each row's offsets and
+// leave a third of its lanes empty.  Each edge adds into its row's accumulator (a uniform
+// comparison against the group's row offsets picks it); the rows are reduced by fixed butterflies
+// and lane q applies the update of the group's q-th row.  A group of more than MAX_GROUP_ROWS rows
+// (short rows, from caller-supplied affinities) is processed row by row.  The new embedding goes
+// to the other buffer of the pair because other rows still read y this epoch.
 #include "internal.hpp"
 
 namespace bht {
 
 constexpr int UP_THREADS = 256;
-constexpr int UNROLL = 4;
+constexpr int GROUP_UNROLL = 8;
+constexpr int GROUP_EDGES = 64 * GROUP_UNROLL;  // edge window of a row group
+constexpr int MAX_GROUP_ROWS = 8;
+constexpr int MAX_ATT_BLOCKS = MAX_ATTRACT_BLOCKS;
 
-int attract_blocks(int n) {
-    uint64_t b = ceil_div((uint64_t)(n > 0 ? n : 1), UP_THREADS / 64);
-    return (int)(b < 4096 ? b : 4096);
+uint32_t row_groups_count(uint64_t nnz) {
+    return (uint32_t)((nnz + GROUP_EDGES - 1) / GROUP_EDGES);
+}
+
+int attract_blocks(int ngroups) {
+    uint64_t b = ceil_div((uint64_t)(ngroups > 0 ? ngroups : 1), UP_THREADS / 64);
+    return (int)(b < (uint64_t)MAX_ATT_BLOCKS ? b : (uint64_t)MAX_ATT_BLOCKS);
+}
+
+// groups[m] = first row whose first edge is at or after m * GROUP_EDGES (rows are nondecreasing);
+// groups[ngroups] = n.  Group m is rows [groups[m], groups[m + 1]).
+__global__ void k_row_groups(const uint32_t *rows, int n, uint32_t ngroups, uint32_t *groups) {
+    const uint32_t m = blockIdx.x * 256 + threadIdx.x;
+    if (m > ngroups) return;
+    if (m == ngroups) {
+        groups[m] = (uint32_t)n;
+        return;
+    }
+    const uint64_t target = (uint64_t)m * GROUP_EDGES;
+    uint32_t lo = 0, hi = (uint32_t)n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if ((uint64_t)rows[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    groups[m] = lo;
+}
+
+void build_row_groups(const uint32_t *rows, int n, uint64_t nnz, uint32_t *groups,
+                      hipStream_t s) {
+    const uint32_t ng = row_groups_count(nnz);
+    hipLaunchKernelGGL(k_row_groups, dim3(ceil_div((uint64_t)ng + 1, 256)), dim3(256), 0, s, rows,
+                       n, ng, groups);
+    BHT_LAUNCH_CHECK();
+}
+
+struct StepCtx {
+    float inv_z, momentum;
+    float *y_next;
+};
+
+// Gradient step of row i from its attraction sums (mod.rs:470-516); returns the new coordinates.
+template <int D>
+__device__ __forceinline__ void update_row(const JobDev &J, const StepCtx &c, int i,
+                                           const float *acc, const float *yi, double &m0,
+                                           double &m1) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const size_t k = (size_t)i * D + d;
+        const float gradient = acc[d] - J.frep[k] * c.inv_z;
+        float uy = J.uy[k];
+        float gain = J.gains[k];
+        gain = rust_signum(gradient) != rust_signum(uy) ? gain + 0.2f : gain * 0.8f;
+        if (gain < 0.01f) gain = 0.01f;
+        uy = c.momentum * uy - J.learning_rate * gain * gradient;
+        const float yn = yi[d] + uy;
+        J.gains[k] = gain;
+        J.uy[k] = uy;
+        c.y_next[k] = yn;
+        if (d == 0) m0 += (double)yn;
+        else m1 += (double)yn;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ float2 load_y(const float *y, uint32_t i) {
+    if (D == 2) return reinterpret_cast<const float2 *>(y)[i];
+    return make_float2(y[i], 0.0f);
 }
 
 template <int D, bool UPDATE>
@@ -28,102 +102,136 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
     const int wave = (bc.lb * UP_THREADS + threadIdx.x) >> 6;
     const int nwaves = (bc.nb * UP_THREADS) >> 6;
     const int epoch = J.scal->epoch;
-    const int n = J.n;
     const float *y = J.ybuf[epoch & 1];
-    float *y_next = J.ybuf[(epoch + 1) & 1];
-    const uint32_t *rows = J.rows, *cols = J.cols;
+    const uint32_t *rows = J.rows, *cols = J.cols, *groups = J.groups;
     float *vals = J.vals;
     const float inv_exag = J.inv_exag;
-    float inv_z = 0.0f, momentum = J.momentum;
+    StepCtx sc{0.0f, J.momentum, J.ybuf[(epoch + 1) & 1]};
     bool stop_now = false;
     if (UPDATE) {
-        inv_z = J.scal->inv_z;
+        sc.inv_z = J.scal->inv_z;
         // stop_lying fires after epoch == stop_lying_epoch (lib.rs:1168-1171): the next epoch is
         // the first to read the rescaled values, and rescales them in place as it reads (with
         // the copies of the reference-position values that KL weights by).
         stop_now = J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
         // the momentum switch likewise takes effect after epoch == momentum_switch_epoch
-        momentum = epoch > J.momentum_switch_epoch ? J.final_momentum : J.momentum;
+        sc.momentum = epoch > J.momentum_switch_epoch ? J.final_momentum : J.momentum;
     }
     double m0 = 0.0, m1 = 0.0;
-    for (int i = wave; i < n; i += nwaves) {
-        float yi0, yi1 = 0.0f;
-        if (D == 2) {
-            const float2 v = reinterpret_cast<const float2 *>(y)[i];
-            yi0 = v.x;
-            yi1 = v.y;
-        } else {
-            yi0 = y[i];
+    for (int g = wave; g < J.ngroups; g += nwaves) {
+        const int r0 = (int)groups[g], r1 = (int)groups[g + 1];
+        const int nr = r1 - r0;
+        if (nr <= 0) continue;
+        if (UPDATE && stop_now)
+            for (int i = r0 + lane; i < r1 && i < J.qv_n; i += 64) J.qv[i] = J.qv[i] * inv_exag;
+        if (nr > MAX_GROUP_ROWS) {
+            // many short rows: one row at a time, lanes over its edges
+            for (int i = r0; i < r1; i++) {
+                const float2 yi = load_y<D>(y, i);
+                float a0 = 0.0f, a1 = 0.0f;
+                for (uint32_t e = rows[i] + lane; e < rows[i + 1]; e += 64) {
+                    float pv = vals[e];
+                    if (UPDATE && stop_now) {
+                        pv = pv * inv_exag;
+                        vals[e] = pv;
+                    }
+                    const float2 yj = load_y<D>(y, cols[e]);
+                    const float d0 = yi.x - yj.x;
+                    const float d1 = D == 2 ? yi.y - yj.y : 0.0f;
+                    float dist = 0.0f;
+                    dist += d0 * d0;
+                    if (D == 2) dist += d1 * d1;
+                    const float f = pv / (dist + 1.0f);
+                    a0 += f * d0;
+                    if (D == 2) a1 += f * d1;
+                }
+                a0 = wave_sum_f32(a0);
+                if (D == 2) a1 = wave_sum_f32(a1);
+                if (lane != 0) continue;
+                const float acc[2] = {a0, a1}, yv[2] = {yi.x, yi.y};
+                if (!UPDATE) {
+                    pos_out[(size_t)i * D] = a0;
+                    if (D == 2) pos_out[(size_t)i * D + 1] = a1;
+                } else {
+                    update_row<D>(J, sc, i, acc, yv, m0, m1);
+                }
+            }
+            continue;
         }
-        if (UPDATE && stop_now && lane == 0 && i < J.qv_n) J.qv[i] = J.qv[i] * inv_exag;
-        float acc0 = 0.0f, acc1 = 0.0f;
-        const uint32_t e0 = rows[i], e1 = rows[i + 1];
-        for (uint32_t base = e0; base < e1; base += 64 * UNROLL) {
-            uint32_t col[UNROLL];
-            float pv[UNROLL];
+        // the group's row offsets and positions (uniform across the wave)
+        uint32_t bnd[MAX_GROUP_ROWS + 1];
+        float2 yr[MAX_GROUP_ROWS];
 #pragma unroll
-            for (int u = 0; u < UNROLL; u++) {
+        for (int q = 0; q <= MAX_GROUP_ROWS; q++) bnd[q] = q <= nr ? rows[r0 + q] : 0xffffffffu;
+#pragma unroll
+        for (int q = 0; q < MAX_GROUP_ROWS; q++)
+            yr[q] = q < nr ? load_y<D>(y, r0 + q) : make_float2(0.0f, 0.0f);
+        float acc[MAX_GROUP_ROWS][2];
+#pragma unroll
+        for (int q = 0; q < MAX_GROUP_ROWS; q++) acc[q][0] = acc[q][1] = 0.0f;
+        const uint32_t e_end = bnd[nr];
+        for (uint32_t base = bnd[0]; base < e_end; base += GROUP_EDGES) {
+            uint32_t col[GROUP_UNROLL];
+            float pv[GROUP_UNROLL];
+#pragma unroll
+            for (int u = 0; u < GROUP_UNROLL; u++) {
                 const uint32_t e = base + lane + 64 * u;
-                const bool ok = e < e1;
-                col[u] = ok ? __builtin_nontemporal_load(cols + e) : (uint32_t)i;
+                const bool ok = e < e_end;
+                col[u] = ok ? __builtin_nontemporal_load(cols + e) : (uint32_t)r0;
                 pv[u] = ok ? __builtin_nontemporal_load(vals + e) : 0.0f;
                 if (UPDATE && stop_now && ok) {
                     pv[u] = pv[u] * inv_exag;
                     vals[e] = pv[u];
                 }
             }
-            float yj0[UNROLL], yj1[UNROLL];
+            float2 yj[GROUP_UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; u++) {
-                if (D == 2) {
-                    const float2 v = reinterpret_cast<const float2 *>(y)[col[u]];
-                    yj0[u] = v.x;
-                    yj1[u] = v.y;
-                } else {
-                    yj0[u] = y[col[u]];
-                    yj1[u] = 0.0f;
-                }
-            }
+            for (int u = 0; u < GROUP_UNROLL; u++) yj[u] = load_y<D>(y, col[u]);
 #pragma unroll
-            for (int u = 0; u < UNROLL; u++) {
-                const float d0 = yi0 - yj0[u];
+            for (int u = 0; u < GROUP_UNROLL; u++) {
+                const uint32_t e = base + lane + 64 * u;
+                int q = 0;
+#pragma unroll
+                for (int t = 1; t < MAX_GROUP_ROWS; t++) q += e >= bnd[t] ? 1 : 0;
+                float2 yi = yr[0];
+#pragma unroll
+                for (int t = 1; t < MAX_GROUP_ROWS; t++) yi = q == t ? yr[t] : yi;
+                const float d0 = yi.x - yj[u].x;
+                const float d1 = D == 2 ? yi.y - yj[u].y : 0.0f;
                 float dist = 0.0f;
                 dist += d0 * d0;
-                float d1 = 0.0f;
-                if (D == 2) {
-                    d1 = yi1 - yj1[u];
-                    dist += d1 * d1;
+                if (D == 2) dist += d1 * d1;
+                const float f = pv[u] / (dist + 1.0f);
+                const float f0 = f * d0, f1 = f * d1;
+#pragma unroll
+                for (int t = 0; t < MAX_GROUP_ROWS; t++) {
+                    acc[t][0] += q == t ? f0 : 0.0f;
+                    if (D == 2) acc[t][1] += q == t ? f1 : 0.0f;
                 }
-                const float factor = pv[u] / (dist + 1.0f);
-                acc0 += factor * d0;
-                if (D == 2) acc1 += factor * d1;
             }
         }
-        acc0 = wave_sum_f32(acc0);
-        if (D == 2) acc1 = wave_sum_f32(acc1);
-        if (lane != 0) continue;
-        if (!UPDATE) {
-            pos_out[(size_t)i * D] = acc0;
-            if (D == 2) pos_out[(size_t)i * D + 1] = acc1;
-            continue;
-        }
-        const float acc[2] = {acc0, acc1};
-        const float yi[2] = {yi0, yi1};
+        // row sums: lane q keeps the sums of the group's q-th row
+        float mine0 = 0.0f, mine1 = 0.0f;
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            const size_t k = (size_t)i * D + d;
-            const float gradient = acc[d] - J.frep[k] * inv_z;
-            float uy = J.uy[k];
-            float gain = J.gains[k];
-            gain = rust_signum(gradient) != rust_signum(uy) ? gain + 0.2f : gain * 0.8f;
-            if (gain < 0.01f) gain = 0.01f;
-            uy = momentum * uy - J.learning_rate * gain * gradient;
-            const float yn = yi[d] + uy;
-            J.gains[k] = gain;
-            J.uy[k] = uy;
-            y_next[k] = yn;
-            if (d == 0) m0 += (double)yn;
-            else m1 += (double)yn;
+        for (int t = 0; t < MAX_GROUP_ROWS; t++) {
+            if (t >= nr) break;
+            const float s0 = wave_sum_f32(acc[t][0]);
+            const float s1 = D == 2 ? wave_sum_f32(acc[t][1]) : 0.0f;
+            if (lane == t) {
+                mine0 = s0;
+                mine1 = s1;
+            }
+        }
+        if (lane >= nr) continue;
+        const int i = r0 + lane;
+        const float accv[2] = {mine0, mine1};
+        if (!UPDATE) {
+            pos_out[(size_t)i * D] = mine0;
+            if (D == 2) pos_out[(size_t)i * D + 1] = mine1;
+        } else {
+            const float2 yi = load_y<D>(y, i);
+            const float yv[2] = {yi.x, yi.y};
+            update_row<D>(J, sc, i, accv, yv, m0, m1);
         }
     }
     if (!UPDATE) return;

@@ -91,6 +91,8 @@ struct JobDev {
     float *uy, *gains, *frep;
     const uint32_t *rows, *cols;
     float *vals;
+    const uint32_t *groups;  // row groups of the attraction kernel (ngroups + 1 row indices)
+    int ngroups;
     float *qv;          // values at the reference's CSR positions [0, qv_n) (KL weights)
     int qv_n;
     EpochScalars *scal;
@@ -132,7 +134,11 @@ __device__ __forceinline__ float *job_y(const JobDev &J) { return J.ybuf[J.scal-
 
 // Block counts of one job (the same formulas size the single-job grids and the batch maps).
 int point_blocks(int n);
-int attract_blocks(int n);
+int attract_blocks(int ngroups);
+constexpr int MAX_ATTRACT_BLOCKS = 16384;  // partial sums of the update: 2 doubles per block
+// Row groups of the attraction kernel for a CSR of nnz entries (attract.hip).
+uint32_t row_groups_count(uint64_t nnz);
+void build_row_groups(const uint32_t *rows, int n, uint64_t nnz, uint32_t *groups, hipStream_t s);
 int spread_blocks(int n);
 int traverse_blocks(int n);
 void job_block_counts(JobDev &J, bool batched);
