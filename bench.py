@@ -204,7 +204,7 @@ def cpu_baseline(aff, y_seed, state_late, min_boxes, n_epochs=3):
         log(f"cpu baseline {label}: {dt / n_epochs:.3f} s/epoch")
     per_epoch = float(np.mean(times))
     return {"value": 1.0 / per_epoch, "unit": "epochs/s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle (C restatement of the reference loop, reference grid >= "
+            "sample": f"oracle (C restatement of the reference loop, the same grid >= "
                       f"{min_boxes} boxes) on this host: {n_epochs} epochs from the seed state + "
                       f"{n_epochs} epochs from the GPU run's final state, mean s/epoch "
                       f"(early {times[0]:.3f}, late {times[1]:.3f})"}
@@ -217,7 +217,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-epochs", type=int, default=3)
+    ap.add_argument("--cpu-epochs", type=int, default=40,
+                    help="CPU baseline: epochs timed from the seed state and from the final state")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="leave out the batched and serving summaries of the default line")
     ap.add_argument("--jobs", type=int, default=256, help="batched workload: number of jobs")
     ap.add_argument("--job-epochs", type=int, default=1000,
                     help="batched / serving workloads: epochs per job")
@@ -303,7 +306,19 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(aff, y_seed, state, 10, args.cpu_epochs)
+        cpu = cpu_baseline(aff, y_seed, state, wl["min_boxes"], args.cpu_epochs)
+    del aff, state, t
+    batched = serving_line = None
+    if args.workload == "c3" and not args.no_batched:
+        # the metric's second half, on reduced streams: 64 batched jobs against their plain fits,
+        # and 24 requests at 12/s through the request loop against plain FIFO fits
+        sub = argparse.Namespace(**vars(args))
+        sub.jobs, sub.plain_stride, sub.job_epochs = 64, 4, 1000
+        b = bench_batched(sub)
+        batched = {k: b[k] for k in ("config", "batched", "plain", "vs_plain", "parity")}
+        sub.requests, sub.rate, sub.step_epochs = 24, 12.0, 25
+        v = bench_serving(sub)
+        serving_line = {k: v[k] for k in ("config", "batched", "plain", "vs_plain", "parity")}
 
     out = {"metric": METRIC, "value": round(eps, 3), "unit": "epochs/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup,
@@ -313,7 +328,8 @@ def main():
                       "perplexity": 30, "epochs": args.steps, "min_boxes": wl["min_boxes"],
                       "nnz": nnz, "affinity_build_s": round(t_aff, 3),
                       "final_grid": grid, "parallelism": "single GPU"},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu,
+           "batched_jobs": batched, "serving": serving_line}
     print(json.dumps(out), flush=True)
 
 

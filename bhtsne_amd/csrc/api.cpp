@@ -301,7 +301,7 @@ void ensure_fitsne_workspace(bht_job *j) {
     j->svals.ensure(c, n * 4);
     j->keys_alt.ensure(c, n * 4);
     j->svals_alt.ensure(c, n * 4);
-    j->sort_temp.ensure(c, radix_sort_temp_bytes(n));
+    j->sort_temp.ensure(c, group_sort_temp_bytes(n));
     const size_t nodes_total = j->dim() == 2 ? (size_t)j->nodes_cap * j->nodes_cap : j->nodes_cap;
     j->grid.ensure(c, nodes_total * sizeof(float4));
     if (j->dim() == 2) j->spec.ensure(c, 2 * (size_t)j->nodes_cap * j->len_cap * sizeof(float2));
@@ -330,7 +330,7 @@ void ensure_bh_workspace(bht_job *j) {
     j->codes_alt.ensure(c, n * 8);
     j->bidx.ensure(c, n * 4);
     j->bidx_alt.ensure(c, n * 4);
-    j->bsort_temp.ensure(c, radix_sort_temp_bytes(n));
+    j->bsort_temp.ensure(c, group_sort_temp_bytes(n));
     j->bnode.ensure(c, 2 * n * sizeof(float4));
     j->bmeta.ensure(c, 2 * n * 4);
     j->brange.ensure(c, 4 * n * 4);
@@ -941,7 +941,7 @@ void batch_rebuild(bht_batch *b) {
         g.keys_alt.ensure(c, arena * ksize);
         g.vals.ensure(c, arena * 4);
         g.vals_alt.ensure(c, arena * 4);
-        g.sort_temp.ensure(c, radix_sort_temp_bytes(arena));
+        g.sort_temp.ensure(c, group_sort_temp_bytes(arena));
         g.segs.ensure(c, segs.size() * sizeof(SortSeg));
         g.tile_seg.ensure(c, std::max<size_t>(tile_seg.size(), 1) * 4);
         BHT_HIP(hipMemcpy(g.segs.p, segs.data(), segs.size() * sizeof(SortSeg),
@@ -1690,20 +1690,20 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
     b += 2 * (uint64_t)MAX_ATTRACT_BLOCKS * 8;      // update partials
     b += ne * 12 * 4 + radix_sort_temp_bytes(ne);   // affinity-build peak (transient)
     if (method == BHT_METHOD_BARNES_HUT) {
-        b += n * (2 * 8 + 2 * 4) + radix_sort_temp_bytes(n);  // Morton sort
+        b += n * (2 * 8 + 2 * 4) + group_sort_temp_bytes(n);  // Morton sort
         b += 2 * n * 16 + 2 * n * 4 + 4 * n * 4;               // tree
-        b += pad_tile(n) * 24 + radix_sort_temp_bytes(pad_tile(n));  // batch arena
+        b += pad_tile(n) * 24 + group_sort_temp_bytes(pad_tile(n));  // batch arena
     } else {
         uint32_t maxb = std::min<uint32_t>(std::max<uint32_t>(p.max_boxes, 1), MAX_BOXES);
         maxb = std::max(maxb, std::min<uint32_t>(std::max<uint32_t>(p.min_boxes, 1), MAX_BOXES));
         const uint64_t nodes = 3ull * maxb, len = (uint64_t)next_smooth(6 * (int64_t)maxb - 1);
         const uint64_t h = len / 2 + 1;
-        b += n * 16 + radix_sort_temp_bytes(n);                   // box sort
+        b += n * 16 + group_sort_temp_bytes(n);                   // box sort
         b += (D == 2 ? nodes * nodes : nodes) * 16;               // node grid
         if (D == 2) b += 2 * nodes * len * 8;                     // packed spectra
         b += 2 * h * h * 4;                                       // kernel transforms
         b += 2 * ((n + 63) / 64) * 36 * 4;                        // spread partials
-        b += pad_tile(n) * 16 + radix_sort_temp_bytes(pad_tile(n));  // batch arena
+        b += pad_tile(n) * 16 + group_sort_temp_bytes(pad_tile(n));  // batch arena
     }
     *bytes = b;
     return BHT_OK;

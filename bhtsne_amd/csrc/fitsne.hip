@@ -93,8 +93,11 @@ __device__ __forceinline__ int locate_box(float v, const GridPlan &p, float &loc
 
 // Mean of the last update's per-block sums (written by the attraction kernel), applied once;
 // advances the job's epoch index (which selects its current embedding buffer).
-__global__ void __launch_bounds__(256) k_mean_finalize(const JobDev *jobs, const uint32_t *map) {
-    __shared__ double red[4];
+constexpr int FIN_THREADS = 1024;  // up to 16384 attraction blocks' partials per job
+
+__global__ void __launch_bounds__(FIN_THREADS) k_mean_finalize(const JobDev *jobs,
+                                                               const uint32_t *map) {
+    __shared__ double red[FIN_THREADS / 64];
     const BlockCtx bc = block_ctx(map, jobs, F_ONE);
     const JobDev &J = jobs[bc.job];
     EpochScalars *scal = J.scal;
@@ -109,7 +112,7 @@ __global__ void __launch_bounds__(256) k_mean_finalize(const JobDev *jobs, const
         return;
     }
     double t0 = 0.0, t1 = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += 256) {
+    for (int b = threadIdx.x; b < nparts; b += FIN_THREADS) {
         t0 += mpart[2 * b];
         t1 += mpart[2 * b + 1];
     }
@@ -226,7 +229,8 @@ __global__ void __launch_bounds__(256) k_plan(const JobDev *jobs, const uint32_t
 void center_plan(const LaunchSet &L, bool do_plan, hipStream_t s) {
     const int *lo = L.tw ? L.tw->len_of_boxes : nullptr, *of = L.tw ? L.tw->off_of_boxes : nullptr;
     const int plan = do_plan && L.tw != nullptr;
-    hipLaunchKernelGGL(k_mean_finalize, dim3(L.nb[F_ONE]), dim3(256), 0, s, L.jobs, L.map[F_ONE]);
+    hipLaunchKernelGGL(k_mean_finalize, dim3(L.nb[F_ONE]), dim3(FIN_THREADS), 0, s, L.jobs,
+                       L.map[F_ONE]);
     if (L.dim == 2) {
         hipLaunchKernelGGL(k_center<2>, dim3(L.nb[F_PT]), dim3(PT_THREADS), 0, s, L.jobs,
                            L.map[F_PT]);

@@ -179,22 +179,23 @@ template <typename T>
 void scan_exclusive(const uint32_t *in, T *out, uint64_t n, void *temp, T *total_out,
                     hipStream_t s);
 constexpr int RS_TILE_KEYS = 2048;      // keys per radix-sort tile (segment alignment)
+// Stable LSD radix sort of (key, u32 value) pairs, 8-bit digits (affinity construction).
 size_t radix_sort_temp_bytes(uint64_t n);
 template <typename K>
 bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
                       int begin_bit, int end_bit, void *temp, hipStream_t s);
-// Segmented form: each segment sorted independently in place of its own range; ntiles tiles.
-template <typename K>
-bool radix_sort_pairs_seg(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt,
-                          const SortSeg *segs, const uint32_t *tile_seg, uint32_t ntiles,
-                          int begin_bit, int end_bit, void *temp, hipStream_t s);
-// Sorts the group's arena: the plain sort for one job, the segmented one for a batch.
+// The per-epoch sorts of a launch group (box keys, Morton codes) with wider digits: the plain
+// sort for one job, the segmented one for a batch.  Temp: group_sort_temp_bytes of the arena.
+constexpr int GROUP_SORT_BITS = 10;
+size_t group_sort_temp_bytes(uint64_t n);
 template <typename K>
 bool sort_group(const LaunchSet &L, int end_bit, hipStream_t s);
 // Key bits of the per-epoch sorts and whether their result lands in the alternate buffers.
-inline int box_key_bits(int dim) { return dim == 2 ? 24 : 16; }  // boxes^D <= 10^6 < 2^20
+inline int box_key_bits(int dim) { return dim == 2 ? 20 : 10; }  // boxes^D <= 1000^D
 constexpr int MORTON_KEY_BITS = 64;
-inline bool sort_result_alt(int end_bit) { return (((end_bit + 7) / 8) & 1) != 0; }
+inline bool sort_result_alt(int end_bit) {
+    return (((end_bit + GROUP_SORT_BITS - 1) / GROUP_SORT_BITS) & 1) != 0;
+}
 
 // ---- FIt-SNE epoch (fitsne.hip, fft.hip) ---------------------------------------------------
 // Applies the pending recentring of the last update, then bounds and (do_plan) the grid plan.

@@ -122,7 +122,6 @@ namespace {
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 2048 keys per tile
-constexpr int RS_BINS = 256;
 
 static_assert(RS_TILE == RS_TILE_KEYS, "segment alignment must match the tile size");
 
@@ -139,6 +138,7 @@ struct TileCtx {
     uint64_t out_adj;      // added to the scanned offset: segment base - keys before it
 };
 
+template <int BINS>
 __device__ __forceinline__ TileCtx tile_ctx(uint64_t n, unsigned ntiles, const SortSeg *segs,
                                             const uint32_t *tile_seg) {
     TileCtx c;
@@ -154,56 +154,64 @@ __device__ __forceinline__ TileCtx tile_ctx(uint64_t n, unsigned ntiles, const S
     const unsigned lt = blockIdx.x - g.tile0;
     c.first = g.base + (uint64_t)lt * RS_TILE;
     c.end = g.base + g.n;
-    c.h0 = (size_t)RS_BINS * g.tile0 + lt;
+    c.h0 = (size_t)BINS * g.tile0 + lt;
     c.hstride = g.ntiles;
     c.out_adj = g.base - g.dense0;
     return c;
 }
 
-template <typename K>
+// Digit counts of one tile (BITS-bit digits at `shift`).
+template <typename K, int BITS>
 __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K *keys, uint64_t n, int shift,
                                                         uint32_t *hist, unsigned ntiles,
                                                         const SortSeg *segs,
                                                         const uint32_t *tile_seg) {
-    __shared__ uint32_t h[RS_BINS];
-    const TileCtx tc = tile_ctx(n, ntiles, segs, tile_seg);
-    h[threadIdx.x] = 0;
+    constexpr int BINS = 1 << BITS;
+    __shared__ uint32_t h[BINS];
+    const TileCtx tc = tile_ctx<BINS>(n, ntiles, segs, tile_seg);
+    for (int b = threadIdx.x; b < BINS; b += RS_THREADS) h[b] = 0;
     __syncthreads();
     for (int r = 0; r < RS_ITEMS; r++) {
         uint64_t idx = tc.first + (uint64_t)r * RS_THREADS + threadIdx.x;
-        if (idx < tc.end) atomicAdd(&h[(unsigned)(keys[idx] >> shift) & 0xffu], 1u);
+        if (idx < tc.end) atomicAdd(&h[(unsigned)(keys[idx] >> shift) & (BINS - 1u)], 1u);
     }
     __syncthreads();
-    hist[tc.h0 + (size_t)threadIdx.x * tc.hstride] = h[threadIdx.x];
+    for (int b = threadIdx.x; b < BINS; b += RS_THREADS)
+        hist[tc.h0 + (size_t)b * tc.hstride] = h[b];
 }
 
-template <typename K>
+// Stable scatter of one tile: rounds of 256 keys; within a round a key's rank among the equal
+// digits of its wave comes from one ballot per digit bit, and the waves before it add their
+// counts.
+template <typename K, int BITS>
 __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *keys, const uint32_t *vals,
                                                            K *keys_out, uint32_t *vals_out,
                                                            uint64_t n, int shift,
                                                            const uint32_t *offs,
                                                            unsigned ntiles, const SortSeg *segs,
                                                            const uint32_t *tile_seg) {
-    __shared__ uint32_t base[RS_BINS];
-    __shared__ uint32_t wcnt[RS_THREADS / 64][RS_BINS];
+    constexpr int BINS = 1 << BITS;
+    __shared__ uint32_t base[BINS];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][BINS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const TileCtx tc = tile_ctx(n, ntiles, segs, tile_seg);
-    base[threadIdx.x] =
-        offs[tc.h0 + (size_t)threadIdx.x * tc.hstride] + (uint32_t)tc.out_adj;
+    const TileCtx tc = tile_ctx<BINS>(n, ntiles, segs, tile_seg);
+    for (int b = threadIdx.x; b < BINS; b += RS_THREADS)
+        base[b] = offs[tc.h0 + (size_t)b * tc.hstride] + (uint32_t)tc.out_adj;
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint64_t tile0 = tc.first;
     n = tc.end;
     for (int r = 0; r < RS_ITEMS; r++) {
+        for (int b = threadIdx.x; b < BINS; b += RS_THREADS)
 #pragma unroll
-        for (int q = 0; q < RS_THREADS / 64; q++) wcnt[q][threadIdx.x] = 0;
+            for (int q = 0; q < RS_THREADS / 64; q++) wcnt[q][b] = 0;
         __syncthreads();
         uint64_t idx = tile0 + (uint64_t)r * RS_THREADS + threadIdx.x;
         bool valid = idx < n;
         K key = valid ? keys[idx] : (K)0;
-        unsigned d = (unsigned)(key >> shift) & 0xffu;
+        unsigned d = (unsigned)(key >> shift) & (BINS - 1u);
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < BITS; b++) {
             uint64_t m = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? m : ~m;
         }
@@ -217,41 +225,47 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *keys, const 
             vals_out[pos] = vals[idx];
         }
         __syncthreads();
-        uint32_t add = 0;
+        for (int b = threadIdx.x; b < BINS; b += RS_THREADS) {
+            uint32_t add = 0;
 #pragma unroll
-        for (int q = 0; q < RS_THREADS / 64; q++) add += wcnt[q][threadIdx.x];
-        base[threadIdx.x] += add;
+            for (int q = 0; q < RS_THREADS / 64; q++) add += wcnt[q][b];
+            base[b] += add;
+        }
         __syncthreads();
     }
 }
 
 }  // namespace
 
-size_t radix_sort_temp_bytes(uint64_t n) {
+static size_t temp_bytes_bins(uint64_t n, int bins) {
     uint64_t ntiles = ceil_div(n, RS_TILE);
-    uint64_t hist = ntiles * RS_BINS;
+    uint64_t hist = ntiles * (uint64_t)bins;
     return (size_t)hist * 2 * sizeof(uint32_t) + scan_temp_bytes(hist) + 256;
 }
 
-// Sorts (keys, vals) by bits [begin_bit, end_bit) of the key, stable.  Ping-pongs between the
-// two buffer pairs; returns true when the result is in (keys_alt, vals_alt).
-template <typename K>
+size_t radix_sort_temp_bytes(uint64_t n) { return temp_bytes_bins(n, 1 << 8); }
+size_t group_sort_temp_bytes(uint64_t n) { return temp_bytes_bins(n, 1 << GROUP_SORT_BITS); }
+
+// Sorts (keys, vals) by bits [begin_bit, end_bit) of the key, stable, BITS bits per pass.
+// Ping-pongs between the two buffer pairs; returns true when the result is in (keys_alt,
+// vals_alt).
+template <typename K, int BITS>
 static bool radix_passes(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
                          unsigned ntiles, const SortSeg *segs, const uint32_t *tile_seg,
                          int begin_bit, int end_bit, void *temp, hipStream_t s) {
-    uint64_t nh = (uint64_t)ntiles * RS_BINS;
+    uint64_t nh = (uint64_t)ntiles << BITS;
     uint32_t *hist = (uint32_t *)temp;
     uint32_t *offs = hist + nh;
     void *stemp = (void *)(((uintptr_t)(offs + nh) + 255) & ~(uintptr_t)255);
     K *ki = keys, *ko = keys_alt;
     uint32_t *vi = vals, *vo = vals_alt;
     bool alt = false;
-    for (int shift = begin_bit; shift < end_bit; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist<K>, dim3(ntiles), dim3(RS_THREADS), 0, s, ki, n, shift, hist,
-                           ntiles, segs, tile_seg);
+    for (int shift = begin_bit; shift < end_bit; shift += BITS) {
+        hipLaunchKernelGGL((k_rs_hist<K, BITS>), dim3(ntiles), dim3(RS_THREADS), 0, s, ki, n,
+                           shift, hist, ntiles, segs, tile_seg);
         scan_exclusive<uint32_t>(hist, offs, nh, stemp, nullptr, s);
-        hipLaunchKernelGGL(k_rs_scatter<K>, dim3(ntiles), dim3(RS_THREADS), 0, s, ki, vi, ko, vo,
-                           n, shift, offs, ntiles, segs, tile_seg);
+        hipLaunchKernelGGL((k_rs_scatter<K, BITS>), dim3(ntiles), dim3(RS_THREADS), 0, s, ki, vi,
+                           ko, vo, n, shift, offs, ntiles, segs, tile_seg);
         BHT_LAUNCH_CHECK();
         K *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;
@@ -264,39 +278,31 @@ template <typename K>
 bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
                       int begin_bit, int end_bit, void *temp, hipStream_t s) {
     if (n == 0) return false;
-    return radix_passes<K>(keys, vals, keys_alt, vals_alt, n, ceil_div(n, RS_TILE), nullptr,
-                           nullptr, begin_bit, end_bit, temp, s);
+    return radix_passes<K, 8>(keys, vals, keys_alt, vals_alt, n, ceil_div(n, RS_TILE), nullptr,
+                              nullptr, begin_bit, end_bit, temp, s);
 }
 
-// The temp space of a segmented sort is radix_sort_temp_bytes(ntiles * RS_TILE).
-template <typename K>
-bool radix_sort_pairs_seg(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt,
-                          const SortSeg *segs, const uint32_t *tile_seg, uint32_t ntiles,
-                          int begin_bit, int end_bit, void *temp, hipStream_t s) {
-    if (ntiles == 0) return false;
-    return radix_passes<K>(keys, vals, keys_alt, vals_alt, (uint64_t)ntiles * RS_TILE, ntiles,
-                           segs, tile_seg, begin_bit, end_bit, temp, s);
-}
-
+// The per-epoch sorts: GROUP_SORT_BITS-bit digits; one segment (a job alone) or one segment per
+// member (a batch; the arena's tiles are ntiles * RS_TILE keys, temp group_sort_temp_bytes of
+// that).
 template <typename K>
 bool sort_group(const LaunchSet &L, int end_bit, hipStream_t s) {
-    if (L.segs)
-        return radix_sort_pairs_seg<K>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt, L.segs,
-                                       L.tile_seg, L.sort_tiles, 0, end_bit, L.sort_temp, s);
-    return radix_sort_pairs<K>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt, L.sort_n, 0,
-                               end_bit, L.sort_temp, s);
+    if (L.segs) {
+        if (L.sort_tiles == 0) return false;
+        return radix_passes<K, GROUP_SORT_BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt,
+                                                (uint64_t)L.sort_tiles * RS_TILE, L.sort_tiles,
+                                                L.segs, L.tile_seg, 0, end_bit, L.sort_temp, s);
+    }
+    if (L.sort_n == 0) return false;
+    return radix_passes<K, GROUP_SORT_BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt,
+                                            L.sort_n, ceil_div(L.sort_n, RS_TILE), nullptr,
+                                            nullptr, 0, end_bit, L.sort_temp, s);
 }
 
 template bool radix_sort_pairs<uint32_t>(uint32_t *, uint32_t *, uint32_t *, uint32_t *, uint64_t,
                                          int, int, void *, hipStream_t);
 template bool radix_sort_pairs<uint64_t>(uint64_t *, uint32_t *, uint64_t *, uint32_t *, uint64_t,
                                          int, int, void *, hipStream_t);
-template bool radix_sort_pairs_seg<uint32_t>(uint32_t *, uint32_t *, uint32_t *, uint32_t *,
-                                             const SortSeg *, const uint32_t *, uint32_t, int, int,
-                                             void *, hipStream_t);
-template bool radix_sort_pairs_seg<uint64_t>(uint64_t *, uint32_t *, uint64_t *, uint32_t *,
-                                             const SortSeg *, const uint32_t *, uint32_t, int, int,
-                                             void *, hipStream_t);
 template bool sort_group<uint32_t>(const LaunchSet &, int, hipStream_t);
 template bool sort_group<uint64_t>(const LaunchSet &, int, hipStream_t);
 
