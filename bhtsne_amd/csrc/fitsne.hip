@@ -312,8 +312,12 @@ __global__ void __launch_bounds__(256) k_zero_grid(const JobDev *jobs, const uin
 // ---- 3. box-segmented spreading (interpolation.rs:247-269) ----------------------------------
 //
 // Points sorted by box form runs; each box's 3^D nodes receive only that run's charges.  One wave
-// reduces 64 sorted points with a segmented scan; a run wholly inside a chunk is written directly,
-// a run crossing chunk boundaries leaves partials that k_spread_fixup adds in chunk order.
+// takes 64 sorted points: each lane writes its point's (3^D nodes x terms) contributions as a row
+// of an LDS slab, then lane t walks the rows summing term t, closing a sum at every run end
+// (wave-uniform masks of run ends).  Within a box the charges are therefore added in point order,
+// as the reference's sequential spreading adds them.  A run wholly inside a chunk is written
+// directly; a run crossing chunk boundaries leaves partials that k_spread_fixup adds in chunk
+// order.
 
 constexpr int NV2 = 36;  // 9 nodes x 4 terms (D = 2)
 constexpr int NV1 = 9;   // 3 nodes x 3 terms (D = 1)
@@ -374,6 +378,24 @@ __device__ __forceinline__ void write_box(float4 *grid, const GridPlan &p, uint3
     }
 }
 
+// One term of a box's node values (write_box for a single lane's term t).
+template <int D>
+__device__ __forceinline__ void write_box_term(float4 *grid, const GridPlan &p, uint32_t key,
+                                               int t, float value) {
+    float *g = reinterpret_cast<float *>(grid);
+    if (D == 2) {
+        const uint32_t b0 = key / (uint32_t)p.boxes, b1 = key % (uint32_t)p.boxes;
+        const int c = t >> 2, k0 = c % 3, k1 = c / 3;
+        const size_t node = (size_t)(3 * b0 + k0) * p.nodes + (3 * b1 + k1);
+        g[node * 4 + (t & 3)] = value;
+    } else {
+        g[(size_t)(3 * key + t / 3) * 4 + t % 3] = value;
+    }
+}
+
+constexpr int SLAB_ROW = 40;             // floats per point row of the spreading slab (>= NV2)
+constexpr int SLAB_ROW4 = SLAB_ROW / 4;
+
 template <int D>
 __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const uint32_t *map) {
     constexpr int NV = D == 2 ? NV2 : NV1;
@@ -392,45 +414,69 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const
     const int nwaves = (bc.nb * PT_THREADS) >> 6;
     float *part_first = part;
     float *part_last = part + (size_t)nchunks * NV;
+    // the chunk's contributions, one row per point; lane t then sums term t down the rows
+    __shared__ float4 slab_mem[PT_THREADS / 64][64][SLAB_ROW4];
+    float *slab = reinterpret_cast<float *>(slab_mem[threadIdx.x >> 6]);
     for (int c = wave0; c < nchunks; c += nwaves) {
-        int s = c * 64 + lane;
-        bool valid = s < n;
-        uint32_t key = valid ? keys[s] : 0xffffffffu;
-        float v[NV];
-        if (valid) {
-            point_contrib<D>(y, vals[s], p, lb, v);
-        } else {
+        const int s = c * 64 + lane;
+        const bool valid = s < n;
+        const uint32_t key = valid ? keys[s] : 0xffffffffu;
+        {
+            float v[NV];
+            if (valid) {
+                point_contrib<D>(y, vals[s], p, lb, v);
+            } else {
 #pragma unroll
-            for (int t = 0; t < NV; t++) v[t] = 0.0f;
-        }
-        uint32_t prev = __shfl_up(key, 1, 64);
-        bool head = lane == 0 || prev != key;
-        uint64_t heads = __ballot(head);
-        uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-        int my_head = 63 - __clzll(heads & upto);
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            bool take = lane - off >= my_head;
-#pragma unroll
-            for (int t = 0; t < NV; t++) {
-                float u = __shfl_up(v[t], off, 64);
-                if (take) v[t] += u;
+                for (int t = 0; t < NV; t++) v[t] = 0.0f;
             }
+            float4 *row = reinterpret_cast<float4 *>(slab + lane * SLAB_ROW);
+#pragma unroll
+            for (int q = 0; q < (NV + 3) / 4; q++)
+                row[q] = make_float4(v[4 * q], 4 * q + 1 < NV ? v[4 * q + 1] : 0.0f,
+                                     4 * q + 2 < NV ? v[4 * q + 2] : 0.0f,
+                                     4 * q + 3 < NV ? v[4 * q + 3] : 0.0f);
         }
-        if (!valid) continue;
-        bool tail = (s == n - 1) || lane == 63 || ((heads >> (lane + 1)) & 1ull);
-        if (!tail) continue;
-        int s_head = c * 64 + my_head;
-        bool starts = s_head == 0 || keys[s_head - 1] != key;
-        bool ends = s == n - 1 || keys[s + 1] != key;
-        if (starts && ends) {
-            write_box<D>(grid, p, key, v);
-        } else {
-            if (!starts)
-                for (int t = 0; t < NV; t++) part_first[(size_t)c * NV + t] = v[t];
-            if (!ends)
-                for (int t = 0; t < NV; t++) part_last[(size_t)c * NV + t] = v[t];
+        // run structure of the chunk, as wave-uniform masks: a run ends at lane p (a tail) when
+        // the next sorted key differs or the chunk ends; per tail, whether the run began in this
+        // chunk and whether it continues past it
+        const uint32_t next = __shfl_down(key, 1, 64);
+        const bool tail = valid && (s == n - 1 || lane == 63 || next != key);
+        const uint32_t prev = __shfl_up(key, 1, 64);
+        const bool head = valid && (lane == 0 || prev != key);
+        const uint64_t heads = __ballot(head), tails = __ballot(tail);
+        bool starts = false, ends = false;
+        if (tail) {
+            const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+            const int my_head = 63 - __clzll(heads & upto);
+            const int s_head = c * 64 + my_head;
+            starts = s_head == 0 || keys[s_head - 1] != key;
+            ends = s == n - 1 || keys[s + 1] != key;
         }
+        const uint64_t starts_m = __ballot(starts), ends_m = __ballot(ends);
+        __builtin_amdgcn_wave_barrier();
+        // lane t < NV: sequential sum of term t over each run (point order within a box)
+        float acc = 0.0f;
+        uint64_t rest = tails;
+        int row = 0;
+        while (rest) {
+            const int pt = __ffsll((long long)rest) - 1;  // uniform: next run end
+            rest &= rest - 1;
+            if (lane < NV)
+                for (; row <= pt; row++) acc += slab[row * SLAB_ROW + lane];
+            row = pt + 1;
+            const uint32_t rkey = __shfl(key, pt, 64);
+            const bool rs = (starts_m >> pt) & 1ull, re = (ends_m >> pt) & 1ull;
+            if (lane < NV) {
+                if (rs && re) {
+                    write_box_term<D>(grid, p, rkey, lane, acc);
+                } else {
+                    if (!rs) part_first[(size_t)c * NV + lane] = acc;
+                    if (!re) part_last[(size_t)c * NV + lane] = acc;
+                }
+            }
+            acc = 0.0f;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
