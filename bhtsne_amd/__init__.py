@@ -425,6 +425,20 @@ def knn(x, k: int, ctx: Optional[Context] = None, mode: str = "auto", stats: boo
     return (idx, dist, int(rows.value)) if stats else (idx, dist)
 
 
+def exact_repulsion_f64(y, ctx: Optional[Context] = None):
+    """FP64 brute-force non-edge gradient terms and Z (the exact path, src/lib.rs:570-622)."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    y = np.ascontiguousarray(y, np.float32)
+    if y.ndim == 1:
+        y = y.reshape(-1, 1)
+    n, d = y.shape
+    neg = np.zeros((n, d), np.float64)
+    z = C.c_double(0.0)
+    check(L.bht_exact_repulsion_f64(ctx.handle, ptr(y), n, d, ptr(neg), C.byref(z)))
+    return neg, z.value
+
+
 def search_beta(dist, perplexity: float, mode: int = _lib.BANDWIDTH_FP64,
                 ctx: Optional[Context] = None):
     """Per-row bandwidth search (src/tsne/mod.rs:201-275)."""

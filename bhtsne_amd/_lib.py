@@ -94,6 +94,8 @@ SIGNATURES = {
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     "bht_knn_f32_ex": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "bht_exact_repulsion_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
+                                            C.c_void_p, C.POINTER(C.c_double)]),
     "bht_batch_create": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "bht_batch_destroy": (None, [C.c_void_p]),
     "bht_batch_add": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float]),

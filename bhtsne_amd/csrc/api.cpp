@@ -1521,6 +1521,30 @@ bht_status bht_knn_f32_ex(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_i
     });
 }
 
+bht_status bht_exact_repulsion_f64(bht_ctx *ctx, const float *y, uint64_t n, uint32_t d,
+                                   double *neg_out, double *z_out) {
+    if (!ctx || !y || !neg_out || !z_out) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (d != 1 && d != 2) throw Error(ERR_INVALID, "d must be 1 or 2");
+        if (n == 0 || n >= (1ull << 31)) throw Error(ERR_INVALID, "n must be in [1, 2^31)");
+        const int nb = exact_repulsion_blocks((int64_t)n);
+        DevBuf dy(ctx), dn(ctx), dz(ctx);
+        dy.ensure(ctx, (size_t)n * d * 4);
+        dn.ensure(ctx, (size_t)n * d * 8);
+        dz.ensure(ctx, (size_t)nb * 8);
+        BHT_HIP(hipMemcpy(dy.p, y, (size_t)n * d * 4, hipMemcpyHostToDevice));
+        exact_repulsion_f64(dy.as<float>(), (int64_t)n, (int)d, dn.as<double>(), dz.as<double>(),
+                            ctx->stream);
+        BHT_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<double> zp(nb);
+        BHT_HIP(hipMemcpy(zp.data(), dz.p, (size_t)nb * 8, hipMemcpyDeviceToHost));
+        BHT_HIP(hipMemcpy(neg_out, dn.p, (size_t)n * d * 8, hipMemcpyDeviceToHost));
+        double z = 0.0;
+        for (double v : zp) z += v;
+        *z_out = z;
+    });
+}
+
 bht_status bht_knn_f32(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,
                        uint32_t *idx_out, float *dist_out) {
     return bht_knn_f32_ex(ctx, x, n, d_in, k, BHT_KNN_AUTO, idx_out, dist_out, nullptr);

@@ -222,4 +222,9 @@ void sparse_kl(const float *y, int n, int dim, const uint32_t *rows, const uint3
 void sum_values_f64(const float *vals, uint64_t nnz, EpochScalars *scal, double *dpart,
                     hipStream_t s);
 
+// FP64 brute-force repulsion (exact.hip): neg (n * dim doubles) and one partial Z per block.
+int exact_repulsion_blocks(int64_t n);
+void exact_repulsion_f64(const float *y, int64_t n, int dim, double *neg, double *zpart,
+                         hipStream_t s);
+
 }  // namespace bht

@@ -167,6 +167,12 @@ bht_status bht_knn_f32(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, 
 bht_status bht_knn_f32_ex(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_in, uint32_t k,
                           uint32_t mode, uint32_t *idx_out, float *dist_out,
                           uint64_t *exact_rows);
+/* The non-edge terms of the exact gradient in FP64 (src/lib.rs:570-622): for every point i,
+ * neg_i = sum_j q_ij^2 (y_i - y_j) over all other points and Z = sum of q_ij over ordered pairs,
+ * q_ij = 1 / (1 + |y_i - y_j|^2).  O(n^2); the yardstick for the approximate gradients' error.
+ * y: n * d f32 (d = 1 or 2); neg_out: n * d doubles. */
+bht_status bht_exact_repulsion_f64(bht_ctx *ctx, const float *y, uint64_t n, uint32_t d,
+                                   double *neg_out, double *z_out);
 /* Per-row bandwidth search: conditional P (n * k) from neighbour distances (n * k). */
 bht_status bht_search_beta_f32(bht_ctx *ctx, const float *dist, uint64_t n, uint32_t k,
                                float perplexity, uint32_t mode, float *p_out);
