@@ -27,10 +27,21 @@ def exact_attraction_rows(y, aff, sel):
     return out
 
 
-def gradient_error(job, sample: int = 5000, seed: int = 0) -> float:
-    """Relative L2 error, over `sample` rows, of the prepared job's gradient at its current state
-    (attraction - repulsion / Z as the epoch computes it) against the exact FP64 gradient."""
+def exact_gradient_rows(y, aff, sel, ctx=None):
+    """The exact FP64 gradient of the rows in sel at embedding y (n x d, sample order)."""
     from . import exact_repulsion_f64
+    eneg, ez = exact_repulsion_f64(y, ctx)
+    return exact_attraction_rows(y, aff, sel) - eneg[np.asarray(sel)] / ez
+
+
+def rel_l2(g, ge) -> float:
+    g = np.asarray(g, np.float64)
+    return float(np.sqrt(((g - ge) ** 2).sum() / (ge ** 2).sum()))
+
+
+def gradient_sample(job, sample: int = 5000, seed: int = 0):
+    """(the prepared job's gradient on a row sample, the exact FP64 one, the sample, y, the
+    affinities): the job's gradient is attraction - repulsion / Z as its epoch computes it."""
     pos, neg, inv = job.forces()
     y, _, _ = job.get_state()
     n, d = job._n, job._d
@@ -39,8 +50,12 @@ def gradient_error(job, sample: int = 5000, seed: int = 0) -> float:
     aff = job.affinities()
     rng = np.random.default_rng(seed)
     sel = np.sort(rng.choice(n, size=min(sample, n), replace=False))
-    eneg, ez = exact_repulsion_f64(y, job._ctx)
-    epos = exact_attraction_rows(y, aff, sel)
     g = pos[sel].astype(np.float64) - neg[sel].astype(np.float64) * inv
-    ge = epos - eneg[sel] / ez
-    return float(np.sqrt(((g - ge) ** 2).sum() / (ge ** 2).sum()))
+    return g, exact_gradient_rows(y, aff, sel, job._ctx), sel, y, aff
+
+
+def gradient_error(job, sample: int = 5000, seed: int = 0) -> float:
+    """Relative L2 error, over `sample` rows, of the prepared job's gradient at its current state
+    against the exact FP64 gradient."""
+    g, ge, _, _, _ = gradient_sample(job, sample, seed)
+    return rel_l2(g, ge)
