@@ -314,6 +314,13 @@ class TSNE:
         check(self._L.bht_job_get_timing(self._h, out))
         return float(out[0]), float(out[1]), int(out[2])
 
+    def run_state(self):
+        """(next epoch index of the prepared run, whether its P is still exaggerated)."""
+        e = C.c_uint64(0)
+        x = C.c_int(0)
+        check(self._L.bht_job_run_state(self._h, C.byref(e), C.byref(x)))
+        return int(e.value), bool(x.value)
+
     def grid_info(self):
         b, L_ = C.c_uint32(), C.c_uint32()
         lo, hi = C.c_float(), C.c_float()

@@ -762,6 +762,15 @@ void finish_run(bht_job *j) {
     j->prepared = false;
 }
 
+// Whether the P values on the device still carry the early exaggeration.  During a run the epoch
+// after stop_lying_epoch applies the rescale as it reads P, so until that epoch has run the device
+// values are exaggerated although the reference's schedule has already rescaled them.
+bool device_p_exaggerated(const bht_job *j) {
+    const uint64_t sl = j->params.stop_lying_epoch;
+    if (j->prepared) return sl != 0 && j->epoch <= sl + 1;
+    return sl != 0 && !j->stop_lying_fired;
+}
+
 bool has_cached_affinities(const bht_job *j) {  // lib.rs:1192-1196
     return j->have_p && j->have_cached_perplexity &&
            j->cached_perplexity == j->params.perplexity;
@@ -1311,7 +1320,11 @@ bht_status bht_job_get_affinities(const bht_job *job, uint64_t *rows, uint32_t *
         if (cols && job->nnz) std::memcpy(cols, c.data(), job->nnz * 4);
         if (vals) {
             if (job->nnz) std::memcpy(vals, v.data(), job->nnz * 4);
-            if (!job->stop_lying_fired) {  // lib.rs:1023-1029
+            // lib.rs:1023-1029; between epoch stop_lying_epoch and the next one the reference has
+            // rescaled P while the device values are rescaled by that next epoch
+            const uint64_t sl = job->params.stop_lying_epoch;
+            const bool pending = job->prepared && sl != 0 && job->epoch == sl + 1;
+            if (!job->stop_lying_fired || pending) {
                 const float scale = 1.0f / job->params.early_exaggeration;
                 for (uint64_t i = 0; i < job->nnz; i++) vals[i] = vals[i] * scale;
             }
@@ -1418,6 +1431,13 @@ bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, cons
         BHT_HIP(hipMemsetAsync(&sc->mean_pending, 0, sizeof(int), job->stream));
         BHT_HIP(hipStreamSynchronize(job->stream));
     });
+}
+
+bht_status bht_job_run_state(const bht_job *job, uint64_t *next_epoch, int *exaggerated) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    if (next_epoch) *next_epoch = job->epoch;
+    if (exaggerated) *exaggerated = device_p_exaggerated(job) ? 1 : 0;
+    return BHT_OK;
 }
 
 bht_status bht_job_grid_info(const bht_job *job, uint32_t *boxes, uint32_t *fft_len, float *lo,

@@ -183,6 +183,21 @@ static_assert(KCAND + MC <= KBUF, "candidate buffer");
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// Element (row, d) of the samples, 0 outside [0, n) x [0, D).  The load address is clamped into
+// the array so that the load is unconditional (no branch per operand).
+__device__ __forceinline__ float operand(const float *x, int n, int D, int row, int d) {
+    const bool ok = row < n && d < D;
+    const float v = x[(size_t)(row < n ? row : n - 1) * D + (d < D ? d : D - 1)];
+    return ok ? v : 0.0f;
+}
+
+// Block barrier ordering LDS accesses only: global loads issued before it stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __global__ void __launch_bounds__(256) k_sqnorm(const float *x, int n, int D, float *nrm,
                                                 uint32_t *max_bits) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -218,10 +233,7 @@ __global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *x, const 
     for (int rt = 0; rt < 2; rt++) {
         const int row = q0 + rt * 16 + (lane & 15);
 #pragma unroll
-        for (int kb = 0; kb < MKS; kb++) {
-            const int d = kb * 4 + (lane >> 4);
-            a[rt][kb] = (kb < ksteps && row < n && d < D) ? x[(size_t)row * D + d] : 0.0f;
-        }
+        for (int kb = 0; kb < MKS; kb++) a[rt][kb] = operand(x, n, D, row, kb * 4 + (lane >> 4));
     }
     // output element r of a lane is row 4 * (lane / 16) + r of the tile, column lane % 16
     float qn[2][4];
@@ -233,16 +245,24 @@ __global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *x, const 
             qn[rt][r] = row < n ? nrm[row] : 0.0f;
         }
     __syncthreads();
+    // B operands (4 x 16 per step): lane holds dimension 4 * step + lane / 16 of candidate column
+    // lane % 16.  The next tile's operands and norms are loaded while this tile is reduced; the
+    // block barriers order LDS only, so those loads stay in flight across them.
+    float b[MKS];
+    float cn = 0.0f;
+    {
+        const int col = w * 16 + (lane & 15);
+#pragma unroll
+        for (int kb = 0; kb < MKS; kb++) b[kb] = operand(x, n, D, col, kb * 4 + (lane >> 4));
+        cn = col < n ? nrm[col] : 0.0f;
+    }
     for (int c0 = 0; c0 < n; c0 += MC) {
         const int col = c0 + w * 16 + (lane & 15);
-        // B operands (4 x 16 per step): lane holds dimension 4 * step + lane / 16 of candidate
-        // column lane % 16
-        float b[MKS];
+        const int ncol = col + MC;
+        float bn[MKS];
 #pragma unroll
-        for (int kb = 0; kb < MKS; kb++) {
-            const int d = kb * 4 + (lane >> 4);
-            b[kb] = (kb < ksteps && col < n && d < D) ? x[(size_t)col * D + d] : 0.0f;
-        }
+        for (int kb = 0; kb < MKS; kb++) bn[kb] = operand(x, n, D, ncol, kb * 4 + (lane >> 4));
+        const float cnn = nrm[ncol < n ? ncol : n - 1];
         v4f acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int kb = 0; kb < MKS; kb++) {
@@ -251,7 +271,6 @@ __global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *x, const 
                 acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][kb], b[kb], acc1, 0, 0, 0);
             }
         }
-        const float cn = col < n ? nrm[col] : 0.0f;
 #pragma unroll
         for (int rt = 0; rt < 2; rt++)
 #pragma unroll
@@ -266,7 +285,7 @@ __global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *x, const 
                     buf[qi][pos] = key;
                 }
             }
-        __syncthreads();
+        lds_barrier();
         for (int qi = w; qi < MQ; qi += KNN_THREADS / 64) {
             const uint32_t c = cnt[qi];
             if (c <= (uint32_t)(KBUF - MC)) continue;
@@ -281,7 +300,10 @@ __global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *x, const 
             }
             __builtin_amdgcn_wave_barrier();
         }
-        __syncthreads();
+        lds_barrier();
+#pragma unroll
+        for (int kb = 0; kb < MKS; kb++) b[kb] = bn[kb];
+        cn = cnn;
     }
     for (int qi = w; qi < MQ; qi += KNN_THREADS / 64) {
         const int gq = q0 + qi;

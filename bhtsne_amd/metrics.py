@@ -48,6 +48,8 @@ def gradient_sample(job, sample: int = 5000, seed: int = 0):
     y = y.reshape(n, d)
     pos, neg = pos.reshape(n, d), neg.reshape(n, d)
     aff = job.affinities()
+    if job.run_state()[1]:  # the epoch's P: exported values times the exaggeration
+        aff.values = aff.values * np.float32(job.params()["early_exaggeration"])
     rng = np.random.default_rng(seed)
     sel = np.sort(rng.choice(n, size=min(sample, n), replace=False))
     g = pos[sel].astype(np.float64) - neg[sel].astype(np.float64) * inv

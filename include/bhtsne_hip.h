@@ -143,6 +143,9 @@ bht_status bht_job_synchronize(bht_job *job);
 bht_status bht_job_forces(bht_job *job, float *attractive, float *repulsive, float *inverse_z);
 bht_status bht_job_get_state(const bht_job *job, float *y, float *uy, float *gains);
 bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, const float *gains);
+/* Position of the prepared run: the next epoch index, and whether P is still exaggerated (the
+ * epoch's P is then the exported affinities times early_exaggeration). */
+bht_status bht_job_run_state(const bht_job *job, uint64_t *next_epoch, int *exaggerated);
 /* FIt-SNE grid of the last repulsion: boxes per axis, FFT length, joint min and max. */
 bht_status bht_job_grid_info(const bht_job *job, uint32_t *boxes, uint32_t *fft_len, float *lo,
                              float *hi);
