@@ -297,8 +297,9 @@ def main():
     # the device, edge terms in FP64 on a 5,000-row sample)
     from bhtsne_amd import metrics
     t0 = time.perf_counter()
-    grad_err = metrics.gradient_error(t, sample=5000, seed=0)
-    log(f"gradient rel L2 error vs FP64 exact (5k rows, final state): {grad_err:.3e} "
+    errs = metrics.error_report(t, sample=5000, seed=0)
+    log(f"vs FP64 exact (5k rows, final state): gradient rel L2 {errs['gradient']:.3e}, "
+        f"repulsion F-/Z rel L2 {errs['repulsion']:.3e}, Z rel {errs['z']:.2e} "
         f"({time.perf_counter() - t0:.1f}s)")
     t.finish()
     eps = args.steps / elapsed
@@ -340,7 +341,11 @@ def main():
                       "perplexity": 30, "epochs": args.steps, "min_boxes": wl["min_boxes"],
                       "nnz": nnz, "affinity_build_s": round(t_aff, 3),
                       "final_grid": grid, "parallelism": "single GPU",
-                      "gradient_rel_error_vs_fp64_exact": float(f"{grad_err:.4e}")},
+                      "error_vs_fp64_exact": {
+                          "state": "final (epoch 750), 5000 sampled rows",
+                          "gradient_rel_l2": float(f"{errs['gradient']:.4e}"),
+                          "repulsion_rel_l2": float(f"{errs['repulsion']:.4e}"),
+                          "z_rel": float(f"{errs['z']:.4e}")}},
            "roofline": roofline, "cpu_baseline": cpu,
            "batched_jobs": batched, "serving": serving_line}
     print(json.dumps(out), flush=True)

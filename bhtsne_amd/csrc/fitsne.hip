@@ -34,13 +34,15 @@ void job_block_counts(JobDev &J, bool batched) {
     J.nblk[F_ATT] = attract_blocks(J.ngroups);
     J.nblk[F_TRAV] = traverse_blocks(n);
     J.nblk[F_LINE1] = 2;
-    const int cap = batched ? 128 : 2048;
+    // transform lines per job: 2 * nodes (rows) and 2 * L (columns), a few hundred for batch-sized
+    // jobs; a batch gives each job up to 512 blocks so that most lines get a block of their own
+    const int cap = batched ? 512 : 2048;
     const int64_t nodes_total =
         J.dim == 2 ? (int64_t)J.nodes_cap * J.nodes_cap : (int64_t)J.nodes_cap;
     J.nblk[F_ZERO] = (int)std::max<int64_t>(1, std::min<int64_t>(batched ? 64 : 1024,
                                                                  (nodes_total + 255) / 256));
     const int hcap = J.len_cap / 2 + 1;
-    J.nblk[F_KSPEC] = std::max(1, std::min(batched ? cap : 1024, (hcap + 1) / 2));
+    J.nblk[F_KSPEC] = std::max(1, std::min(batched ? cap / 2 : 1024, (hcap + 1) / 2));
     J.nblk[F_ROWS] = std::max(1, std::min(cap, 2 * J.nodes_cap));
     J.nblk[F_COLS] = std::max(1, std::min(cap, 2 * J.len_cap));
 }

@@ -61,3 +61,27 @@ def gradient_error(job, sample: int = 5000, seed: int = 0) -> float:
     against the exact FP64 gradient."""
     g, ge, _, _, _ = gradient_sample(job, sample, seed)
     return rel_l2(g, ge)
+
+
+def error_report(job, sample: int = 5000, seed: int = 0) -> dict:
+    """The gradient's error against the exact FP64 one, and the error of its repulsive term
+    (F- / Z) alone.  Near equilibrium attraction and repulsion nearly cancel, so the gradient's
+    relative error grows while the approximation's own accuracy is the repulsion's."""
+    from . import exact_repulsion_f64
+    pos, neg, inv = job.forces()
+    y, _, _ = job.get_state()
+    n, d = job._n, job._d
+    y = y.reshape(n, d)
+    pos, neg = pos.reshape(n, d), neg.reshape(n, d)
+    aff = job.affinities()
+    if job.run_state()[1]:
+        aff.values = aff.values * np.float32(job.params()["early_exaggeration"])
+    rng = np.random.default_rng(seed)
+    sel = np.sort(rng.choice(n, size=min(sample, n), replace=False))
+    eneg, ez = exact_repulsion_f64(y, job._ctx)
+    rep = neg[sel].astype(np.float64) * inv
+    erep = eneg[sel] / ez
+    g = pos[sel].astype(np.float64) - rep
+    ge = exact_attraction_rows(y, aff, sel) - erep
+    return {"gradient": rel_l2(g, ge), "repulsion": rel_l2(rep, erep),
+            "z": abs(1.0 / inv - ez) / ez, "rows": int(len(sel))}
