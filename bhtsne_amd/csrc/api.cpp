@@ -575,10 +575,24 @@ void reorder_points(bht_job *j, const uint32_t *order) {
 }
 
 // Epochs after which the points are renumbered (large jobs only, where y outgrows an L2).
+bool reorder_enabled(const bht_job *j) {
+    return !(j->params.flags & BHT_FLAG_NO_REORDER) && j->n >= 32768;
+}
+
+// The run starts in the affinity graph's diffusion order (groups of mutual neighbours together);
+// once the embedding has formed its clusters, the repulsion's spatial order takes over.
 bool reorder_due(const bht_job *j, uint64_t epoch) {
-    if ((j->params.flags & BHT_FLAG_NO_REORDER) || j->n < 32768) return false;
-    if (epoch == 4 || epoch == 14 || epoch == 39) return true;
-    return epoch % 100 == 99;
+    return reorder_enabled(j) && epoch >= 199 && epoch % 100 == 99;
+}
+
+// Renumbers the points in the diffusion order of the affinity graph (before the first epoch).
+void reorder_by_diffusion(bht_job *j) {
+    DevBuf temp(j->ctx);
+    temp.ensure(j->ctx, diffusion_order_temp_bytes((int)j->n));
+    const uint32_t *order =
+        diffusion_order(j->rows.as<uint32_t>(), j->cols.as<uint32_t>(), j->vals.as<float>(),
+                        (int)j->n, 16, temp.p, j->stream);
+    reorder_points(j, order);  // synchronises the stream before temp is released
 }
 
 // Builds P (CSR) from per-row neighbour indices / distances already on the device.
@@ -700,6 +714,7 @@ void prepare_run(bht_job *j, int method, float theta, bool cached) {
     j->run_momentum = j->params.momentum;
     j->prepared = true;
     sync_jobdev(j);
+    if (reorder_enabled(j)) reorder_by_diffusion(j);
 }
 
 // Repulsive forces and Z of the current embedding; returns the point indices in the spatial

@@ -108,4 +108,87 @@ void csr_permute(const uint32_t *rows, const uint32_t *cols, const float *vals,
     BHT_LAUNCH_CHECK();
 }
 
+// ---- diffusion order ---------------------------------------------------------------------------
+
+__global__ void k_diffuse_init(int n, float *v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        // splitmix-style hash of the index to [0, 1)
+        uint64_t z = (uint64_t)i * 0x9e3779b97f4a7c15ull + 0x632be59bd9b4e019ull;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        v[i] = (float)(z >> 40) * (1.0f / 16777216.0f);
+    }
+}
+
+// v_out[i] = sum_j p_ij v_in[j] / sum_j p_ij, one wave per row
+__global__ void __launch_bounds__(256) k_diffuse_step(const uint32_t *rows, const uint32_t *cols,
+                                                      const float *vals, int n, const float *v_in,
+                                                      float *v_out) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * 256) >> 6;
+    for (int i = wave; i < n; i += nwaves) {
+        float num = 0.0f, den = 0.0f;
+        for (uint32_t e = rows[i] + lane; e < rows[i + 1]; e += 64) {
+            const float p = vals[e];
+            num += p * v_in[cols[e]];
+            den += p;
+        }
+        num = wave_sum_f32(num);
+        den = wave_sum_f32(den);
+        if (lane == 0) v_out[i] = den > 0.0f ? num / den : v_in[i];
+    }
+}
+
+// sort keys: the bits of a non-negative float order like the float
+__global__ void k_float_keys(const float *v, int n, uint32_t *keys, uint32_t *idx) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        keys[i] = __float_as_uint(fmaxf(v[i], 0.0f));
+        idx[i] = (uint32_t)i;
+    }
+}
+
+static void diffusion_layout(int n, size_t *off, size_t *total) {
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    const size_t piece[7] = {nn * 4, nn * 4, nn * 4, nn * 4, nn * 4, nn * 4,
+                             radix_sort_temp_bytes(nn)};
+    size_t at = 0;
+    for (int i = 0; i < 7; i++) {
+        off[i] = at;
+        at += (piece[i] + 255) & ~(size_t)255;
+    }
+    *total = at;
+}
+
+size_t diffusion_order_temp_bytes(int n) {
+    size_t off[7], total;
+    diffusion_layout(n, off, &total);
+    return total;
+}
+
+const uint32_t *diffusion_order(const uint32_t *rows, const uint32_t *cols, const float *vals,
+                                int n, int iters, void *temp, hipStream_t s) {
+    size_t off[7], total;
+    diffusion_layout(n, off, &total);
+    char *p = (char *)temp;
+    float *v0 = (float *)(p + off[0]), *v1 = (float *)(p + off[1]);
+    uint32_t *keys = (uint32_t *)(p + off[2]), *idx = (uint32_t *)(p + off[3]);
+    uint32_t *keys_alt = (uint32_t *)(p + off[4]), *idx_alt = (uint32_t *)(p + off[5]);
+    void *stemp = p + off[6];
+    hipLaunchKernelGGL(k_diffuse_init, dim3(nb_for(n)), dim3(256), 0, s, n, v0);
+    const int nb = (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), 4), 8192);
+    for (int t = 0; t < iters; t++) {
+        hipLaunchKernelGGL(k_diffuse_step, dim3(nb), dim3(256), 0, s, rows, cols, vals, n, v0, v1);
+        float *tmp = v0;
+        v0 = v1;
+        v1 = tmp;
+    }
+    hipLaunchKernelGGL(k_float_keys, dim3(nb_for(n)), dim3(256), 0, s, v0, n, keys, idx);
+    BHT_LAUNCH_CHECK();
+    const bool alt = radix_sort_pairs<uint32_t>(keys, idx, keys_alt, idx_alt, (uint64_t)n, 0, 32,
+                                                stemp, s);
+    return alt ? idx_alt : idx;
+}
+
 }  // namespace bht

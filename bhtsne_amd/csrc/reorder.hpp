@@ -17,4 +17,13 @@ void csr_permute(const uint32_t *rows, const uint32_t *cols, const float *vals,
                  void *scan_temp, uint32_t *rows_new, uint32_t *cols_new, float *vals_new,
                  hipStream_t s);
 
+// A point order from the affinity graph alone: `iters` steps of v <- D^-1 P v from a hashed start
+// vector make v nearly constant within each well-connected group of points (diffusion mixes
+// fast inside a group and slowly across groups), so sorting by v puts each group's points next
+// to each other.  Workspace: diffusion_order_temp_bytes(n).  Returns the order (device, n
+// indices: new point i = current point order[i]) inside the workspace.
+size_t diffusion_order_temp_bytes(int n);
+const uint32_t *diffusion_order(const uint32_t *rows, const uint32_t *cols, const float *vals,
+                                int n, int iters, void *temp, hipStream_t s);
+
 }  // namespace bht
