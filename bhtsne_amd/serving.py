@@ -53,10 +53,8 @@ class Result:
 
 
 def _make_job(req: Request, ctx: Context) -> TSNE:
-    # flags = no point renumbering: the batch never renumbers, and the plain baseline then does
-    # the same arithmetic, so both give the same result bit for bit
     t = TSNE(req.x, d_out=req.d_out, ctx=ctx).perplexity(req.perplexity).epochs(req.epochs)
-    t._set(flags=1, max_boxes=req.max_boxes)
+    t._set(max_boxes=req.max_boxes)
     if req.y0 is not None:
         t.initial_embedding(req.y0)
     return t

@@ -73,7 +73,8 @@ typedef struct bht_params {
     uint32_t flags;                 /* BHT_FLAG_* */
 } bht_params;
 
-/* Large jobs renumber their points spatially a few times per run so that the attraction's
+/* Jobs of 32768 points or more are renumbered once, before the first epoch, in an order of
+ * their affinity graph that keeps well-connected points together, so that the attraction's
  * gathers stay in cache (results are unchanged up to summation order; outputs are always in
  * sample order).  This flag keeps the input order. */
 #define BHT_FLAG_NO_REORDER 1u
@@ -190,8 +191,7 @@ bht_status bht_symmetrize(bht_ctx *ctx, const uint32_t *cols, const float *vals,
  * many independent jobs together: every epoch kernel is one launch for all members (of the same
  * method and d_out), and the epoch is replayed from a captured graph.  Each member computes
  * exactly what bht_job_fit_sne / bht_job_barnes_hut computes for it alone (same kernels, same
- * block partition, same summation order); only the point renumbering of large single jobs is not
- * applied.  A request loop admits jobs with bht_batch_add as others finish, so a batch serves a
+ * block partition, same summation order, same point renumbering).  A request loop admits jobs with bht_batch_add as others finish, so a batch serves a
  * stream of requests. */
 typedef struct bht_batch bht_batch;
 

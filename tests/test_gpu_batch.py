@@ -27,6 +27,9 @@ SPECS = [
     (400, 16, 2, "fitsne", 70, 5),
     (900, 32, 2, "bh", 55, 6),
     (3000, 20, 1, "fitsne", 35, 7),
+    # above 32768 points: renumbered before the first epoch, alone and in the batch alike
+    (36000, 16, 2, "fitsne", 30, 8),
+    (33000, 16, 2, "bh", 25, 9),
 ]
 
 
@@ -79,7 +82,7 @@ def test_batch_equals_alone(bt, alone, graph):
 def test_continuous_admission_equals_alone(bt, alone):
     # jobs join while others run (the serving loop's admission) and leave as they finish
     b = bt.Batch()
-    order = [0, 3, 1, 2, 4, 5, 6]
+    order = [0, 3, 1, 7, 2, 4, 8, 5, 6]
     jobs = {}
     pending = list(order)
     finished = []
@@ -171,7 +174,7 @@ def test_serving_respects_hbm_budget(bt):
 
 def test_footprint_covers_allocation(bt):
     ctx = bt.Context(0)
-    for spec in (SPECS[1], SPECS[3], SPECS[2]):
+    for spec in (SPECS[1], SPECS[3], SPECS[2], SPECS[7]):
         n, d_in, d_out, method, _, _ = spec
         before = ctx.bytes_in_use()
         t = bt.TSNE(_data(n, d_in, 11), d_out=d_out, ctx=ctx).perplexity(30.0).epochs(3)
