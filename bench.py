@@ -315,6 +315,9 @@ def main():
                 "kernel": "k_attract<2,true> (attraction + gains/momentum update)",
                 "avg_ms": round(upd_avg_s * 1e3, 4),
                 "repulsion_avg_ms": round(rep_ms / max(timed, 1), 4)}
+    if roofline["traffic"] is None:
+        roofline["traffic_note"] = ("no counter pass of this kernel on record; see "
+                                    "profiles/k_attract_c3_hbm_traffic.md")
 
     cpu = None
     if not args.no_cpu:

@@ -12,22 +12,38 @@
 // and lane q applies the update of the group's q-th row.  A group of more than MAX_GROUP_ROWS rows
 // (short rows, from caller-supplied affinities) is processed row by row.  The new embedding goes
 // to the other buffer of the pair because other rows still read y this epoch.
+//
+// Each block owns a contiguous run of groups and first copies the positions of an index window
+// centred on its rows into LDS (WIN_FLOATS floats).  The points are numbered so that P's
+// well-connected groups are contiguous, so most gathered columns fall inside the window and are
+// read from LDS; the rest are read from global memory.  Gathers are single 8-byte reads from
+// scattered lines, so it is their number of L2 requests, not their bytes, that the window saves.
 #include "internal.hpp"
 
 namespace bht {
 
-constexpr int UP_THREADS = 256;
+constexpr int UP_THREADS = 512;
 constexpr int GROUP_UNROLL = 8;
 constexpr int GROUP_EDGES = 64 * GROUP_UNROLL;  // edge window of a row group
 constexpr int MAX_GROUP_ROWS = 8;
 constexpr int MAX_ATT_BLOCKS = MAX_ATTRACT_BLOCKS;
+// 80,000 bytes of LDS per block: two blocks (16 waves) per CU.  The LDS of a CU is 160 KiB; a
+// block asking for the whole of it ran at half the speed, as if only one block fitted.
+constexpr int WIN_FLOATS = 20000;
+// A block takes ngroups / 1024 groups, at least 8 (one per wave) and at most 64: large jobs get
+// long runs (few window copies), small ones enough blocks to keep their chains of dependent loads
+// short.
+constexpr int MIN_GROUPS_PER_BLOCK = UP_THREADS / 64, MAX_GROUPS_PER_BLOCK = 64;
 
 uint32_t row_groups_count(uint64_t nnz) {
     return (uint32_t)((nnz + GROUP_EDGES - 1) / GROUP_EDGES);
 }
 
 int attract_blocks(int ngroups) {
-    uint64_t b = ceil_div((uint64_t)(ngroups > 0 ? ngroups : 1), UP_THREADS / 64);
+    const uint64_t ng = (uint64_t)(ngroups > 0 ? ngroups : 1);
+    const uint64_t per = std::min<uint64_t>(
+        std::max<uint64_t>(ng / 1024, MIN_GROUPS_PER_BLOCK), MAX_GROUPS_PER_BLOCK);
+    const uint64_t b = ceil_div(ng, per);
     return (int)(b < (uint64_t)MAX_ATT_BLOCKS ? b : (uint64_t)MAX_ATT_BLOCKS);
 }
 
@@ -92,18 +108,61 @@ __device__ __forceinline__ float2 load_y(const float *y, uint32_t i) {
     return make_float2(y[i], 0.0f);
 }
 
+// Positions of points [lo, lo + count) copied to LDS; lookups outside it go to global memory.
+template <int D>
+struct YWindow {
+    const float *y;
+    const float *win;
+    uint32_t lo, count;
+    __device__ __forceinline__ bool holds(uint32_t i) const { return i - lo < count; }
+    __device__ __forceinline__ float2 lds(uint32_t i) const { return load_y<D>(win, i - lo); }
+    __device__ __forceinline__ float2 operator()(uint32_t i) const {
+        return holds(i) ? lds(i) : load_y<D>(y, i);
+    }
+};
+
+// Window of at most WIN_FLOATS / D points centred on rows [r_begin, r_end), its start a multiple
+// of 4 / D points so that the copy runs in 16-byte loads; the whole block copies it.
+template <int D>
+__device__ YWindow<D> stage_window(const float *y, int n, int r_begin, int r_end, float *win) {
+    constexpr int WPTS = WIN_FLOATS / D;
+    constexpr int ALIGN = 4 / D;
+    int count = WPTS < n ? WPTS : n;
+    int lo = ((r_begin + r_end) >> 1) - count / 2;
+    if (lo > n - count) lo = n - count;
+    if (lo < 0) lo = 0;
+    lo -= lo % ALIGN;
+    if (count > n - lo) count = n - lo;
+    const int nf = count * D;
+    const float *src = y + (size_t)lo * D;
+    for (int k = threadIdx.x; 4 * k < nf; k += blockDim.x) {
+        if (4 * k + 3 < nf) {
+            reinterpret_cast<float4 *>(win)[k] = reinterpret_cast<const float4 *>(src)[k];
+        } else {
+            for (int t = 4 * k; t < nf; t++) win[t] = src[t];
+        }
+    }
+    __syncthreads();
+    return YWindow<D>{y, win, (uint32_t)lo, (uint32_t)count};
+}
+
 template <int D, bool UPDATE>
 __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, const uint32_t *map,
                                                         float *pos_out) {
     __shared__ double red[UP_THREADS / 64];
+    __shared__ __align__(16) float win[WIN_FLOATS];
     const BlockCtx bc = block_ctx(map, jobs, F_ATT);
     const JobDev &J = jobs[bc.job];
     const int lane = threadIdx.x & 63;
-    const int wave = (bc.lb * UP_THREADS + threadIdx.x) >> 6;
-    const int nwaves = (bc.nb * UP_THREADS) >> 6;
+    constexpr int NW = UP_THREADS / 64;
     const int epoch = J.scal->epoch;
-    const float *y = J.ybuf[epoch & 1];
     const uint32_t *rows = J.rows, *cols = J.cols, *groups = J.groups;
+    // this block's groups [g_begin, g_end) and the window around their rows
+    const int g_begin = (int)((int64_t)J.ngroups * bc.lb / bc.nb);
+    const int g_end = (int)((int64_t)J.ngroups * (bc.lb + 1) / bc.nb);
+    const int r_begin = g_begin < g_end ? (int)groups[g_begin] : 0;
+    const int r_end = g_begin < g_end ? (int)groups[g_end] : 0;
+    const YWindow<D> y = stage_window<D>(J.ybuf[epoch & 1], J.n, r_begin, r_end, win);
     float *vals = J.vals;
     const float inv_exag = J.inv_exag;
     StepCtx sc{0.0f, J.momentum, J.ybuf[(epoch + 1) & 1]};
@@ -118,7 +177,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
         sc.momentum = epoch > J.momentum_switch_epoch ? J.final_momentum : J.momentum;
     }
     double m0 = 0.0, m1 = 0.0;
-    for (int g = wave; g < J.ngroups; g += nwaves) {
+    for (int g = g_begin + (int)(threadIdx.x >> 6); g < g_end; g += NW) {
         const int r0 = (int)groups[g], r1 = (int)groups[g + 1];
         const int nr = r1 - r0;
         if (nr <= 0) continue;
@@ -127,7 +186,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
         if (nr > MAX_GROUP_ROWS) {
             // many short rows: one row at a time, lanes over its edges
             for (int i = r0; i < r1; i++) {
-                const float2 yi = load_y<D>(y, i);
+                const float2 yi = y(i);
                 float a0 = 0.0f, a1 = 0.0f;
                 for (uint32_t e = rows[i] + lane; e < rows[i + 1]; e += 64) {
                     float pv = vals[e];
@@ -135,7 +194,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
                         pv = pv * inv_exag;
                         vals[e] = pv;
                     }
-                    const float2 yj = load_y<D>(y, cols[e]);
+                    const float2 yj = y(cols[e]);
                     const float d0 = yi.x - yj.x;
                     const float d1 = D == 2 ? yi.y - yj.y : 0.0f;
                     float dist = 0.0f;
@@ -165,7 +224,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
         for (int q = 0; q <= MAX_GROUP_ROWS; q++) bnd[q] = q <= nr ? rows[r0 + q] : 0xffffffffu;
 #pragma unroll
         for (int q = 0; q < MAX_GROUP_ROWS; q++)
-            yr[q] = q < nr ? load_y<D>(y, r0 + q) : make_float2(0.0f, 0.0f);
+            yr[q] = q < nr ? y(r0 + q) : make_float2(0.0f, 0.0f);
         float acc[MAX_GROUP_ROWS][2];
 #pragma unroll
         for (int q = 0; q < MAX_GROUP_ROWS; q++) acc[q][0] = acc[q][1] = 0.0f;
@@ -184,9 +243,17 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
                     vals[e] = pv[u];
                 }
             }
+            // window reads for all lanes (clamped), then global reads for the columns outside
             float2 yj[GROUP_UNROLL];
+            bool inside[GROUP_UNROLL];
 #pragma unroll
-            for (int u = 0; u < GROUP_UNROLL; u++) yj[u] = load_y<D>(y, col[u]);
+            for (int u = 0; u < GROUP_UNROLL; u++) {
+                inside[u] = y.holds(col[u]);
+                yj[u] = y.lds(inside[u] ? col[u] : y.lo);
+            }
+#pragma unroll
+            for (int u = 0; u < GROUP_UNROLL; u++)
+                if (!inside[u]) yj[u] = load_y<D>(y.y, col[u]);
 #pragma unroll
             for (int u = 0; u < GROUP_UNROLL; u++) {
                 const uint32_t e = base + lane + 64 * u;
@@ -229,7 +296,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
             pos_out[(size_t)i * D] = mine0;
             if (D == 2) pos_out[(size_t)i * D + 1] = mine1;
         } else {
-            const float2 yi = load_y<D>(y, i);
+            const float2 yi = y(i);
             const float yv[2] = {yi.x, yi.y};
             update_row<D>(J, sc, i, accv, yv, m0, m1);
         }
