@@ -400,8 +400,7 @@ void fill_jobdev(bht_job *j) {
 
 // Row groups of the current CSR layout (after any change of rows: build, injection, renumbering).
 void refresh_row_groups(bht_job *j) {
-    const uint32_t ng = row_groups_count(j->nnz);
-    j->groups.ensure(j->ctx, ((size_t)ng + 1) * 4);
+    j->groups.ensure(j->ctx, row_groups_bytes(j->nnz));
     build_row_groups(j->rows.as<uint32_t>(), (int)j->n, j->nnz, j->groups.as<uint32_t>(),
                      j->stream);
 }
@@ -1745,7 +1744,7 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
     const uint64_t ne = n * k, nnz = 2 * ne;
     uint64_t b = n * d_in * 4;                      // samples
     b += (n + 1) * 4 + nnz * 8 + n * 4;             // CSR + KL weights
-    b += ((uint64_t)row_groups_count(nnz) + 1) * 4; // row groups
+    b += row_groups_bytes(nnz);                     // row-group descriptors
     b += 6 * n * D * 4 + 4096 * 8 + 4096 * 16;      // optimiser state + partials
     b += 2 * (uint64_t)MAX_ATTRACT_BLOCKS * 8;      // update partials
     b += ne * 12 * 4 + radix_sort_temp_bytes(ne);   // affinity-build peak (transient)

@@ -25,7 +25,6 @@ namespace bht {
 constexpr int UP_THREADS = 512;
 constexpr int GROUP_UNROLL = 8;
 constexpr int GROUP_EDGES = 64 * GROUP_UNROLL;  // edge window of a row group
-constexpr int MAX_GROUP_ROWS = 8;
 constexpr int MAX_ATT_BLOCKS = MAX_ATTRACT_BLOCKS;
 // 80,000 bytes of LDS per block: two blocks (16 waves) per CU.  The LDS of a CU is 160 KiB; a
 // block asking for the whole of it ran at half the speed, as if only one block fitted.
@@ -47,23 +46,40 @@ int attract_blocks(int ngroups) {
     return (int)(b < (uint64_t)MAX_ATT_BLOCKS ? b : (uint64_t)MAX_ATT_BLOCKS);
 }
 
-// groups[m] = first row whose first edge is at or after m * GROUP_EDGES (rows are nondecreasing);
-// groups[ngroups] = n.  Group m is rows [groups[m], groups[m + 1]).
-__global__ void k_row_groups(const uint32_t *rows, int n, uint32_t ngroups, uint32_t *groups) {
-    const uint32_t m = blockIdx.x * 256 + threadIdx.x;
-    if (m > ngroups) return;
-    if (m == ngroups) {
-        groups[m] = (uint32_t)n;
-        return;
-    }
-    const uint64_t target = (uint64_t)m * GROUP_EDGES;
+size_t row_groups_bytes(uint64_t nnz) {
+    return ((size_t)row_groups_count(nnz) + 1) * GROUP_DESC_WORDS * sizeof(uint32_t);
+}
+
+// first row whose first edge is at or after `target` (rows are nondecreasing), n if none
+__device__ uint32_t first_row_at(const uint32_t *rows, int n, uint64_t target) {
     uint32_t lo = 0, hi = (uint32_t)n;
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo) / 2;
         if ((uint64_t)rows[mid] < target) lo = mid + 1;
         else hi = mid;
     }
-    groups[m] = lo;
+    return lo;
+}
+
+// Group m is rows [first_row_at(m * GROUP_EDGES), first_row_at((m + 1) * GROUP_EDGES)).
+__global__ void k_row_groups(const uint32_t *rows, int n, uint32_t ngroups, uint32_t *groups) {
+    const uint32_t m = blockIdx.x * 256 + threadIdx.x;
+    if (m > ngroups) return;
+    uint32_t *d = groups + (size_t)GROUP_DESC_WORDS * m;
+    if (m == ngroups) {
+        d[0] = (uint32_t)n;
+        for (int q = 1; q < GROUP_DESC_WORDS; q++) d[q] = q == 1 ? 0u : 0xffffffffu;
+        return;
+    }
+    const uint32_t r0 = first_row_at(rows, n, (uint64_t)m * GROUP_EDGES);
+    const uint32_t r1 = m + 1 == ngroups ? (uint32_t)n
+                                         : first_row_at(rows, n, (uint64_t)(m + 1) * GROUP_EDGES);
+    const uint32_t nr = r1 - r0;
+    d[0] = r0;
+    d[1] = nr;
+    for (int q = 0; q <= MAX_GROUP_ROWS; q++)
+        d[2 + q] = (uint32_t)q <= nr && nr <= MAX_GROUP_ROWS ? rows[r0 + q] : 0xffffffffu;
+    d[GROUP_DESC_WORDS - 1] = 0u;
 }
 
 void build_row_groups(const uint32_t *rows, int n, uint64_t nnz, uint32_t *groups,
@@ -75,56 +91,103 @@ void build_row_groups(const uint32_t *rows, int n, uint64_t nnz, uint32_t *group
 }
 
 struct StepCtx {
-    float inv_z, momentum;
-    float *y_next;
+    float inv_z, momentum, learning_rate;
 };
 
-// Gradient step of row i from its attraction sums (mod.rs:470-516); returns the new coordinates.
+// The job's arrays as global-memory pointers (the kernel mixes them with LDS reads).
+struct AttractMem {
+    BHT_GLOBAL const uint32_t *rows, *cols, *groups;
+    BHT_GLOBAL float *vals, *qv, *uy, *gains, *y_next;
+    BHT_GLOBAL const float *frep;
+};
+
+// Per-row operands of the gradient step: the repulsive force, the velocity and the gains.
 template <int D>
-__device__ __forceinline__ void update_row(const JobDev &J, const StepCtx &c, int i,
-                                           const float *acc, const float *yi, double &m0,
-                                           double &m1) {
+struct RowState {
+    float frep[D], uy[D], gain[D];
+};
+
+template <int D>
+__device__ __forceinline__ RowState<D> load_row_state(const AttractMem &M, int i) {
+    RowState<D> r;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         const size_t k = (size_t)i * D + d;
-        const float gradient = acc[d] - J.frep[k] * c.inv_z;
-        float uy = J.uy[k];
-        float gain = J.gains[k];
+        r.frep[d] = M.frep[k];
+        r.uy[d] = M.uy[k];
+        r.gain[d] = M.gains[k];
+    }
+    return r;
+}
+
+// Gradient step of row i from its attraction sums (mod.rs:470-516): stores the new coordinates,
+// velocity and gains, and adds the new coordinates to the recentring sums.
+template <int D>
+__device__ __forceinline__ void apply_update(const AttractMem &M, const StepCtx &c, int i,
+                                             const float *acc, const float *yi,
+                                             const RowState<D> &st, double &m0, double &m1) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const size_t k = (size_t)i * D + d;
+        const float gradient = acc[d] - st.frep[d] * c.inv_z;
+        float uy = st.uy[d];
+        float gain = st.gain[d];
         gain = rust_signum(gradient) != rust_signum(uy) ? gain + 0.2f : gain * 0.8f;
         if (gain < 0.01f) gain = 0.01f;
-        uy = c.momentum * uy - J.learning_rate * gain * gradient;
+        uy = c.momentum * uy - c.learning_rate * gain * gradient;
         const float yn = yi[d] + uy;
-        J.gains[k] = gain;
-        J.uy[k] = uy;
-        c.y_next[k] = yn;
+        M.gains[k] = gain;
+        M.uy[k] = uy;
+        M.y_next[k] = yn;
         if (d == 0) m0 += (double)yn;
         else m1 += (double)yn;
     }
 }
 
 template <int D>
-__device__ __forceinline__ float2 load_y(const float *y, uint32_t i) {
-    if (D == 2) return reinterpret_cast<const float2 *>(y)[i];
-    return make_float2(y[i], 0.0f);
+__device__ __forceinline__ void update_row(const AttractMem &M, const StepCtx &c, int i,
+                                           const float *acc, const float *yi, double &m0,
+                                           double &m1) {
+    apply_update<D>(M, c, i, acc, yi, load_row_state<D>(M, i), m0, m1);
 }
+
+// clang vector types: loads through address-space-qualified pointers (HIP's float2 / float4
+// cannot be copy-constructed from such references)
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 
 // Positions of points [lo, lo + count) copied to LDS; lookups outside it go to global memory.
 template <int D>
 struct YWindow {
-    const float *y;
-    const float *win;
+    BHT_GLOBAL const float *y;
+    BHT_LDS const float *win;
     uint32_t lo, count;
     __device__ __forceinline__ bool holds(uint32_t i) const { return i - lo < count; }
-    __device__ __forceinline__ float2 lds(uint32_t i) const { return load_y<D>(win, i - lo); }
+    __device__ __forceinline__ float2 lds(uint32_t i) const {
+        if (D == 2) {
+            const f2v v = ((BHT_LDS const f2v *)win)[i - lo];
+            return make_float2(v.x, v.y);
+        }
+        return make_float2(win[i - lo], 0.0f);
+    }
+    __device__ __forceinline__ float2 global(uint32_t i) const {
+        if (D == 2) {
+            const f2v v = ((BHT_GLOBAL const f2v *)y)[i];
+            return make_float2(v.x, v.y);
+        }
+        return make_float2(y[i], 0.0f);
+    }
     __device__ __forceinline__ float2 operator()(uint32_t i) const {
-        return holds(i) ? lds(i) : load_y<D>(y, i);
+        if (holds(i)) return lds(i);
+        return global(i);
     }
 };
 
 // Window of at most WIN_FLOATS / D points centred on rows [r_begin, r_end), its start a multiple
 // of 4 / D points so that the copy runs in 16-byte loads; the whole block copies it.
 template <int D>
-__device__ YWindow<D> stage_window(const float *y, int n, int r_begin, int r_end, float *win) {
+__device__ YWindow<D> stage_window(BHT_GLOBAL const float *y, int n, int r_begin, int r_end,
+                                   BHT_LDS float *win) {
     constexpr int WPTS = WIN_FLOATS / D;
     constexpr int ALIGN = 4 / D;
     int count = WPTS < n ? WPTS : n;
@@ -134,10 +197,10 @@ __device__ YWindow<D> stage_window(const float *y, int n, int r_begin, int r_end
     lo -= lo % ALIGN;
     if (count > n - lo) count = n - lo;
     const int nf = count * D;
-    const float *src = y + (size_t)lo * D;
+    BHT_GLOBAL const float *src = y + (size_t)lo * D;
     for (int k = threadIdx.x; 4 * k < nf; k += blockDim.x) {
         if (4 * k + 3 < nf) {
-            reinterpret_cast<float4 *>(win)[k] = reinterpret_cast<const float4 *>(src)[k];
+            ((BHT_LDS f4v *)win)[k] = ((BHT_GLOBAL const f4v *)src)[k];
         } else {
             for (int t = 4 * k; t < nf; t++) win[t] = src[t];
         }
@@ -156,16 +219,27 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
     const int lane = threadIdx.x & 63;
     constexpr int NW = UP_THREADS / 64;
     const int epoch = J.scal->epoch;
-    const uint32_t *rows = J.rows, *cols = J.cols, *groups = J.groups;
+    AttractMem M;
+    M.rows = global_view(J.rows);
+    M.cols = global_view(J.cols);
+    M.groups = global_view(J.groups);
+    M.vals = global_view(J.vals);
+    M.qv = global_view(J.qv);
+    M.uy = global_view(J.uy);
+    M.gains = global_view(J.gains);
+    M.y_next = global_view(J.ybuf[(epoch + 1) & 1]);
+    M.frep = global_view((const float *)J.frep);
+    BHT_GLOBAL const uint32_t *rows = M.rows, *cols = M.cols, *groups = M.groups;
+    BHT_GLOBAL float *vals = M.vals;
     // this block's groups [g_begin, g_end) and the window around their rows
     const int g_begin = (int)((int64_t)J.ngroups * bc.lb / bc.nb);
     const int g_end = (int)((int64_t)J.ngroups * (bc.lb + 1) / bc.nb);
-    const int r_begin = g_begin < g_end ? (int)groups[g_begin] : 0;
-    const int r_end = g_begin < g_end ? (int)groups[g_end] : 0;
-    const YWindow<D> y = stage_window<D>(J.ybuf[epoch & 1], J.n, r_begin, r_end, win);
-    float *vals = J.vals;
+    const int r_begin = g_begin < g_end ? (int)groups[(size_t)GROUP_DESC_WORDS * g_begin] : 0;
+    const int r_end = g_begin < g_end ? (int)groups[(size_t)GROUP_DESC_WORDS * g_end] : 0;
+    const YWindow<D> y = stage_window<D>(global_view((const float *)J.ybuf[epoch & 1]), J.n,
+                                         r_begin, r_end, lds_view(win));
     const float inv_exag = J.inv_exag;
-    StepCtx sc{0.0f, J.momentum, J.ybuf[(epoch + 1) & 1]};
+    StepCtx sc{0.0f, J.momentum, J.learning_rate};
     bool stop_now = false;
     if (UPDATE) {
         sc.inv_z = J.scal->inv_z;
@@ -177,12 +251,17 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
         sc.momentum = epoch > J.momentum_switch_epoch ? J.final_momentum : J.momentum;
     }
     double m0 = 0.0, m1 = 0.0;
-    for (int g = g_begin + (int)(threadIdx.x >> 6); g < g_end; g += NW) {
-        const int r0 = (int)groups[g], r1 = (int)groups[g + 1];
-        const int nr = r1 - r0;
+    // the wave index as a uniform value, so that the group loop and its descriptor loads are scalar
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int g = g_begin + wid; g < g_end; g += NW) {
+        BHT_GLOBAL const uint32_t *desc = groups + (size_t)GROUP_DESC_WORDS * g;
+        uint32_t dw[2 + MAX_GROUP_ROWS + 1];
+#pragma unroll
+        for (int q = 0; q < 2 + MAX_GROUP_ROWS + 1; q++) dw[q] = desc[q];
+        const int r0 = (int)dw[0], nr = (int)dw[1], r1 = r0 + nr;
         if (nr <= 0) continue;
         if (UPDATE && stop_now)
-            for (int i = r0 + lane; i < r1 && i < J.qv_n; i += 64) J.qv[i] = J.qv[i] * inv_exag;
+            for (int i = r0 + lane; i < r1 && i < J.qv_n; i += 64) M.qv[i] = M.qv[i] * inv_exag;
         if (nr > MAX_GROUP_ROWS) {
             // many short rows: one row at a time, lanes over its edges
             for (int i = r0; i < r1; i++) {
@@ -212,16 +291,22 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
                     pos_out[(size_t)i * D] = a0;
                     if (D == 2) pos_out[(size_t)i * D + 1] = a1;
                 } else {
-                    update_row<D>(J, sc, i, acc, yv, m0, m1);
+                    update_row<D>(M, sc, i, acc, yv, m0, m1);
                 }
             }
             continue;
         }
+        // the operands of lane q's update (row r0 + q), loaded now so that their latency overlaps
+        // the edge loop
+        RowState<D> own{};
+        float2 yown = make_float2(0.0f, 0.0f);
+        if (UPDATE && lane < nr) {
+            own = load_row_state<D>(M, r0 + lane);
+            yown = y(r0 + lane);
+        }
         // the group's row offsets and positions (uniform across the wave)
-        uint32_t bnd[MAX_GROUP_ROWS + 1];
+        const uint32_t *bnd = dw + 2;
         float2 yr[MAX_GROUP_ROWS];
-#pragma unroll
-        for (int q = 0; q <= MAX_GROUP_ROWS; q++) bnd[q] = q <= nr ? rows[r0 + q] : 0xffffffffu;
 #pragma unroll
         for (int q = 0; q < MAX_GROUP_ROWS; q++)
             yr[q] = q < nr ? y(r0 + q) : make_float2(0.0f, 0.0f);
@@ -253,7 +338,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
             }
 #pragma unroll
             for (int u = 0; u < GROUP_UNROLL; u++)
-                if (!inside[u]) yj[u] = load_y<D>(y.y, col[u]);
+                if (!inside[u]) yj[u] = y.global(col[u]);
 #pragma unroll
             for (int u = 0; u < GROUP_UNROLL; u++) {
                 const uint32_t e = base + lane + 64 * u;
@@ -296,9 +381,8 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
             pos_out[(size_t)i * D] = mine0;
             if (D == 2) pos_out[(size_t)i * D + 1] = mine1;
         } else {
-            const float2 yi = y(i);
-            const float yv[2] = {yi.x, yi.y};
-            update_row<D>(J, sc, i, accv, yv, m0, m1);
+            const float yv[2] = {yown.x, yown.y};
+            apply_update<D>(M, sc, i, accv, yv, own, m0, m1);
         }
     }
     if (!UPDATE) return;

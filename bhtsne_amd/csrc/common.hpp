@@ -40,6 +40,21 @@ inline unsigned ceil_div(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1)
 
 // ---- device helpers ------------------------------------------------------------------------
 
+// Pointers read from a device table are generic, and the compiler turns accesses through them
+// into flat instructions. A flat load counts on the LDS counter as well, so every wait for an LDS
+// read would also wait for all outstanding flat loads. Kernels that mix LDS reads with global
+// loads access global memory through these address-space-qualified views instead.
+#define BHT_GLOBAL __attribute__((address_space(1)))
+#define BHT_LDS __attribute__((address_space(3)))
+template <class T>
+__device__ __forceinline__ BHT_GLOBAL T *global_view(T *p) {
+    return (BHT_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ BHT_LDS T *lds_view(T *p) {
+    return (BHT_LDS T *)p;
+}
+
 __device__ __forceinline__ float wave_sum_f32(float v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -91,7 +91,7 @@ struct JobDev {
     float *uy, *gains, *frep;
     const uint32_t *rows, *cols;
     float *vals;
-    const uint32_t *groups;  // row groups of the attraction kernel (ngroups + 1 row indices)
+    const uint32_t *groups;  // row-group descriptors of the attraction kernel (ngroups + 1)
     int ngroups;
     float *qv;          // values at the reference's CSR positions [0, qv_n) (KL weights)
     int qv_n;
@@ -138,6 +138,12 @@ int attract_blocks(int ngroups);
 constexpr int MAX_ATTRACT_BLOCKS = 16384;  // partial sums of the update: 2 doubles per block
 // Row groups of the attraction kernel for a CSR of nnz entries (attract.hip).
 uint32_t row_groups_count(uint64_t nnz);
+// Row group m is described by GROUP_DESC_WORDS words: its first row, its row count, the CSR
+// offsets of its first MAX_GROUP_ROWS + 1 rows (all-ones past the count), and a pad word.
+// Descriptor ngroups is a sentinel whose first row is n.
+constexpr int MAX_GROUP_ROWS = 8;
+constexpr int GROUP_DESC_WORDS = 12;
+size_t row_groups_bytes(uint64_t nnz);
 void build_row_groups(const uint32_t *rows, int n, uint64_t nnz, uint32_t *groups, hipStream_t s);
 int spread_blocks(int n);
 int traverse_blocks(int n);
