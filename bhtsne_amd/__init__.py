@@ -308,8 +308,8 @@ class TSNE:
         return self
 
     def timing(self):
-        """(repulsion ms, attraction+update ms, epochs) measured with HIP events on the job's
-        stream while timing was enabled."""
+        """(repulsion ms, attraction ms, epochs) measured with HIP events while timing was
+        enabled; the two phases run concurrently after each epoch's recentring."""
         out = (C.c_double * 3)()
         check(self._L.bht_job_get_timing(self._h, out))
         return float(out[0]), float(out[1]), int(out[2])

@@ -201,7 +201,7 @@ struct bht_job {
 
     // optimiser state; y[cur] is the embedding of the next epoch (cur = epoch parity, which the
     // kernels read from the device epoch counter)
-    DevBuf y[2], uy, gains, frep, pos;
+    DevBuf y[2], uy, gains, frep, fattr, pos;
     int cur = 0;
     bool have_init = false;
     std::vector<float> init;
@@ -235,7 +235,13 @@ struct bht_job {
     int fit_kind = FIT_NONE;
     float fit_theta = 0.5f;
 
-    // per-phase timing with events on this job's stream (3 events per timed epoch)
+    // An epoch forks after the recentring: the repulsion runs on `side` while the attraction runs
+    // on `stream`, and the update on `stream` waits for both (fork / join events).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
+    // per-phase timing with events (4 per timed epoch: fork, repulsion done, attraction done,
+    // update done)
     bool timing = false;
     std::vector<hipEvent_t> events;
     size_t events_used = 0;
@@ -243,19 +249,25 @@ struct bht_job {
     explicit bht_job(bht_ctx *c, uint64_t n_, const bht_params &p) : ctx(c), n(n_), params(p) {
         BHT_HIP(hipSetDevice(ctx->device));
         BHT_HIP(hipStreamCreate(&stream));
+        BHT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        BHT_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        BHT_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     }
     ~bht_job() {
         for (auto e : events) hipEventDestroy(e);
+        if (ev_fork) hipEventDestroy(ev_fork);
+        if (ev_join) hipEventDestroy(ev_join);
+        if (side) hipStreamDestroy(side);
         if (stream) hipStreamDestroy(stream);
     }
-    void mark() {
+    void mark(hipStream_t s) {
         if (!timing) return;
         if (events_used == events.size()) {
             hipEvent_t e;
             BHT_HIP(hipEventCreate(&e));
             events.push_back(e);
         }
-        BHT_HIP(hipEventRecord(events[events_used++], stream));
+        BHT_HIP(hipEventRecord(events[events_used++], s));
     }
     uint32_t dim() const { return params.d_out; }
     float *ycur() const { return y[cur].as<float>(); }
@@ -281,10 +293,11 @@ void ensure_state_buffers(bht_job *j) {
     j->uy.ensure(j->ctx, b);
     j->gains.ensure(j->ctx, b);
     j->frep.ensure(j->ctx, b);
+    j->fattr.ensure(j->ctx, b);
     j->pos.ensure(j->ctx, b);
     j->scal.ensure(j->ctx, sizeof(EpochScalars));
     j->dpart.ensure(j->ctx, 4096 * sizeof(double));
-    j->mpart.ensure(j->ctx, 2 * (size_t)MAX_ATTRACT_BLOCKS * sizeof(double));
+    j->mpart.ensure(j->ctx, 2 * (size_t)MAX_UPDATE_PARTS * sizeof(double));
     j->fpart.ensure(j->ctx, 4 * 1024 * sizeof(float));
 }
 
@@ -363,6 +376,7 @@ void fill_jobdev(bht_job *j) {
     d.uy = j->uy.as<float>();
     d.gains = j->gains.as<float>();
     d.frep = j->frep.as<float>();
+    d.fattr = j->fattr.as<float>();
     d.rows = j->rows.as<uint32_t>();
     d.cols = j->cols.as<uint32_t>();
     d.vals = j->vals.as<float>();
@@ -579,6 +593,7 @@ void reorder_points(bht_job *j, const uint32_t *order) {
 
 // Large jobs (y outgrows an XCD's L2) are renumbered once, before the first epoch.
 constexpr uint64_t REORDER_MIN_POINTS = 32768;
+constexpr uint64_t FORK_MIN_POINTS = 32768;
 
 bool reorder_enabled(const bht_job *j) {
     return !(j->params.flags & BHT_FLAG_NO_REORDER) && j->n >= REORDER_MIN_POINTS;
@@ -744,12 +759,31 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     if (j->in_batch) throw Error(ERR_STATE, "the job is advanced by its batch");
     const uint64_t len = j->n * j->dim();
     std::vector<float> snap(cb ? len : 0);
+    const LaunchSet L = single_set(j);
+    const bool fit = j->method == FIT_FITSNE;
+    // Small jobs are bound by launch latency, which the fork's events would only add to; they
+    // run the same kernels in order on one stream.
+    const bool fork = j->n >= FORK_MIN_POINTS;
+    hipStream_t rs = fork ? j->side : j->stream;
     for (uint64_t e = 0; e < count; e++) {
-        j->mark();
-        repulsion(j);
-        j->mark();
-        attract_update(single_set(j), j->stream);
-        j->mark();
+        // recentre (+ FIt-SNE grid plan), then the repulsion and the attraction side by side
+        center_plan(L, fit, j->stream);
+        if (fork) {
+            BHT_HIP(hipEventRecord(j->ev_fork, j->stream));
+            BHT_HIP(hipStreamWaitEvent(j->side, j->ev_fork, 0));
+        }
+        j->mark(j->stream);
+        if (fit) fitsne_repulsion(L, rs);
+        else if (j->n > 0) bh_repulsion(L, rs);
+        j->mark(rs);
+        attract_forces(L, j->stream);
+        j->mark(j->stream);
+        if (fork) {
+            BHT_HIP(hipEventRecord(j->ev_join, j->side));
+            BHT_HIP(hipStreamWaitEvent(j->stream, j->ev_join, 0));
+        }
+        update_points(L, j->stream);
+        j->mark(j->stream);
         j->cur = 1 - j->cur;
         const uint64_t epoch = j->epoch++;
         epoch_tail(j, epoch);
@@ -885,6 +919,10 @@ struct BatchGroup {
 struct bht_batch {
     bht_ctx *ctx;
     hipStream_t stream = nullptr;
+    // each group's repulsion runs on `side` beside its attraction (fork / join events; under
+    // capture they become parallel branches of the graph)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<bht_job *> members;
     std::vector<bht_job *> finished;
     std::vector<std::unique_ptr<BatchGroup>> groups;
@@ -900,9 +938,15 @@ struct bht_batch {
     explicit bht_batch(bht_ctx *c) : ctx(c) {
         BHT_HIP(hipSetDevice(ctx->device));
         BHT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        BHT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        BHT_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        BHT_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     }
     ~bht_batch() {
         drop_graph();
+        if (ev_fork) hipEventDestroy(ev_fork);
+        if (ev_join) hipEventDestroy(ev_join);
+        if (side) hipStreamDestroy(side);
         if (stream) hipStreamDestroy(stream);
     }
     void drop_graph() {
@@ -1037,19 +1081,22 @@ void batch_rebuild(bht_batch *b) {
     b->dirty = false;
 }
 
-// One epoch of every member: each group's repulsion, then its attraction + update.
+// One epoch of every member: per group the recentring, then the repulsion (on the side stream)
+// beside the attraction, then the update.
 void batch_epoch_launches(bht_batch *b) {
     for (auto &gp : b->groups) {
         const LaunchSet &L = gp->L;
         if (L.njobs == 0) continue;
-        if (gp->method == M_FITSNE) {
-            center_plan(L, true, b->stream);
-            fitsne_repulsion(L, b->stream);
-        } else {
-            center_plan(L, false, b->stream);
-            bh_repulsion(L, b->stream);
-        }
-        attract_update(L, b->stream);
+        const bool fit = gp->method == M_FITSNE;
+        center_plan(L, fit, b->stream);
+        BHT_HIP(hipEventRecord(b->ev_fork, b->stream));
+        BHT_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+        if (fit) fitsne_repulsion(L, b->side);
+        else bh_repulsion(L, b->side);
+        attract_forces(L, b->stream);
+        BHT_HIP(hipEventRecord(b->ev_join, b->side));
+        BHT_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+        update_points(L, b->stream);
     }
 }
 
@@ -1074,6 +1121,13 @@ void batch_launch_epoch(bht_batch *b, bool may_capture) {
         const hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (!recorded || e != hipSuccess || g == nullptr) {
             if (g) hipGraphDestroy(g);
+            // a capture abandoned between fork and join leaves the side stream capturing
+            hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(b->side, &st) == hipSuccess &&
+                st != hipStreamCaptureStatusNone) {
+                hipGraph_t g2 = nullptr;
+                if (hipStreamEndCapture(b->side, &g2) == hipSuccess && g2) hipGraphDestroy(g2);
+            }
             (void)hipGetLastError();
             batch_epoch_launches(b);
             return;
@@ -1483,17 +1537,18 @@ bht_status bht_job_get_timing(bht_job *job, double *out3) {
     if (!job || !out3) return fail(BHT_ERR_INVALID, "null argument");
     return guarded([&] {
         BHT_HIP(hipStreamSynchronize(job->stream));
-        double rep = 0.0, upd = 0.0;
-        size_t epochs = job->events_used / 3;
+        BHT_HIP(hipStreamSynchronize(job->side));
+        double rep = 0.0, att = 0.0;
+        size_t epochs = job->events_used / 4;
         for (size_t e = 0; e < epochs; e++) {
             float a = 0.f, b = 0.f;
-            BHT_HIP(hipEventElapsedTime(&a, job->events[3 * e], job->events[3 * e + 1]));
-            BHT_HIP(hipEventElapsedTime(&b, job->events[3 * e + 1], job->events[3 * e + 2]));
+            BHT_HIP(hipEventElapsedTime(&a, job->events[4 * e], job->events[4 * e + 1]));
+            BHT_HIP(hipEventElapsedTime(&b, job->events[4 * e], job->events[4 * e + 2]));
             rep += a;
-            upd += b;
+            att += b;
         }
         out3[0] = rep;
-        out3[1] = upd;
+        out3[1] = att;
         out3[2] = (double)epochs;
     });
 }
@@ -1746,7 +1801,8 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
     b += (n + 1) * 4 + nnz * 8 + n * 4;             // CSR + KL weights
     b += row_groups_bytes(nnz);                     // row-group descriptors
     b += 6 * n * D * 4 + 4096 * 8 + 4096 * 16;      // optimiser state + partials
-    b += 2 * (uint64_t)MAX_ATTRACT_BLOCKS * 8;      // update partials
+    b += 2 * (uint64_t)MAX_UPDATE_PARTS * 8;        // update partials
+    b += n * D * 4;                                 // attraction sums
     b += ne * 12 * 4 + radix_sort_temp_bytes(ne);   // affinity-build peak (transient)
     if (!(p.flags & BHT_FLAG_NO_REORDER) && n >= REORDER_MIN_POINTS)
         b += 5 * n * 4 + n * D * 4 + scan_temp_bytes(n);  // renumbering; its transients fit above

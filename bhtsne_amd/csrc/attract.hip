@@ -13,6 +13,10 @@
 // (short rows, from caller-supplied affinities) is processed row by row.  The new embedding goes
 // to the other buffer of the pair because other rows still read y this epoch.
 //
+// The kernel writes each row's attraction sums; k_update then applies the gradient step per point
+// (with the repulsion, which runs concurrently on another stream), so that the attraction does
+// not wait for the repulsion.
+//
 // Each block owns a contiguous run of groups and first copies the positions of an index window
 // centred on its rows into LDS (WIN_FLOATS floats).  The points are numbered so that P's
 // well-connected groups are contiguous, so most gathered columns fall inside the window and are
@@ -23,6 +27,7 @@
 namespace bht {
 
 constexpr int UP_THREADS = 512;
+constexpr int UPD_THREADS = 256;  // k_update runs over the point-parallel (F_PT) block partition
 constexpr int GROUP_UNROLL = 8;
 constexpr int GROUP_EDGES = 64 * GROUP_UNROLL;  // edge window of a row group
 constexpr int MAX_ATT_BLOCKS = MAX_ATTRACT_BLOCKS;
@@ -94,6 +99,12 @@ struct StepCtx {
     float inv_z, momentum, learning_rate;
 };
 
+__device__ __forceinline__ StepCtx step_ctx(const JobDev &J, int epoch) {
+    // the momentum switch takes effect after epoch == momentum_switch_epoch (lib.rs:1173-1175)
+    return StepCtx{J.scal->inv_z, epoch > J.momentum_switch_epoch ? J.final_momentum : J.momentum,
+                   J.learning_rate};
+}
+
 // The job's arrays as global-memory pointers (the kernel mixes them with LDS reads).
 struct AttractMem {
     BHT_GLOBAL const uint32_t *rows, *cols, *groups;
@@ -142,13 +153,6 @@ __device__ __forceinline__ void apply_update(const AttractMem &M, const StepCtx 
         if (d == 0) m0 += (double)yn;
         else m1 += (double)yn;
     }
-}
-
-template <int D>
-__device__ __forceinline__ void update_row(const AttractMem &M, const StepCtx &c, int i,
-                                           const float *acc, const float *yi, double &m0,
-                                           double &m1) {
-    apply_update<D>(M, c, i, acc, yi, load_row_state<D>(M, i), m0, m1);
 }
 
 // clang vector types: loads through address-space-qualified pointers (HIP's float2 / float4
@@ -209,7 +213,9 @@ __device__ YWindow<D> stage_window(BHT_GLOBAL const float *y, int n, int r_begin
     return YWindow<D>{y, win, (uint32_t)lo, (uint32_t)count};
 }
 
-template <int D, bool UPDATE>
+// EPOCH: the sums go to the job's fattr buffer, and the first epoch after stop-lying rescales P as
+// it reads it; otherwise (the forces export) they go to pos_out and nothing else is written.
+template <int D, bool EPOCH>
 __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, const uint32_t *map,
                                                         float *pos_out) {
     __shared__ double red[UP_THREADS / 64];
@@ -227,8 +233,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
     M.qv = global_view(J.qv);
     M.uy = global_view(J.uy);
     M.gains = global_view(J.gains);
-    M.y_next = global_view(J.ybuf[(epoch + 1) & 1]);
-    M.frep = global_view((const float *)J.frep);
+    BHT_GLOBAL float *out = global_view(EPOCH ? J.fattr : pos_out);
     BHT_GLOBAL const uint32_t *rows = M.rows, *cols = M.cols, *groups = M.groups;
     BHT_GLOBAL float *vals = M.vals;
     // this block's groups [g_begin, g_end) and the window around their rows
@@ -239,18 +244,10 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
     const YWindow<D> y = stage_window<D>(global_view((const float *)J.ybuf[epoch & 1]), J.n,
                                          r_begin, r_end, lds_view(win));
     const float inv_exag = J.inv_exag;
-    StepCtx sc{0.0f, J.momentum, J.learning_rate};
-    bool stop_now = false;
-    if (UPDATE) {
-        sc.inv_z = J.scal->inv_z;
-        // stop_lying fires after epoch == stop_lying_epoch (lib.rs:1168-1171): the next epoch is
-        // the first to read the rescaled values, and rescales them in place as it reads (with
-        // the copies of the reference-position values that KL weights by).
-        stop_now = J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
-        // the momentum switch likewise takes effect after epoch == momentum_switch_epoch
-        sc.momentum = epoch > J.momentum_switch_epoch ? J.final_momentum : J.momentum;
-    }
-    double m0 = 0.0, m1 = 0.0;
+    // stop_lying fires after epoch == stop_lying_epoch (lib.rs:1168-1171): the next epoch is the
+    // first to read the rescaled values, and rescales them in place as it reads (with the copies
+    // of the reference-position values that KL weights by).
+    const bool stop_now = EPOCH && J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
     // the wave index as a uniform value, so that the group loop and its descriptor loads are scalar
     const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (int g = g_begin + wid; g < g_end; g += NW) {
@@ -260,7 +257,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
         for (int q = 0; q < 2 + MAX_GROUP_ROWS + 1; q++) dw[q] = desc[q];
         const int r0 = (int)dw[0], nr = (int)dw[1], r1 = r0 + nr;
         if (nr <= 0) continue;
-        if (UPDATE && stop_now)
+        if (stop_now)
             for (int i = r0 + lane; i < r1 && i < J.qv_n; i += 64) M.qv[i] = M.qv[i] * inv_exag;
         if (nr > MAX_GROUP_ROWS) {
             // many short rows: one row at a time, lanes over its edges
@@ -269,7 +266,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
                 float a0 = 0.0f, a1 = 0.0f;
                 for (uint32_t e = rows[i] + lane; e < rows[i + 1]; e += 64) {
                     float pv = vals[e];
-                    if (UPDATE && stop_now) {
+                    if (stop_now) {
                         pv = pv * inv_exag;
                         vals[e] = pv;
                     }
@@ -286,23 +283,10 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
                 a0 = wave_sum_f32(a0);
                 if (D == 2) a1 = wave_sum_f32(a1);
                 if (lane != 0) continue;
-                const float acc[2] = {a0, a1}, yv[2] = {yi.x, yi.y};
-                if (!UPDATE) {
-                    pos_out[(size_t)i * D] = a0;
-                    if (D == 2) pos_out[(size_t)i * D + 1] = a1;
-                } else {
-                    update_row<D>(M, sc, i, acc, yv, m0, m1);
-                }
+                out[(size_t)i * D] = a0;
+                if (D == 2) out[(size_t)i * D + 1] = a1;
             }
             continue;
-        }
-        // the operands of lane q's update (row r0 + q), loaded now so that their latency overlaps
-        // the edge loop
-        RowState<D> own{};
-        float2 yown = make_float2(0.0f, 0.0f);
-        if (UPDATE && lane < nr) {
-            own = load_row_state<D>(M, r0 + lane);
-            yown = y(r0 + lane);
         }
         // the group's row offsets and positions (uniform across the wave)
         const uint32_t *bnd = dw + 2;
@@ -323,7 +307,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
                 const bool ok = e < e_end;
                 col[u] = ok ? __builtin_nontemporal_load(cols + e) : (uint32_t)r0;
                 pv[u] = ok ? __builtin_nontemporal_load(vals + e) : 0.0f;
-                if (UPDATE && stop_now && ok) {
+                if (stop_now && ok) {
                     pv[u] = pv[u] * inv_exag;
                     vals[e] = pv[u];
                 }
@@ -376,19 +360,40 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
         }
         if (lane >= nr) continue;
         const int i = r0 + lane;
-        const float accv[2] = {mine0, mine1};
-        if (!UPDATE) {
-            pos_out[(size_t)i * D] = mine0;
-            if (D == 2) pos_out[(size_t)i * D + 1] = mine1;
-        } else {
-            const float yv[2] = {yown.x, yown.y};
-            apply_update<D>(M, sc, i, accv, yv, own, m0, m1);
-        }
+        out[(size_t)i * D] = mine0;
+        if (D == 2) out[(size_t)i * D + 1] = mine1;
     }
-    if (!UPDATE) return;
-    // per-block sums of the new embedding for the next recentring
-    double s0 = block_sum_f64(m0, red);
-    double s1 = D == 2 ? block_sum_f64(m1, red) : 0.0;
+}
+
+// Gradient step of every point from its attraction sums (fattr) and the repulsion (frep, Z), and
+// the per-block sums of the new embedding for the next recentring.
+template <int D>
+__global__ void __launch_bounds__(UPD_THREADS) k_update(const JobDev *jobs, const uint32_t *map) {
+    __shared__ double red[UPD_THREADS / 64];
+    const BlockCtx bc = block_ctx(map, jobs, F_PT);
+    const JobDev &J = jobs[bc.job];
+    const int epoch = J.scal->epoch;
+    const StepCtx sc = step_ctx(J, epoch);
+    AttractMem M;
+    M.uy = global_view(J.uy);
+    M.gains = global_view(J.gains);
+    M.y_next = global_view(J.ybuf[(epoch + 1) & 1]);
+    M.frep = global_view((const float *)J.frep);
+    BHT_GLOBAL const float *y = global_view((const float *)J.ybuf[epoch & 1]);
+    BHT_GLOBAL const float *fattr = global_view((const float *)J.fattr);
+    double m0 = 0.0, m1 = 0.0;
+    const int stride = bc.nb * UPD_THREADS;
+    for (int i = bc.lb * UPD_THREADS + threadIdx.x; i < J.n; i += stride) {
+        float acc[2] = {0.0f, 0.0f}, yi[2] = {0.0f, 0.0f};
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            acc[d] = fattr[(size_t)i * D + d];
+            yi[d] = y[(size_t)i * D + d];
+        }
+        apply_update<D>(M, sc, i, acc, yi, load_row_state<D>(M, i), m0, m1);
+    }
+    const double s0 = block_sum_f64(m0, red);
+    const double s1 = D == 2 ? block_sum_f64(m1, red) : 0.0;
     if (threadIdx.x == 0) {
         J.mpart[2 * bc.lb] = s0;
         J.mpart[2 * bc.lb + 1] = s1;
@@ -399,7 +404,7 @@ __global__ void __launch_bounds__(UP_THREADS) k_attract(const JobDev *jobs, cons
     }
 }
 
-void attract_update(const LaunchSet &L, hipStream_t s) {
+void attract_forces(const LaunchSet &L, hipStream_t s) {
     const dim3 g(L.nb[F_ATT]);
     if (L.dim == 2)
         hipLaunchKernelGGL((k_attract<2, true>), g, dim3(UP_THREADS), 0, s, L.jobs, L.map[F_ATT],
@@ -407,6 +412,15 @@ void attract_update(const LaunchSet &L, hipStream_t s) {
     else
         hipLaunchKernelGGL((k_attract<1, true>), g, dim3(UP_THREADS), 0, s, L.jobs, L.map[F_ATT],
                            nullptr);
+    BHT_LAUNCH_CHECK();
+}
+
+void update_points(const LaunchSet &L, hipStream_t s) {
+    const dim3 g(L.nb[F_PT]);
+    if (L.dim == 2)
+        hipLaunchKernelGGL(k_update<2>, g, dim3(UPD_THREADS), 0, s, L.jobs, L.map[F_PT]);
+    else
+        hipLaunchKernelGGL(k_update<1>, g, dim3(UPD_THREADS), 0, s, L.jobs, L.map[F_PT]);
     BHT_LAUNCH_CHECK();
 }
 

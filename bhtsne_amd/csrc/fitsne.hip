@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_mean_finalize(const JobDev *job
     const JobDev &J = jobs[bc.job];
     EpochScalars *scal = J.scal;
     const double *mpart = J.mpart;
-    const int nparts = J.nblk[F_ATT], n = J.n;
+    const int nparts = J.nblk[F_PT], n = J.n;  // partials of k_update
     const int pending = scal->mean_pending;
     if (!pending) {
         if (threadIdx.x == 0) {

@@ -89,6 +89,7 @@ struct JobDev {
     // optimiser state; the embedding of epoch e is ybuf[e & 1]
     float *ybuf[2];
     float *uy, *gains, *frep;
+    float *fattr;       // attraction sums of the epoch (k_attract -> k_update)
     const uint32_t *rows, *cols;
     float *vals;
     const uint32_t *groups;  // row-group descriptors of the attraction kernel (ngroups + 1)
@@ -135,7 +136,8 @@ __device__ __forceinline__ float *job_y(const JobDev &J) { return J.ybuf[J.scal-
 // Block counts of one job (the same formulas size the single-job grids and the batch maps).
 int point_blocks(int n);
 int attract_blocks(int ngroups);
-constexpr int MAX_ATTRACT_BLOCKS = 16384;  // partial sums of the update: 2 doubles per block
+constexpr int MAX_ATTRACT_BLOCKS = 16384;
+constexpr int MAX_UPDATE_PARTS = 1024;  // point_blocks() cap: partial sums of the update
 // Row groups of the attraction kernel for a CSR of nnz entries (attract.hip).
 uint32_t row_groups_count(uint64_t nnz);
 // Row group m is described by GROUP_DESC_WORDS words: its first row, its row count, the CSR
@@ -216,8 +218,10 @@ void fft_convolve(const LaunchSet &L, hipStream_t s);
 // Barnes-Hut (bh.hip): after center_plan (no plan), frep and Z; returns job 0's sorted indices.
 const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s);
 
-// Attraction + fused update (attract.hip).  rows are u32 CSR offsets (nnz < 2^32).
-void attract_update(const LaunchSet &L, hipStream_t s);
+// Attraction sums of the epoch into fattr (attract.hip; rows are u32 CSR offsets, nnz < 2^32),
+// and the gradient step from fattr, frep and Z.
+void attract_forces(const LaunchSet &L, hipStream_t s);
+void update_points(const LaunchSet &L, hipStream_t s);
 // Attraction alone of job 0 into pos.
 void attract_only(const LaunchSet &L, float *pos, hipStream_t s);
 void scale_values(float *vals, uint64_t nnz, float scale, hipStream_t s);

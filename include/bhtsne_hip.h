@@ -151,9 +151,11 @@ bht_status bht_job_run_state(const bht_job *job, uint64_t *next_epoch, int *exag
 bht_status bht_job_grid_info(const bht_job *job, uint32_t *boxes, uint32_t *fft_len, float *lo,
                              float *hi);
 
-/* Per-phase device time of the epochs run while timing is enabled, measured with HIP events on
- * the job's own stream.  out[0] = repulsion ms (grid plan .. gather), out[1] = attraction+update
- * ms, out[2] = epochs timed.  Enabling resets the counters. */
+/* Per-phase device time of the epochs run while timing is enabled, measured with HIP events.
+ * After the recentring an epoch runs the repulsion and the attraction concurrently (two streams)
+ * and then the update.  out[0] = repulsion ms (box sort .. Z, on the second stream), out[1] =
+ * attraction ms (the attraction kernel), both from the fork; out[2] = epochs timed.  Enabling
+ * resets the counters. */
 bht_status bht_job_set_timing(bht_job *job, int enable);
 bht_status bht_job_get_timing(bht_job *job, double *out3);
 
