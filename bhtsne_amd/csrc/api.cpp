@@ -191,8 +191,10 @@ struct bht_job {
     uint32_t d_in = 0;
     bool have_data = false;
 
-    // affinities: CSR with u32 offsets on the device, and its row groups (attraction work units)
-    DevBuf rows, cols, vals, groups;
+    // affinities: CSR with u32 offsets on the device, and its column-tiled copy (the attraction's
+    // layout: edges, values, block / segment / group tables)
+    DevBuf rows, cols, vals;
+    DevBuf tkey, tval, tblk, tseg_tile, tseg_e0, tseg_g0, tgrp;
     uint64_t nnz = 0;
     bool have_p = false;
     bool have_cached_perplexity = false;
@@ -380,8 +382,16 @@ void fill_jobdev(bht_job *j) {
     d.rows = j->rows.as<uint32_t>();
     d.cols = j->cols.as<uint32_t>();
     d.vals = j->vals.as<float>();
-    d.groups = j->groups.as<uint32_t>();
-    d.ngroups = (int)row_groups_count(j->nnz);
+    d.tile_rows = tiled_rows_per_block((int)j->n);
+    d.tile_points = tiled_tile_points((int)j->dim());
+    d.tile_nnz = (uint32_t)j->nnz;
+    d.tkey = j->tkey.as<uint32_t>();
+    d.tval = j->tval.as<float>();
+    d.blk_seg = j->tblk.as<uint32_t>();
+    d.seg_tile = j->tseg_tile.as<uint32_t>();
+    d.seg_e0 = j->tseg_e0.as<uint32_t>();
+    d.seg_g0 = j->tseg_g0.as<uint32_t>();
+    d.grp_e0 = j->tgrp.as<uint32_t>();
     d.qv = j->qv.as<float>();
     d.qv_n = (int)std::min<uint64_t>(j->n, j->nnz);
     d.scal = j->scal.as<EpochScalars>();
@@ -412,16 +422,34 @@ void fill_jobdev(bht_job *j) {
     job_block_counts(d, false);
 }
 
-// Row groups of the current CSR layout (after any change of rows: build, injection, renumbering).
-void refresh_row_groups(bht_job *j) {
-    j->groups.ensure(j->ctx, row_groups_bytes(j->nnz));
-    build_row_groups(j->rows.as<uint32_t>(), (int)j->n, j->nnz, j->groups.as<uint32_t>(),
-                     j->stream);
+// Column-tiled copy of the current P (after any change of the CSR or of its values before a run:
+// build, injection, normalisation, renumbering).
+void refresh_tiled(bht_job *j) {
+    bht_ctx *c = j->ctx;
+    const int n = (int)j->n, D = (int)j->dim();
+    const TiledSizes z = tiled_sizes(n, D, j->nnz);
+    const uint64_t nnz1 = std::max<uint64_t>(j->nnz, 1);
+    j->tkey.ensure(c, nnz1 * 4);
+    j->tval.ensure(c, nnz1 * 4);
+    j->tblk.ensure(c, ((size_t)tiled_blocks(n) + 1) * 4);
+    j->tseg_tile.ensure(c, z.max_segs * 4);
+    j->tseg_e0.ensure(c, z.max_segs * 4);
+    j->tseg_g0.ensure(c, z.max_segs * 4);
+    j->tgrp.ensure(c, (z.max_groups + 1) * 4);
+    DevBuf temp(c);
+    temp.ensure(c, z.temp);
+    TiledOut o{j->tkey.as<uint32_t>(), j->tval.as<float>(), j->tblk.as<uint32_t>(),
+               j->tseg_tile.as<uint32_t>(), j->tseg_e0.as<uint32_t>(), j->tseg_g0.as<uint32_t>(),
+               j->tgrp.as<uint32_t>()};
+    tiled_build(j->rows.as<uint32_t>(), j->cols.as<uint32_t>(), j->vals.as<float>(), n, D,
+                j->nnz, o, temp.p, j->stream);
+    BHT_HIP(hipStreamSynchronize(j->stream));  // before the workspace is released
 }
 
-// Refreshes the job's own device table entry (single-job launches).
-void sync_jobdev(bht_job *j) {
-    refresh_row_groups(j);
+// Refreshes the job's own device table entry (single-job launches); with `layout`, also the
+// tiled copy of P.
+void sync_jobdev(bht_job *j, bool layout = true) {
+    if (layout) refresh_tiled(j);
     fill_jobdev(j);
     j->jd.ensure(j->ctx, sizeof(JobDev));
     BHT_HIP(hipMemcpyAsync(j->jd.p, &j->hd, sizeof(JobDev), hipMemcpyHostToDevice, j->stream));
@@ -730,8 +758,10 @@ void prepare_run(bht_job *j, int method, float theta, bool cached) {
     j->epoch = 0;
     j->run_momentum = j->params.momentum;
     j->prepared = true;
-    sync_jobdev(j);
-    if (reorder_enabled(j)) reorder_by_diffusion(j);
+    // the tiled copy of P is built once the points have their final numbering
+    const bool renumber = reorder_enabled(j);
+    sync_jobdev(j, !renumber);
+    if (renumber) reorder_by_diffusion(j);
 }
 
 // Repulsive forces and Z of the current embedding.
@@ -776,7 +806,7 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
         if (fit) fitsne_repulsion(L, rs);
         else if (j->n > 0) bh_repulsion(L, rs);
         j->mark(rs);
-        attract_forces(L, j->stream);
+        attract_tiled(L, nullptr, true, j->stream);
         j->mark(j->stream);
         if (fork) {
             BHT_HIP(hipEventRecord(j->ev_join, j->side));
@@ -798,12 +828,15 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
 void finish_run(bht_job *j) {
     if (!j->prepared) throw Error(ERR_STATE, "no prepared run");
     recenter(j);
-    // stop-lying of the last epoch has not been applied lazily by a following epoch
+    // The epochs read the tiled copy of P and rescale it (tval) at stop-lying; the CSR values
+    // (exports, KL) are rescaled here once the schedule has stopped lying.  If the run ended
+    // right after stop_lying_epoch, no epoch has rescaled the tiled copy yet either.
     const uint64_t sl = j->params.stop_lying_epoch;
-    if (sl != 0 && j->epoch == sl + 1) {
+    if (sl != 0 && j->epoch >= sl + 1) {
         const float inv = 1.0f / j->params.early_exaggeration;
         scale_values(j->vals.as<float>(), j->nnz, inv, j->stream);
         scale_values(j->qv.as<float>(), std::min<uint64_t>(j->n, j->nnz), inv, j->stream);
+        if (j->epoch == sl + 1) scale_values(j->tval.as<float>(), j->nnz, inv, j->stream);
     }
     BHT_HIP(hipStreamSynchronize(j->stream));
     j->fit_kind = j->method;
@@ -919,10 +952,6 @@ struct BatchGroup {
 struct bht_batch {
     bht_ctx *ctx;
     hipStream_t stream = nullptr;
-    // each group's repulsion runs on `side` beside its attraction (fork / join events; under
-    // capture they become parallel branches of the graph)
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<bht_job *> members;
     std::vector<bht_job *> finished;
     std::vector<std::unique_ptr<BatchGroup>> groups;
@@ -938,15 +967,9 @@ struct bht_batch {
     explicit bht_batch(bht_ctx *c) : ctx(c) {
         BHT_HIP(hipSetDevice(ctx->device));
         BHT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        BHT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-        BHT_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-        BHT_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     }
     ~bht_batch() {
         drop_graph();
-        if (ev_fork) hipEventDestroy(ev_fork);
-        if (ev_join) hipEventDestroy(ev_join);
-        if (side) hipStreamDestroy(side);
         if (stream) hipStreamDestroy(stream);
     }
     void drop_graph() {
@@ -1081,21 +1104,20 @@ void batch_rebuild(bht_batch *b) {
     b->dirty = false;
 }
 
-// One epoch of every member: per group the recentring, then the repulsion (on the side stream)
-// beside the attraction, then the update.
+// One epoch of every member: per group the recentring, the repulsion, the attraction and the
+// update, in order on the batch stream.  (The many jobs of a batch fill the device without the
+// two-stream fork of a single large job; batches of many jobs in two groups that ran the
+// repulsion beside the attraction came out different from their members run alone, in graph
+// replay as well as in plain launches, so the batch does not fork.)
 void batch_epoch_launches(bht_batch *b) {
     for (auto &gp : b->groups) {
         const LaunchSet &L = gp->L;
         if (L.njobs == 0) continue;
         const bool fit = gp->method == M_FITSNE;
         center_plan(L, fit, b->stream);
-        BHT_HIP(hipEventRecord(b->ev_fork, b->stream));
-        BHT_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
-        if (fit) fitsne_repulsion(L, b->side);
-        else bh_repulsion(L, b->side);
-        attract_forces(L, b->stream);
-        BHT_HIP(hipEventRecord(b->ev_join, b->side));
-        BHT_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+        if (fit) fitsne_repulsion(L, b->stream);
+        else bh_repulsion(L, b->stream);
+        attract_tiled(L, nullptr, true, b->stream);
         update_points(L, b->stream);
     }
 }
@@ -1121,13 +1143,6 @@ void batch_launch_epoch(bht_batch *b, bool may_capture) {
         const hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (!recorded || e != hipSuccess || g == nullptr) {
             if (g) hipGraphDestroy(g);
-            // a capture abandoned between fork and join leaves the side stream capturing
-            hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-            if (hipStreamIsCapturing(b->side, &st) == hipSuccess &&
-                st != hipStreamCaptureStatusNone) {
-                hipGraph_t g2 = nullptr;
-                if (hipStreamEndCapture(b->side, &g2) == hipSuccess && g2) hipGraphDestroy(g2);
-            }
             (void)hipGetLastError();
             batch_epoch_launches(b);
             return;
@@ -1389,11 +1404,12 @@ bht_status bht_job_get_affinities(const bht_job *job, uint64_t *rows, uint32_t *
         if (cols && job->nnz) std::memcpy(cols, c.data(), job->nnz * 4);
         if (vals) {
             if (job->nnz) std::memcpy(vals, v.data(), job->nnz * 4);
-            // lib.rs:1023-1029; between epoch stop_lying_epoch and the next one the reference has
-            // rescaled P while the device values are rescaled by that next epoch
+            // lib.rs:1023-1029: the export holds P without the exaggeration.  The device CSR values
+            // keep it for the whole of a run that has a stop-lying epoch (the epochs rescale
+            // their tiled copy) and lose it at the end of the run once stop-lying has fired.
             const uint64_t sl = job->params.stop_lying_epoch;
-            const bool pending = job->prepared && sl != 0 && job->epoch == sl + 1;
-            if (!job->stop_lying_fired || pending) {
+            const bool exaggerated = job->prepared ? sl != 0 : sl != 0 && !job->stop_lying_fired;
+            if (exaggerated) {
                 const float scale = 1.0f / job->params.early_exaggeration;
                 for (uint64_t i = 0; i < job->nnz; i++) vals[i] = vals[i] * scale;
             }
@@ -1466,7 +1482,7 @@ bht_status bht_job_forces(bht_job *job, float *attractive, float *repulsive, flo
         if (!job->prepared) throw Error(ERR_STATE, "bht_job_prepare must precede bht_job_forces");
         if (job->in_batch) throw Error(ERR_STATE, "the job is advanced by its batch");
         repulsion(job);
-        attract_only(single_set(job), job->pos.as<float>(), job->stream);
+        attract_tiled(single_set(job), job->pos.as<float>(), false, job->stream);
         EpochScalars sc;
         BHT_HIP(hipMemcpyAsync(&sc, job->scal.p, sizeof(sc), hipMemcpyDeviceToHost, job->stream));
         if (attractive) download_points(job, job->pos.as<float>(), attractive);
@@ -1799,11 +1815,13 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
     const uint64_t ne = n * k, nnz = 2 * ne;
     uint64_t b = n * d_in * 4;                      // samples
     b += (n + 1) * 4 + nnz * 8 + n * 4;             // CSR + KL weights
-    b += row_groups_bytes(nnz);                     // row-group descriptors
+    const TiledSizes tz = tiled_sizes((int)n, (int)D, nnz);
+    b += tz.persistent;                             // tiled copy of P
     b += 6 * n * D * 4 + 4096 * 8 + 4096 * 16;      // optimiser state + partials
     b += 2 * (uint64_t)MAX_UPDATE_PARTS * 8;        // update partials
     b += n * D * 4;                                 // attraction sums
-    b += ne * 12 * 4 + radix_sort_temp_bytes(ne);   // affinity-build peak (transient)
+    // transients, one at a time: the affinity build, the tiled layout build
+    b += std::max<uint64_t>(ne * 12 * 4 + radix_sort_temp_bytes(ne), tz.temp);
     if (!(p.flags & BHT_FLAG_NO_REORDER) && n >= REORDER_MIN_POINTS)
         b += 5 * n * 4 + n * D * 4 + scan_temp_bytes(n);  // renumbering; its transients fit above
     if (method == BHT_METHOD_BARNES_HUT) {

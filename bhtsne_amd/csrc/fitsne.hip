@@ -31,7 +31,7 @@ void job_block_counts(JobDev &J, bool batched) {
     J.nblk[F_PT] = point_blocks(n);
     J.nblk[F_ONE] = 1;
     J.nblk[F_SPR] = spread_blocks(n);
-    J.nblk[F_ATT] = attract_blocks(J.ngroups);
+    J.nblk[F_ATT] = tiled_blocks(n);
     J.nblk[F_TRAV] = traverse_blocks(n);
     J.nblk[F_LINE1] = 2;
     // transform lines per job: 2 * nodes (rows) and 2 * L (columns), a few hundred for batch-sized

@@ -92,8 +92,12 @@ struct JobDev {
     float *fattr;       // attraction sums of the epoch (k_attract -> k_update)
     const uint32_t *rows, *cols;
     float *vals;
-    const uint32_t *groups;  // row-group descriptors of the attraction kernel (ngroups + 1)
-    int ngroups;
+    // column-tiled copy of P for the attraction (tiled.hip): rows in blocks of tile_rows, columns
+    // in tiles of tile_points; tval is the copy the epochs read (and rescale at stop-lying)
+    uint32_t tile_rows, tile_points, tile_nnz;
+    const uint32_t *tkey;
+    float *tval;
+    const uint32_t *blk_seg, *seg_tile, *seg_e0, *seg_g0, *grp_e0;
     float *qv;          // values at the reference's CSR positions [0, qv_n) (KL weights)
     int qv_n;
     EpochScalars *scal;
@@ -135,18 +139,24 @@ __device__ __forceinline__ float *job_y(const JobDev &J) { return J.ybuf[J.scal-
 
 // Block counts of one job (the same formulas size the single-job grids and the batch maps).
 int point_blocks(int n);
-int attract_blocks(int ngroups);
-constexpr int MAX_ATTRACT_BLOCKS = 16384;
 constexpr int MAX_UPDATE_PARTS = 1024;  // point_blocks() cap: partial sums of the update
-// Row groups of the attraction kernel for a CSR of nnz entries (attract.hip).
-uint32_t row_groups_count(uint64_t nnz);
-// Row group m is described by GROUP_DESC_WORDS words: its first row, its row count, the CSR
-// offsets of its first MAX_GROUP_ROWS + 1 rows (all-ones past the count), and a pad word.
-// Descriptor ngroups is a sentinel whose first row is n.
-constexpr int MAX_GROUP_ROWS = 8;
-constexpr int GROUP_DESC_WORDS = 12;
-size_t row_groups_bytes(uint64_t nnz);
-void build_row_groups(const uint32_t *rows, int n, uint64_t nnz, uint32_t *groups, hipStream_t s);
+// Column-tiled layout of P (tiled.hip).
+struct TiledSizes {
+    uint64_t max_segs, max_groups;  // bounds of the segment and group tables
+    uint64_t persistent;            // bytes of the layout itself
+    uint64_t temp;                  // bytes of workspace of tiled_build
+};
+struct TiledOut {
+    uint32_t *tkey;
+    float *tval;
+    uint32_t *blk_seg, *seg_tile, *seg_e0, *seg_g0, *grp_e0;
+};
+uint32_t tiled_rows_per_block(int n);
+uint32_t tiled_tile_points(int dim);
+int tiled_blocks(int n);
+TiledSizes tiled_sizes(int n, int dim, uint64_t nnz);
+void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, int n, int dim,
+                 uint64_t nnz, const TiledOut &o, void *temp, hipStream_t s);
 int spread_blocks(int n);
 int traverse_blocks(int n);
 void job_block_counts(JobDev &J, bool batched);
@@ -218,12 +228,11 @@ void fft_convolve(const LaunchSet &L, hipStream_t s);
 // Barnes-Hut (bh.hip): after center_plan (no plan), frep and Z; returns job 0's sorted indices.
 const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s);
 
-// Attraction sums of the epoch into fattr (attract.hip; rows are u32 CSR offsets, nnz < 2^32),
-// and the gradient step from fattr, frep and Z.
-void attract_forces(const LaunchSet &L, hipStream_t s);
+// Attraction sums over the tiled layout (tiled.hip): epoch = true writes fattr (and applies the
+// stop-lying rescale to tval), epoch = false writes job 0's sums into pos.
+void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s);
+// The gradient step from fattr, frep and Z (attract.hip).
 void update_points(const LaunchSet &L, hipStream_t s);
-// Attraction alone of job 0 into pos.
-void attract_only(const LaunchSet &L, float *pos, hipStream_t s);
 void scale_values(float *vals, uint64_t nnz, float scale, hipStream_t s);
 // Sparse KL (kl.hip): writes scal->kl_ref / kl_true using scal->inv_z.
 void sparse_kl(const float *y, int n, int dim, const uint32_t *rows, const uint32_t *cols,

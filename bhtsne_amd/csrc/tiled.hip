@@ -1,0 +1,470 @@
+// Column-tiled layout of the affinity CSR and the attraction kernel that reads it
+// (src/tsne/arena.rs:398-427: the attractive half of the gradient, sum_j p_ij q_ij (y_i - y_j)).
+//
+// The attraction gathers y[col] for every edge.  With the points in the diffusion order of P, the
+// columns of a few thousand consecutive rows fall in a handful of index ranges.  So the rows are cut
+// into blocks of TILE_ROWS rows (one workgroup each) and the columns into tiles of WIN_FLOATS / D
+// points (one LDS window each), and a block's edges are stored tile by tile: for each tile the
+// block copies that tile's positions into LDS once and then reads every gathered position from LDS.
+//
+// Layout (built once per CSR layout by tiled_build):
+//   tkey[p], tval[p]   the edges, sorted by (row block, tile, row), stable; tkey packs the row
+//                      within its block (high 16 bits) and the column within its tile (low 16).
+//   blk_seg[b]         first segment of row block b (b = 0 .. nblocks; sentinel = nsegs)
+//   seg_tile[s]        tile of segment s (a segment = one block's edges in one tile)
+//   seg_e0[s]          first edge of segment s (sentinel = nnz)
+//   seg_g0[s]          first group of segment s (sentinel = ngroups)
+//   grp_e0[g]          first edge of group g (sentinel = nnz)
+// A group is the rows of a segment whose first edge lies in one GROUP_SPAN-edge window of it, so
+// a row's edges in a segment belong to exactly one group, which one wave processes alone.
+//
+// Within a group a wave takes GROUP_SPAN edges per pass, 8 consecutive edges per lane.  Each lane
+// sums its edges by row (runs of equal keys), rows complete inside the lane are added to the
+// block's LDS accumulators at once, and the rest are joined across lanes by a segmented scan; the
+// lane where a row ends adds its sum.  Every accumulator is written by one wave per segment, in a
+// fixed order, so the sums are deterministic.  After the last segment the block writes its rows'
+// sums out.
+#include "internal.hpp"
+
+namespace bht {
+
+namespace {
+
+constexpr int T_THREADS = 512;
+constexpr int T_WAVES = T_THREADS / 64;
+constexpr int TILE_ROWS_MAX = 2048;
+constexpr int GROUP_SPAN = 512;  // edges per wave pass: 64 lanes x 8
+// LDS per block: window + the block's row positions + the accumulators, under 80,064 bytes so
+// that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran at half speed)
+constexpr int WIN_FLOATS = 11800;
+constexpr uint32_t NO_ROW = 0xffffu;
+
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+int bits_for(uint64_t count) {  // bits to hold 0 .. count - 1
+    int b = 0;
+    while (b < 64 && (1ull << b) < count) b++;
+    return b;
+}
+
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---- layout build ---------------------------------------------------------------------------
+
+struct TileGeom {
+    uint32_t rows_per_block, tile_points;
+    int row_bits, tile_bits;
+};
+
+// key = block << (tile_bits + row_bits) | tile << row_bits | row within block; one wave per row
+__global__ void k_tile_keys(const uint32_t *rows, const uint32_t *cols, int n, TileGeom g,
+                            uint64_t *keys, uint32_t *idx) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int i = wave; i < n; i += nwaves) {
+        const uint64_t b = (uint32_t)i / g.rows_per_block, r = (uint32_t)i % g.rows_per_block;
+        for (uint32_t e = rows[i] + lane; e < rows[i + 1]; e += 64) {
+            const uint64_t t = cols[e] / g.tile_points;
+            keys[e] = (b << (g.tile_bits + g.row_bits)) | (t << g.row_bits) | r;
+            idx[e] = e;
+        }
+    }
+}
+
+// Packs the sorted edges and flags the first edge of every segment and of every (segment, row).
+__global__ void k_tile_pack(const uint64_t *keys, const uint32_t *idx, const uint32_t *cols,
+                            const float *vals, uint64_t nnz, TileGeom g, uint32_t *tkey,
+                            float *tval, uint32_t *seg_flag, uint32_t *run_flag) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < nnz; p += stride) {
+        const uint64_t k = keys[p];
+        const uint32_t e = idx[p];
+        const uint32_t r = (uint32_t)(k & ((1ull << g.row_bits) - 1));
+        const uint32_t t = (uint32_t)((k >> g.row_bits) & ((1ull << g.tile_bits) - 1));
+        tkey[p] = (r << 16) | (cols[e] - t * g.tile_points);
+        tval[p] = vals[e];
+        const uint64_t prev = p > 0 ? keys[p - 1] : ~0ull;
+        seg_flag[p] = (p == 0 || (prev >> g.row_bits) != (k >> g.row_bits)) ? 1u : 0u;
+        run_flag[p] = (p == 0 || prev != k) ? 1u : 0u;
+    }
+}
+
+// Segment table from the flags and their exclusive scan; runs compacted the same way.
+__global__ void k_tile_segments(const uint64_t *keys, const uint32_t *seg_flag,
+                                const uint32_t *seg_pos, const uint32_t *run_flag,
+                                const uint32_t *run_pos, uint64_t nnz, TileGeom g,
+                                uint32_t *seg_e0, uint32_t *seg_tile, uint32_t *seg_blk,
+                                uint32_t *runs) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < nnz; p += stride) {
+        if (seg_flag[p]) {
+            const uint64_t k = keys[p];
+            const uint32_t s = seg_pos[p];
+            seg_e0[s] = (uint32_t)p;
+            seg_tile[s] = (uint32_t)((k >> g.row_bits) & ((1ull << g.tile_bits) - 1));
+            seg_blk[s] = (uint32_t)(k >> (g.row_bits + g.tile_bits));
+        }
+        if (run_flag[p]) runs[run_pos[p]] = (uint32_t)p;
+    }
+}
+
+// Groups per segment (windows of GROUP_SPAN edges), zero past the last segment (the counts are
+// scanned over the whole table), and the sentinel.
+__global__ void k_tile_seg_groups(uint32_t *seg_e0, const uint32_t *nsegs_p, uint64_t nnz,
+                                  uint32_t max_segs, uint32_t *seg_ngrp) {
+    const uint32_t nsegs = *nsegs_p;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < max_segs; s += stride) {
+        if (s >= nsegs) {
+            if (s == nsegs) seg_e0[s] = (uint32_t)nnz;
+            seg_ngrp[s] = 0;
+            continue;
+        }
+        const uint32_t end = s + 1 < nsegs ? seg_e0[s + 1] : (uint32_t)nnz;
+        seg_ngrp[s] = (end - seg_e0[s] + GROUP_SPAN - 1) / GROUP_SPAN;
+    }
+}
+
+// blk_seg[b] = first segment of block b or later; group starts: the first row run at or after
+// each window start, clipped to its segment.
+__global__ void k_tile_index(const uint32_t *seg_blk, const uint32_t *seg_e0,
+                             const uint32_t *seg_g0, const uint32_t *runs, const uint32_t *counts,
+                             uint32_t nblocks, uint64_t nnz, uint32_t *blk_seg,
+                             uint32_t *grp_e0) {
+    const uint32_t nsegs = counts[0], nruns = counts[1], ngroups = seg_g0[nsegs];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t b = tid; b <= nblocks; b += stride)
+        blk_seg[b] = b == nblocks ? nsegs : lower_bound_u32(seg_blk, nsegs, b);
+    for (uint32_t q = tid; q <= ngroups; q += stride) {
+        if (q == ngroups) {
+            grp_e0[q] = (uint32_t)nnz;
+            continue;
+        }
+        // segment of group q: the last s with seg_g0[s] <= q
+        uint32_t lo = 0, hi = nsegs;
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (seg_g0[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t s = lo;
+        const uint32_t w = seg_e0[s] + (q - seg_g0[s]) * GROUP_SPAN, end = seg_e0[s + 1];
+        const uint32_t k = lower_bound_u32(runs, nruns, w);
+        grp_e0[q] = k < nruns && runs[k] < end ? runs[k] : end;
+    }
+}
+
+// ---- attraction -------------------------------------------------------------------------------
+
+template <int D>
+__device__ __forceinline__ float2 lds_pos(const float *a, uint32_t i) {
+    if (D == 2) {
+        const f2v v = ((BHT_LDS const f2v *)lds_view(a))[i];
+        return make_float2(v.x, v.y);
+    }
+    return make_float2(lds_view(a)[i], 0.0f);
+}
+
+// Copies points [first, first + count) of y (D floats each) into LDS; 16-byte loads when the
+// start is 16-byte aligned.
+template <int D>
+__device__ void stage_points(BHT_GLOBAL const float *y, uint32_t first, uint32_t count,
+                             float *dst) {
+    const uint32_t nf = count * D;
+    BHT_GLOBAL const float *src = y + (size_t)first * D;
+    BHT_LDS float *out = lds_view(dst);
+    if (((first * D) & 3u) == 0) {
+        for (uint32_t k = threadIdx.x; 4 * k < nf; k += T_THREADS) {
+            if (4 * k + 3 < nf) ((BHT_LDS f4v *)out)[k] = ((BHT_GLOBAL const f4v *)src)[k];
+            else
+                for (uint32_t t = 4 * k; t < nf; t++) out[t] = src[t];
+        }
+    } else {
+        for (uint32_t k = threadIdx.x; k < nf; k += T_THREADS) out[k] = src[k];
+    }
+}
+
+// Adds v to the accumulator of row r (exclusive to the calling lane).
+template <int D>
+__device__ __forceinline__ void acc_add(float *acc, uint32_t r, float v0, float v1) {
+    BHT_LDS float *a = lds_view(acc);
+    a[r * D] += v0;
+    if (D == 2) a[r * D + 1] += v1;
+}
+
+template <int D, bool EPOCH>
+__global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
+                                                             const uint32_t *map,
+                                                             float *pos_out) {
+    __shared__ __align__(16) float win[WIN_FLOATS];
+    __shared__ __align__(16) float yrow[TILE_ROWS_MAX * D];
+    __shared__ __align__(16) float acc[TILE_ROWS_MAX * D];
+    const BlockCtx bc = block_ctx(map, jobs, F_ATT);
+    const JobDev &J = jobs[bc.job];
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int epoch = __builtin_amdgcn_readfirstlane(J.scal->epoch);
+    const uint32_t n = (uint32_t)J.n;
+    const uint32_t R = J.tile_rows, W = J.tile_points;
+    const uint32_t nnz = J.tile_nnz;
+    BHT_GLOBAL const float *y = global_view((const float *)J.ybuf[epoch & 1]);
+    BHT_GLOBAL const uint32_t *tkey = global_view(J.tkey);
+    BHT_GLOBAL float *tval = global_view(J.tval);
+    BHT_GLOBAL float *out = global_view(EPOCH ? J.fattr : pos_out);
+    BHT_GLOBAL const uint32_t *blk_seg = global_view(J.blk_seg), *seg_tile = global_view(J.seg_tile);
+    BHT_GLOBAL const uint32_t *seg_g0 = global_view(J.seg_g0), *grp_e0 = global_view(J.grp_e0);
+    // stop_lying fires after epoch == stop_lying_epoch (lib.rs:1168-1171): the first epoch to
+    // read the rescaled values rescales them in place as it reads
+    const bool stop_now = EPOCH && J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
+    const float inv_exag = J.inv_exag;
+
+    const uint32_t row0 = (uint32_t)bc.lb * R;
+    const uint32_t nrows = row0 < n ? (n - row0 < R ? n - row0 : R) : 0u;
+    stage_points<D>(y, row0, nrows, yrow);
+    for (uint32_t k = threadIdx.x; k < nrows * D; k += T_THREADS) lds_view(acc)[k] = 0.0f;
+    const uint32_t s_begin = blk_seg[bc.lb], s_end = blk_seg[bc.lb + 1];
+    for (uint32_t s = s_begin; s < s_end; s++) {
+        const uint32_t t = seg_tile[s];
+        const uint32_t first = t * W, count = n - first < W ? n - first : W;
+        __syncthreads();  // the previous tile's reads are done
+        stage_points<D>(y, first, count, win);
+        __syncthreads();
+        const uint32_t g_end = seg_g0[s + 1];
+        for (uint32_t g = seg_g0[s] + wid; g < g_end; g += T_WAVES) {
+            const uint32_t e_begin = grp_e0[g], e_end = grp_e0[g + 1];
+            for (uint32_t base = e_begin & ~7u; base < e_end; base += GROUP_SPAN) {
+                const uint32_t e0 = base + 8u * lane;
+                uint32_t key[8];
+                float pv[8];
+                if (e0 + 8u <= nnz) {
+                    const u4v k0 = __builtin_nontemporal_load((BHT_GLOBAL const u4v *)(tkey + e0));
+                    const u4v k1 =
+                        __builtin_nontemporal_load((BHT_GLOBAL const u4v *)(tkey + e0 + 4));
+                    const f4v v0 = __builtin_nontemporal_load((BHT_GLOBAL const f4v *)(tval + e0));
+                    const f4v v1 =
+                        __builtin_nontemporal_load((BHT_GLOBAL const f4v *)(tval + e0 + 4));
+                    key[0] = k0.x; key[1] = k0.y; key[2] = k0.z; key[3] = k0.w;
+                    key[4] = k1.x; key[5] = k1.y; key[6] = k1.z; key[7] = k1.w;
+                    pv[0] = v0.x; pv[1] = v0.y; pv[2] = v0.z; pv[3] = v0.w;
+                    pv[4] = v1.x; pv[5] = v1.y; pv[6] = v1.z; pv[7] = v1.w;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const bool in = e0 + u < nnz;
+                        key[u] = in ? tkey[e0 + u] : 0u;
+                        pv[u] = in ? tval[e0 + u] : 0.0f;
+                    }
+                }
+                // edges of this group only; the others get no row
+                uint32_t row[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const bool ok = e0 + u >= e_begin && e0 + u < e_end;
+                    row[u] = ok ? key[u] >> 16 : NO_ROW;
+                    if (!ok) pv[u] = 0.0f;
+                }
+                if (stop_now) {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        pv[u] = pv[u] * inv_exag;
+                        if (row[u] != NO_ROW) tval[e0 + u] = pv[u];
+                    }
+                }
+                float f0[8], f1[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const float2 yi = lds_pos<D>(yrow, row[u] != NO_ROW ? row[u] : 0u);
+                    const float2 yj = lds_pos<D>(win, row[u] != NO_ROW ? key[u] & 0xffffu : 0u);
+                    const float d0 = yi.x - yj.x;
+                    const float d1 = D == 2 ? yi.y - yj.y : 0.0f;
+                    float dist = 0.0f;
+                    dist += d0 * d0;
+                    if (D == 2) dist += d1 * d1;
+                    const float f = pv[u] / (dist + 1.0f);
+                    f0[u] = f * d0;
+                    f1[u] = f * d1;
+                }
+                // runs of equal rows within the lane: the head run (from edge 0), the tail run
+                // (to edge 7), and rows that start and end inside the lane (added at once)
+                uint32_t head_row = row[0], cur_row = row[0];
+                float h0 = 0.0f, h1 = 0.0f, c0 = f0[0], c1 = f1[0];
+                bool multi = false;
+#pragma unroll
+                for (int u = 1; u < 8; u++) {
+                    if (row[u] == cur_row) {
+                        c0 += f0[u];
+                        c1 += f1[u];
+                    } else {
+                        if (!multi) {
+                            h0 = c0;
+                            h1 = c1;
+                            multi = true;
+                        } else if (cur_row != NO_ROW) {
+                            acc_add<D>(acc, cur_row, c0, c1);
+                        }
+                        cur_row = row[u];
+                        c0 = f0[u];
+                        c1 = f1[u];
+                    }
+                }
+                const uint32_t tail_row = cur_row;
+                // segmented inclusive scan of the tail sums over lanes (a segment = equal rows)
+                const uint32_t prev_tail = __shfl_up(tail_row, 1);
+                bool flag = lane == 0 || prev_tail != tail_row;
+                float s0 = c0, s1 = c1;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const float u0 = __shfl_up(s0, o), u1 = __shfl_up(s1, o);
+                    const bool uf = __shfl_up((int)flag, o) != 0;
+                    if (lane >= o) {
+                        if (!flag) {
+                            s0 = u0 + s0;
+                            s1 = u1 + s1;
+                        }
+                        flag = flag || uf;
+                    }
+                }
+                // the head row of a multi-run lane ends here: add what the lanes before hold
+                const float p0 = __shfl_up(s0, 1), p1 = __shfl_up(s1, 1);
+                if (multi && head_row != NO_ROW) {
+                    const bool joins = lane > 0 && prev_tail == head_row;
+                    acc_add<D>(acc, head_row, joins ? p0 + h0 : h0, joins ? p1 + h1 : h1);
+                }
+                // the tail row ends here unless the next lane's first edge continues it
+                const uint32_t next_head = __shfl_down(head_row, 1);
+                if (tail_row != NO_ROW && (lane == 63 || next_head != tail_row))
+                    acc_add<D>(acc, tail_row, s0, s1);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nrows * D; k += T_THREADS)
+        out[(size_t)row0 * D + k] = lds_view(acc)[k];
+}
+
+}  // namespace
+
+uint32_t tiled_rows_per_block(int n) {
+    // about 512 blocks for a large job, at most TILE_ROWS_MAX rows each, a power of two
+    uint32_t r = 64;
+    while (r < TILE_ROWS_MAX && (uint64_t)r * 512 < (uint64_t)(n > 0 ? n : 1)) r <<= 1;
+    return r;
+}
+
+uint32_t tiled_tile_points(int dim) { return (uint32_t)(WIN_FLOATS / dim); }
+
+int tiled_blocks(int n) {
+    const uint32_t r = tiled_rows_per_block(n);
+    return (int)ceil_div((uint64_t)(n > 0 ? n : 1), r);
+}
+
+TiledSizes tiled_sizes(int n, int dim, uint64_t nnz) {
+    TiledSizes z;
+    const uint64_t nb = (uint64_t)tiled_blocks(n);
+    const uint64_t nt = ceil_div((uint64_t)(n > 0 ? n : 1), tiled_tile_points(dim));
+    z.max_segs = std::min<uint64_t>(nnz, nb * nt) + 1;
+    z.max_groups = z.max_segs + nnz / GROUP_SPAN + 1;
+    z.persistent = nnz * 8 + (nb + 1) * 4 + z.max_segs * 12 + (z.max_groups + 1) * 4;
+    const uint64_t nn = std::max<uint64_t>(nnz, 1);
+    z.temp = nn * 8 * 2 + nn * 4 * 2 + radix_sort_temp_bytes(nn) + nn * 4 * 4 +
+             scan_temp_bytes(nn) + z.max_segs * 8 + 64 + 12 * 256;
+    return z;
+}
+
+void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, int n, int dim,
+                 uint64_t nnz, const TiledOut &o, void *temp, hipStream_t s) {
+    const TiledSizes z = tiled_sizes(n, dim, nnz);
+    const uint64_t nn = std::max<uint64_t>(nnz, 1);
+    TileGeom g;
+    g.rows_per_block = tiled_rows_per_block(n);
+    g.tile_points = tiled_tile_points(dim);
+    const uint32_t nblocks = (uint32_t)tiled_blocks(n);
+    const uint64_t ntiles = ceil_div((uint64_t)(n > 0 ? n : 1), g.tile_points);
+    g.row_bits = bits_for(g.rows_per_block);
+    g.tile_bits = std::max(1, bits_for(ntiles));
+    const int key_bits = g.row_bits + g.tile_bits + bits_for(nblocks);
+    if (key_bits > 64) throw Error(ERR_INVALID, "tiled layout: too many points");
+    // temp carve-up (256-byte aligned pieces)
+    char *p = (char *)temp;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += (bytes + 255) & ~(size_t)255;
+        return (void *)q;
+    };
+    uint64_t *keys = (uint64_t *)take(nn * 8), *keys_alt = (uint64_t *)take(nn * 8);
+    uint32_t *idx = (uint32_t *)take(nn * 4), *idx_alt = (uint32_t *)take(nn * 4);
+    void *sort_temp = take(radix_sort_temp_bytes(nn));
+    uint32_t *seg_flag = (uint32_t *)take(nn * 4), *seg_pos = (uint32_t *)take(nn * 4);
+    uint32_t *run_flag = (uint32_t *)take(nn * 4), *run_pos = (uint32_t *)take(nn * 4);
+    void *scan_temp = take(scan_temp_bytes(nn));
+    uint32_t *seg_blk = (uint32_t *)take(z.max_segs * 4);
+    uint32_t *seg_ngrp = (uint32_t *)take(z.max_segs * 4);
+    uint32_t *counts = (uint32_t *)take(64);  // nsegs, nruns
+    if (nnz == 0) {
+        BHT_HIP(hipMemsetAsync(o.blk_seg, 0, (nblocks + 1) * 4, s));
+        BHT_HIP(hipMemsetAsync(o.seg_g0, 0, 4, s));
+        BHT_HIP(hipMemsetAsync(o.grp_e0, 0, 4, s));
+        return;
+    }
+    const int nb_rows = (int)std::min<uint64_t>(ceil_div((uint64_t)n, 4), 8192);
+    hipLaunchKernelGGL(k_tile_keys, dim3(nb_rows), dim3(256), 0, s, rows, cols, n, g, keys, idx);
+    BHT_LAUNCH_CHECK();
+    const bool alt =
+        radix_sort_pairs<uint64_t>(keys, idx, keys_alt, idx_alt, nnz, 0, key_bits, sort_temp, s);
+    const uint64_t *sk = alt ? keys_alt : keys;
+    const uint32_t *si = alt ? idx_alt : idx;
+    const int nb_e = (int)std::min<uint64_t>(ceil_div(nnz, 256), 16384);
+    hipLaunchKernelGGL(k_tile_pack, dim3(nb_e), dim3(256), 0, s, sk, si, cols, vals, nnz, g,
+                       o.tkey, o.tval, seg_flag, run_flag);
+    BHT_LAUNCH_CHECK();
+    scan_exclusive<uint32_t>(seg_flag, seg_pos, nnz, scan_temp, counts, s);
+    scan_exclusive<uint32_t>(run_flag, run_pos, nnz, scan_temp, counts + 1, s);
+    hipLaunchKernelGGL(k_tile_segments, dim3(nb_e), dim3(256), 0, s, sk, seg_flag, seg_pos,
+                       run_flag, run_pos, nnz, g, o.seg_e0, o.seg_tile, seg_blk, idx /*runs*/);
+    BHT_LAUNCH_CHECK();
+    const uint32_t *runs = idx;  // the unsorted index array is free again
+    const int nb_s = (int)std::min<uint64_t>(ceil_div(z.max_segs, 256), 4096);
+    hipLaunchKernelGGL(k_tile_seg_groups, dim3(nb_s), dim3(256), 0, s, o.seg_e0, counts, nnz,
+                       (uint32_t)z.max_segs, seg_ngrp);
+    BHT_LAUNCH_CHECK();
+    // seg_g0 = exclusive scan of the group counts over the segments and the sentinel
+    scan_exclusive<uint32_t>(seg_ngrp, o.seg_g0, z.max_segs, scan_temp, counts + 2, s);
+    const int nb_g = (int)std::min<uint64_t>(ceil_div(z.max_groups + 1, 256), 8192);
+    hipLaunchKernelGGL(k_tile_index, dim3(nb_g), dim3(256), 0, s, seg_blk, o.seg_e0, o.seg_g0,
+                       runs, counts, nblocks, nnz, o.blk_seg, o.grp_e0);
+    BHT_LAUNCH_CHECK();
+}
+
+void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s) {
+    const dim3 g(L.nb[F_ATT]);
+    if (L.dim == 2) {
+        if (epoch)
+            hipLaunchKernelGGL((k_attract_tiled<2, true>), g, dim3(T_THREADS), 0, s, L.jobs,
+                               L.map[F_ATT], nullptr);
+        else
+            hipLaunchKernelGGL((k_attract_tiled<2, false>), g, dim3(T_THREADS), 0, s, L.jobs,
+                               L.map[F_ATT], pos);
+    } else {
+        if (epoch)
+            hipLaunchKernelGGL((k_attract_tiled<1, true>), g, dim3(T_THREADS), 0, s, L.jobs,
+                               L.map[F_ATT], nullptr);
+        else
+            hipLaunchKernelGGL((k_attract_tiled<1, false>), g, dim3(T_THREADS), 0, s, L.jobs,
+                               L.map[F_ATT], pos);
+    }
+    BHT_LAUNCH_CHECK();
+}
+
+}  // namespace bht

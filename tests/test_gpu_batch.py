@@ -186,3 +186,36 @@ def test_footprint_covers_allocation(bt):
         assert used <= bound
         b.run()
         b.close()
+
+
+def test_mixed_batch_of_varied_jobs_equals_alone(bt):
+    # Many jobs of both methods (two launch groups) with uneven sizes, some above the renumbering
+    # threshold, stepped with plain launches and with graph replay; sampled members are compared
+    # bit for bit with their fits alone.
+    specs = [datagen.config_c4_job(j) for j in range(24)]
+    ns = [x.shape[0] for x, _ in specs]
+    sample = sorted({j for j in range(24) if ns[j] >= 32768} | {0, 1, 5, 6})
+    ctx = bt.Context(0)
+
+    def make(j):
+        x, _ = specs[j]
+        t = bt.TSNE(x, ctx=ctx).perplexity(30.0).epochs(300)
+        return t.initial_embedding(datagen.seed_embedding(x.shape[0], 2, j))
+
+    alone = {}
+    for j in sample:
+        t = make(j)
+        if specs[j][1] == "fitsne":
+            t.fit_sne()
+        else:
+            t.barnes_hut(0.5)
+        alone[j] = t.embedding()
+    for graph in (False, True):
+        b = bt.Batch(ctx, graph=graph)
+        jobs = [make(j) for j in range(24)]
+        for j, t in enumerate(jobs):
+            b.add(t, specs[j][1], 0.5)
+        b.run()
+        for j in sample:
+            assert np.array_equal(jobs[j].embedding(), alone[j]), (graph, j, ns[j])
+        b.close()
