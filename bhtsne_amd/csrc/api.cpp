@@ -793,7 +793,7 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     const bool fit = j->method == FIT_FITSNE;
     // Small jobs are bound by launch latency, which the fork's events would only add to; they
     // run the same kernels in order on one stream.
-    const bool fork = j->n >= FORK_MIN_POINTS;
+    const bool fork = j->n >= FORK_MIN_POINTS && !(j->params.flags & BHT_FLAG_SERIAL_EPOCH);
     hipStream_t rs = fork ? j->side : j->stream;
     for (uint64_t e = 0; e < count; e++) {
         // recentre (+ FIt-SNE grid plan), then the repulsion and the attraction side by side

@@ -78,6 +78,9 @@ typedef struct bht_params {
  * gathers stay in cache (results are unchanged up to summation order; outputs are always in
  * sample order).  This flag keeps the input order. */
 #define BHT_FLAG_NO_REORDER 1u
+/* Jobs of 32768 points or more run each epoch's repulsion on a second stream beside the
+ * attraction.  This flag keeps every epoch in order on the job's stream (same results). */
+#define BHT_FLAG_SERIAL_EPOCH 2u
 
 void bht_params_default(bht_params *p);
 
