@@ -32,7 +32,10 @@ namespace {
 
 constexpr int T_THREADS = 512;
 constexpr int T_WAVES = T_THREADS / 64;
-constexpr int TILE_ROWS_MAX = 2048;
+// At most 1024 rows per block: blocks that short retire often enough that the repulsion, which runs
+// beside the attraction, gets CUs (2048-row blocks: 1432 epochs/s at 1M points, 1024: 1517,
+// 512: 1473).
+constexpr int TILE_ROWS_MAX = 1024;
 constexpr int GROUP_SPAN = 512;  // edges per wave pass: 64 lanes x 8
 // LDS per block: window + the block's row positions + the accumulators, under 80,064 bytes so
 // that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran at half speed)
