@@ -15,8 +15,9 @@
 //   seg_e0[s]          first edge of segment s (sentinel = nnz)
 //   seg_g0[s]          first group of segment s (sentinel = ngroups)
 //   grp_e0[g]          first edge of group g (sentinel = nnz)
-// A group is the rows of a segment whose first edge lies in one GROUP_SPAN-edge window of it, so
-// a row's edges in a segment belong to exactly one group, which one wave processes alone.
+// A group is a run of whole rows of a segment, as many as fit one wave pass (GROUP_SPAN edges from
+// the group's first edge rounded down to a multiple of 8), or a single row that is longer.  A
+// row's edges in a segment belong to exactly one group, which one wave processes alone.
 //
 // Within a group a wave takes GROUP_SPAN edges per pass, 8 consecutive edges per lane.  Each lane
 // sums its edges by row (runs of equal keys), rows complete inside the lane are added to the
@@ -122,11 +123,32 @@ __global__ void k_tile_segments(const uint64_t *keys, const uint32_t *seg_flag,
     }
 }
 
-// Groups per segment (windows of GROUP_SPAN edges), zero past the last segment (the counts are
-// scanned over the whole table), and the sentinel.
-__global__ void k_tile_seg_groups(uint32_t *seg_e0, const uint32_t *nsegs_p, uint64_t nnz,
-                                  uint32_t max_segs, uint32_t *seg_ngrp) {
-    const uint32_t nsegs = *nsegs_p;
+// Splits the row runs of the segment [begin, end) into groups, greedily: a group takes whole runs
+// while they fit one GROUP_SPAN-edge pass from its first edge rounded down to a multiple of 8.
+// Calls emit(i, first edge) for every group i and returns the group count.  runs[] holds the first
+// edge of every (segment, row) run in order; a segment starts with a run.
+template <typename F>
+__device__ uint32_t split_groups(const uint32_t *runs, uint32_t nruns, uint32_t begin,
+                                 uint32_t end, F emit) {
+    uint32_t r = lower_bound_u32(runs, nruns, begin);
+    uint32_t start = begin, ng = 1;
+    emit(0u, start);
+    for (r++; r < nruns && runs[r] < end; r++) {
+        const uint32_t run_end = r + 1 < nruns && runs[r + 1] < end ? runs[r + 1] : end;
+        if (run_end - (start & ~7u) > (uint32_t)GROUP_SPAN) {  // run r opens the next group
+            start = runs[r];
+            emit(ng, start);
+            ng++;
+        }
+    }
+    return ng;
+}
+
+// Groups per segment, zero past the last segment (the counts are scanned over the whole table),
+// and the segment sentinel.
+__global__ void k_tile_seg_groups(uint32_t *seg_e0, const uint32_t *runs, const uint32_t *counts,
+                                  uint64_t nnz, uint32_t max_segs, uint32_t *seg_ngrp) {
+    const uint32_t nsegs = counts[0], nruns = counts[1];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < max_segs; s += stride) {
         if (s >= nsegs) {
@@ -135,12 +157,12 @@ __global__ void k_tile_seg_groups(uint32_t *seg_e0, const uint32_t *nsegs_p, uin
             continue;
         }
         const uint32_t end = s + 1 < nsegs ? seg_e0[s + 1] : (uint32_t)nnz;
-        seg_ngrp[s] = (end - seg_e0[s] + GROUP_SPAN - 1) / GROUP_SPAN;
+        seg_ngrp[s] = split_groups(runs, nruns, seg_e0[s], end, [](uint32_t, uint32_t) {});
     }
 }
 
-// blk_seg[b] = first segment of block b or later; group starts: the first row run at or after
-// each window start, clipped to its segment.
+// blk_seg[b] = first segment of block b or later; grp_e0 = the group starts of every segment and
+// the sentinel.
 __global__ void k_tile_index(const uint32_t *seg_blk, const uint32_t *seg_e0,
                              const uint32_t *seg_g0, const uint32_t *runs, const uint32_t *counts,
                              uint32_t nblocks, uint64_t nnz, uint32_t *blk_seg,
@@ -150,22 +172,11 @@ __global__ void k_tile_index(const uint32_t *seg_blk, const uint32_t *seg_e0,
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t b = tid; b <= nblocks; b += stride)
         blk_seg[b] = b == nblocks ? nsegs : lower_bound_u32(seg_blk, nsegs, b);
-    for (uint32_t q = tid; q <= ngroups; q += stride) {
-        if (q == ngroups) {
-            grp_e0[q] = (uint32_t)nnz;
-            continue;
-        }
-        // segment of group q: the last s with seg_g0[s] <= q
-        uint32_t lo = 0, hi = nsegs;
-        while (hi - lo > 1) {
-            const uint32_t mid = lo + (hi - lo) / 2;
-            if (seg_g0[mid] <= q) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t s = lo;
-        const uint32_t w = seg_e0[s] + (q - seg_g0[s]) * GROUP_SPAN, end = seg_e0[s + 1];
-        const uint32_t k = lower_bound_u32(runs, nruns, w);
-        grp_e0[q] = k < nruns && runs[k] < end ? runs[k] : end;
+    if (tid == 0) grp_e0[ngroups] = (uint32_t)nnz;
+    for (uint32_t s = tid; s < nsegs; s += stride) {
+        uint32_t *out = grp_e0 + seg_g0[s];
+        split_groups(runs, nruns, seg_e0[s], seg_e0[s + 1],
+                     [out](uint32_t i, uint32_t e) { out[i] = e; });
     }
 }
 
@@ -378,7 +389,8 @@ TiledSizes tiled_sizes(int n, int dim, uint64_t nnz) {
     const uint64_t nb = (uint64_t)tiled_blocks(n);
     const uint64_t nt = ceil_div((uint64_t)(n > 0 ? n : 1), tiled_tile_points(dim));
     z.max_segs = std::min<uint64_t>(nnz, nb * nt) + 1;
-    z.max_groups = z.max_segs + nnz / GROUP_SPAN + 1;
+    // two consecutive groups of a segment hold more than GROUP_SPAN - 8 edges together
+    z.max_groups = 2 * z.max_segs + 2 * (nnz / (GROUP_SPAN - 8)) + 2;
     z.persistent = nnz * 8 + (nb + 1) * 4 + z.max_segs * 12 + (z.max_groups + 1) * 4;
     const uint64_t nn = std::max<uint64_t>(nnz, 1);
     z.temp = nn * 8 * 2 + nn * 4 * 2 + radix_sort_temp_bytes(nn) + nn * 4 * 4 +
@@ -439,12 +451,12 @@ void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, 
     BHT_LAUNCH_CHECK();
     const uint32_t *runs = idx;  // the unsorted index array is free again
     const int nb_s = (int)std::min<uint64_t>(ceil_div(z.max_segs, 256), 4096);
-    hipLaunchKernelGGL(k_tile_seg_groups, dim3(nb_s), dim3(256), 0, s, o.seg_e0, counts, nnz,
-                       (uint32_t)z.max_segs, seg_ngrp);
+    hipLaunchKernelGGL(k_tile_seg_groups, dim3(nb_s), dim3(256), 0, s, o.seg_e0, runs, counts,
+                       nnz, (uint32_t)z.max_segs, seg_ngrp);
     BHT_LAUNCH_CHECK();
     // seg_g0 = exclusive scan of the group counts over the segments and the sentinel
     scan_exclusive<uint32_t>(seg_ngrp, o.seg_g0, z.max_segs, scan_temp, counts + 2, s);
-    const int nb_g = (int)std::min<uint64_t>(ceil_div(z.max_groups + 1, 256), 8192);
+    const int nb_g = (int)std::min<uint64_t>(ceil_div(z.max_segs + nblocks + 1, 256), 8192);
     hipLaunchKernelGGL(k_tile_index, dim3(nb_g), dim3(256), 0, s, seg_blk, o.seg_e0, o.seg_g0,
                        runs, counts, nblocks, nnz, o.blk_seg, o.grp_e0);
     BHT_LAUNCH_CHECK();
