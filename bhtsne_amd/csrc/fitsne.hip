@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const
 }
 
 // One wave per chunk; the chunk where a spanning run starts (its owner) finds the run's last chunk
-// by binary search of the sorted keys and sums the chunk partials with the lanes (strided, then a
+// by a 64-way search of the sorted keys and sums the chunk partials with the lanes (strided, then a
 // fixed butterfly), so a run of any length costs O(length / 64 / 64) steps.
 template <int D>
 __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const JobDev *jobs,
@@ -508,12 +508,17 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const JobDev *jobs,
         const int first_s = c * 64;
         const bool whole = keys[first_s] == key;
         if (whole && first_s > 0 && keys[first_s - 1] == key) continue;  // not the run's owner
-        // first sorted position past the run
+        // first sorted position past the run, in [lo, hi]: a 64-way search, each lane probing one
+        // point of the range per step (keys[lo - 1] == key; keys[hi] > key or hi == n)
         int lo = last_s + 1, hi = n;
         while (lo < hi) {
-            int mid = lo + (hi - lo) / 2;
-            if (keys[mid] <= key) lo = mid + 1;
-            else hi = mid;
+            const int step = (hi - lo + 63) / 64;
+            const int probe = lo + lane * step;
+            const bool le = probe < hi && keys[probe] <= key;
+            const int c_le = __popcll(__ballot(le));  // the probes at or below the key lead
+            if (c_le == 0) break;
+            hi = min(hi, lo + c_le * step);
+            lo = lo + (c_le - 1) * step + 1;
         }
         const int c_end = (lo - 1) / 64;  // chunk holding the run's last point
         float tot[NV];

@@ -127,15 +127,16 @@ static_assert(RS_TILE == RS_TILE_KEYS, "segment alignment must match the tile si
 
 // Where a tile's keys live and where its digit counts go.  Unsegmented: tile t covers
 // [t * RS_TILE, ...) of n keys and its count of digit d sits at d * ntiles + t.  Segmented: the
-// tile belongs to one segment, covers a slice of that segment's range, and the counts of a
-// segment form a block (digit-major) of their own, in segment order, so that one exclusive scan
-// over all counts gives every segment its dense offsets; the scatter moves them back to the
-// segment's own (padded) range.
+// tile belongs to one segment and covers a slice of that segment's range; the counts of a segment
+// form a block (digit-major) of their own, and the segment's digit totals sit at the block's start
+// in a table of the same shape.  Each digit's counts are scanned within their segment, and the
+// scatter adds the totals of the smaller digits and the segment's base.
 struct TileCtx {
     uint64_t first, end;   // key range of the tile, clipped to the segment
     size_t h0;             // count index of digit 0
     unsigned hstride;      // distance between consecutive digits
-    uint64_t out_adj;      // added to the scanned offset: segment base - keys before it
+    size_t t0;             // index of the segment's digit-0 total
+    uint64_t out_base;     // first output position of the segment
 };
 
 template <int BINS>
@@ -147,7 +148,8 @@ __device__ __forceinline__ TileCtx tile_ctx(uint64_t n, unsigned ntiles, const S
         c.end = n;
         c.h0 = blockIdx.x;
         c.hstride = ntiles;
-        c.out_adj = 0;
+        c.t0 = 0;
+        c.out_base = 0;
         return c;
     }
     const SortSeg g = segs[tile_seg[blockIdx.x]];
@@ -156,7 +158,8 @@ __device__ __forceinline__ TileCtx tile_ctx(uint64_t n, unsigned ntiles, const S
     c.end = g.base + g.n;
     c.h0 = (size_t)BINS * g.tile0 + lt;
     c.hstride = g.ntiles;
-    c.out_adj = g.base - g.dense0;
+    c.t0 = (size_t)BINS * g.tile0;
+    c.out_base = g.base;
     return c;
 }
 
@@ -180,6 +183,78 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K *keys, uint64_t 
         hist[tc.h0 + (size_t)b * tc.hstride] = h[b];
 }
 
+// In place: each digit's tile counts become their exclusive prefix within the segment (tile
+// order), and the digit's total goes to totals.  One wave per (segment, digit).
+template <int BITS>
+__global__ void __launch_bounds__(256) k_rs_digit_scan(uint32_t *hist, uint32_t *totals,
+                                                       unsigned ntiles, const SortSeg *segs,
+                                                       unsigned nsegs) {
+    constexpr unsigned BINS = 1u << BITS;
+    const unsigned lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+    if (wave >= (uint64_t)nsegs * BINS) return;
+    const unsigned g = (unsigned)(wave / BINS), d = (unsigned)(wave % BINS);
+    size_t h0, t0;
+    unsigned cnt;
+    if (segs) {
+        const SortSeg sg = segs[g];
+        h0 = (size_t)BINS * sg.tile0 + (size_t)d * sg.ntiles;
+        t0 = (size_t)BINS * sg.tile0;
+        cnt = sg.ntiles;
+    } else {
+        h0 = (size_t)d * ntiles;
+        t0 = 0;
+        cnt = ntiles;
+    }
+    if (cnt == 0) return;  // an empty segment: no tile reads its totals
+    // each lane scans a contiguous run of the counts; the lanes' sums are scanned across the wave
+    const unsigned per = (cnt + 63) / 64, b = lane * per, e = b + per < cnt ? b + per : cnt;
+    uint32_t sum = 0;
+    for (unsigned t = b; t < e; t++) sum += hist[h0 + t];
+    uint32_t incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += u;
+    }
+    uint32_t run = incl - sum;
+    for (unsigned t = b; t < e; t++) {
+        const uint32_t v = hist[h0 + t];
+        hist[h0 + t] = run;
+        run += v;
+    }
+    if (lane == 63) totals[t0 + d] = incl;
+}
+
+// Exclusive prefix of the BINS digit totals of a segment into base[] (all threads of the block).
+template <int BINS>
+__device__ void digit_prefix(const uint32_t *totals, uint32_t *base, uint32_t *wsum) {
+    constexpr int PER = BINS / RS_THREADS;
+    static_assert(PER >= 1 && BINS % RS_THREADS == 0, "digits per thread");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t v[PER], sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        v[i] = totals[threadIdx.x * PER + i];
+        sum += v[i];
+    }
+    uint32_t incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += u;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (int q = 0; q < w; q++) run += wsum[q];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        base[threadIdx.x * PER + i] = run;
+        run += v[i];
+    }
+}
+
 // Stable scatter of one tile: rounds of 256 keys; within a round a key's rank among the equal
 // digits of its wave comes from one ballot per digit bit, and the waves before it add their
 // counts.
@@ -188,15 +263,20 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *keys, const 
                                                            K *keys_out, uint32_t *vals_out,
                                                            uint64_t n, int shift,
                                                            const uint32_t *offs,
+                                                           const uint32_t *totals,
                                                            unsigned ntiles, const SortSeg *segs,
                                                            const uint32_t *tile_seg) {
     constexpr int BINS = 1 << BITS;
     __shared__ uint32_t base[BINS];
     __shared__ uint32_t wcnt[RS_THREADS / 64][BINS];
+    __shared__ uint32_t wsum[RS_THREADS / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const TileCtx tc = tile_ctx<BINS>(n, ntiles, segs, tile_seg);
+    // base[d] = segment base + keys of smaller digits + keys of digit d in earlier tiles
+    digit_prefix<BINS>(totals + tc.t0, base, wsum);
+    __syncthreads();
     for (int b = threadIdx.x; b < BINS; b += RS_THREADS)
-        base[b] = offs[tc.h0 + (size_t)b * tc.hstride] + (uint32_t)tc.out_adj;
+        base[b] += offs[tc.h0 + (size_t)b * tc.hstride] + (uint32_t)tc.out_base;
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint64_t tile0 = tc.first;
     n = tc.end;
@@ -240,7 +320,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *keys, const 
 static size_t temp_bytes_bins(uint64_t n, int bins) {
     uint64_t ntiles = ceil_div(n, RS_TILE);
     uint64_t hist = ntiles * (uint64_t)bins;
-    return (size_t)hist * 2 * sizeof(uint32_t) + scan_temp_bytes(hist) + 256;
+    return (size_t)hist * 2 * sizeof(uint32_t) + 256;  // tile counts, digit totals
 }
 
 size_t radix_sort_temp_bytes(uint64_t n) { return temp_bytes_bins(n, 1 << 8); }
@@ -249,23 +329,26 @@ size_t group_sort_temp_bytes(uint64_t n) { return temp_bytes_bins(n, 1 << GROUP_
 // Sorts (keys, vals) by bits [begin_bit, end_bit) of the key, stable, BITS bits per pass.
 // Ping-pongs between the two buffer pairs; returns true when the result is in (keys_alt,
 // vals_alt).
+// Three launches per pass: tile counts, their per-digit scan, the scatter.
 template <typename K, int BITS>
 static bool radix_passes(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
-                         unsigned ntiles, const SortSeg *segs, const uint32_t *tile_seg,
-                         int begin_bit, int end_bit, void *temp, hipStream_t s) {
-    uint64_t nh = (uint64_t)ntiles << BITS;
+                         unsigned ntiles, const SortSeg *segs, unsigned nsegs,
+                         const uint32_t *tile_seg, int begin_bit, int end_bit, void *temp,
+                         hipStream_t s) {
+    const uint64_t nh = (uint64_t)ntiles << BITS;
     uint32_t *hist = (uint32_t *)temp;
-    uint32_t *offs = hist + nh;
-    void *stemp = (void *)(((uintptr_t)(offs + nh) + 255) & ~(uintptr_t)255);
+    uint32_t *totals = hist + nh;
+    const unsigned scan_blocks = (unsigned)ceil_div((uint64_t)nsegs << BITS, 4);
     K *ki = keys, *ko = keys_alt;
     uint32_t *vi = vals, *vo = vals_alt;
     bool alt = false;
     for (int shift = begin_bit; shift < end_bit; shift += BITS) {
         hipLaunchKernelGGL((k_rs_hist<K, BITS>), dim3(ntiles), dim3(RS_THREADS), 0, s, ki, n,
                            shift, hist, ntiles, segs, tile_seg);
-        scan_exclusive<uint32_t>(hist, offs, nh, stemp, nullptr, s);
+        hipLaunchKernelGGL((k_rs_digit_scan<BITS>), dim3(scan_blocks), dim3(256), 0, s, hist,
+                           totals, ntiles, segs, nsegs);
         hipLaunchKernelGGL((k_rs_scatter<K, BITS>), dim3(ntiles), dim3(RS_THREADS), 0, s, ki, vi,
-                           ko, vo, n, shift, offs, ntiles, segs, tile_seg);
+                           ko, vo, n, shift, hist, totals, ntiles, segs, tile_seg);
         BHT_LAUNCH_CHECK();
         K *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;
@@ -279,7 +362,7 @@ bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, 
                       int begin_bit, int end_bit, void *temp, hipStream_t s) {
     if (n == 0) return false;
     return radix_passes<K, 8>(keys, vals, keys_alt, vals_alt, n, ceil_div(n, RS_TILE), nullptr,
-                              nullptr, begin_bit, end_bit, temp, s);
+                              1, nullptr, begin_bit, end_bit, temp, s);
 }
 
 // The per-epoch sorts: GROUP_SORT_BITS-bit digits; one segment (a job alone) or one segment per
@@ -291,11 +374,12 @@ bool sort_group(const LaunchSet &L, int end_bit, hipStream_t s) {
         if (L.sort_tiles == 0) return false;
         return radix_passes<K, GROUP_SORT_BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt,
                                                 (uint64_t)L.sort_tiles * RS_TILE, L.sort_tiles,
-                                                L.segs, L.tile_seg, 0, end_bit, L.sort_temp, s);
+                                                L.segs, (unsigned)L.njobs, L.tile_seg, 0, end_bit,
+                                                L.sort_temp, s);
     }
     if (L.sort_n == 0) return false;
     return radix_passes<K, GROUP_SORT_BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt,
-                                            L.sort_n, ceil_div(L.sort_n, RS_TILE), nullptr,
+                                            L.sort_n, ceil_div(L.sort_n, RS_TILE), nullptr, 1,
                                             nullptr, 0, end_bit, L.sort_temp, s);
 }
 
