@@ -263,12 +263,11 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                 uint32_t key[8];
                 float pv[8];
                 if (e0 + 8u <= nnz) {
-                    const u4v k0 = __builtin_nontemporal_load((BHT_GLOBAL const u4v *)(tkey + e0));
-                    const u4v k1 =
-                        __builtin_nontemporal_load((BHT_GLOBAL const u4v *)(tkey + e0 + 4));
-                    const f4v v0 = __builtin_nontemporal_load((BHT_GLOBAL const f4v *)(tval + e0));
-                    const f4v v1 =
-                        __builtin_nontemporal_load((BHT_GLOBAL const f4v *)(tval + e0 + 4));
+                    // plain loads: nontemporal ones measured 3% slower over the whole epoch
+                    const u4v k0 = *(BHT_GLOBAL const u4v *)(tkey + e0);
+                    const u4v k1 = *(BHT_GLOBAL const u4v *)(tkey + e0 + 4);
+                    const f4v v0 = *(BHT_GLOBAL const f4v *)(tval + e0);
+                    const f4v v1 = *(BHT_GLOBAL const f4v *)(tval + e0 + 4);
                     key[0] = k0.x; key[1] = k0.y; key[2] = k0.z; key[3] = k0.w;
                     key[4] = k1.x; key[5] = k1.y; key[6] = k1.z; key[7] = k1.w;
                     pv[0] = v0.x; pv[1] = v0.y; pv[2] = v0.z; pv[3] = v0.w;
