@@ -1,12 +1,11 @@
 // Spatial renumbering of the points of a job.
 //
 // The attraction gathers y[column] for every edge; with the points in input order those gathers
-// land on random cache lines of an array larger than an XCD's L2.  Every so often the job
-// renumbers its points in the order of the current box (or Morton) sort, i.e. spatially, so that a
-// row's neighbours sit at nearby indices.  Per-point state is permuted, CSR rows are moved and
-// their columns relabelled with the edge order inside every row unchanged, so each row's
-// attraction is the same sum as before.  The library keeps the permutation and undoes it at every
-// output.
+// land on random cache lines of an array larger than an XCD's L2.  Before the first epoch a large
+// job renumbers its points in the diffusion order of P (below), so that a row's neighbours sit at
+// nearby indices.  Per-point state is permuted, CSR rows are moved and their columns relabelled
+// with the edge order inside every row unchanged, so each row's attraction is the same sum as
+// before.  The library keeps the permutation and undoes it at every output.
 #include "internal.hpp"
 #include "reorder.hpp"
 
