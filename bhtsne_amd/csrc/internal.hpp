@@ -36,7 +36,8 @@ struct EpochScalars {
     unsigned ctr[8];       // arrival counters of the single-pass reductions
 };
 
-enum CtrSlot { CTR_CENTER = 0, CTR_GATHER = 1, CTR_UPDATE = 2, CTR_PSUM = 3, CTR_KL = 4, CTR_BH = 5 };
+enum CtrSlot { CTR_CENTER = 0, CTR_GATHER = 1, CTR_UPDATE = 2, CTR_PSUM = 3, CTR_KL = 4, CTR_BH = 5,
+               CTR_ATT = 6 };
 
 // Per-length twiddle tables exp(-2 pi i j / L), one per distinct length reachable from
 // boxes = 1..MAX_BOXES (context-wide, built once).

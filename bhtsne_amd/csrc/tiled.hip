@@ -38,6 +38,11 @@ constexpr int T_WAVES = T_THREADS / 64;
 // 512: 1473).
 constexpr int TILE_ROWS_MAX = 1024;
 constexpr int GROUP_SPAN = 512;  // edges per wave pass: 64 lanes x 8
+// A job's epoch launches at most this many attraction blocks, which take row blocks from a work
+// queue: 13/16 of the 512 that fit the device at once (two per CU), so that the repulsion beside
+// it always finds CUs (1M points: 416 blocks 1949 epochs/s, 512: 1872, 352: 1895, one block per
+// row block: 1906).
+constexpr int ATT_QUEUE_BLOCKS = 416;
 // LDS per block: window + the block's row positions + the accumulators, under 80,064 bytes so
 // that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran at half speed)
 constexpr int WIN_FLOATS = 11800;
@@ -225,6 +230,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
     __shared__ __align__(16) float win[WIN_FLOATS];
     __shared__ __align__(16) float yrow[TILE_ROWS_MAX * D];
     __shared__ __align__(16) float acc[TILE_ROWS_MAX * D];
+    __shared__ uint32_t next_lb;
     const BlockCtx bc = block_ctx(map, jobs, F_ATT);
     const JobDev &J = jobs[bc.job];
     const int lane = threadIdx.x & 63;
@@ -244,127 +250,147 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
     const bool stop_now = EPOCH && J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
     const float inv_exag = J.inv_exag;
 
-    const uint32_t row0 = (uint32_t)bc.lb * R;
-    const uint32_t nrows = row0 < n ? (n - row0 < R ? n - row0 : R) : 0u;
-    stage_points<D>(y, row0, nrows, yrow);
-    for (uint32_t k = threadIdx.x; k < nrows * D; k += T_THREADS) lds_view(acc)[k] = 0.0f;
-    const uint32_t s_begin = blk_seg[bc.lb], s_end = blk_seg[bc.lb + 1];
-    for (uint32_t s = s_begin; s < s_end; s++) {
-        const uint32_t t = seg_tile[s];
-        const uint32_t first = t * W, count = n - first < W ? n - first : W;
-        __syncthreads();  // the previous tile's reads are done
-        stage_points<D>(y, first, count, win);
+    // a job's epoch takes row blocks from the work queue (reset by the update kernel); otherwise
+    // (batches, the force export) each block has its own row block
+    const uint32_t nrb = (n + R - 1) / R;
+    const bool queued = EPOCH && map == nullptr;
+    uint32_t lb = (uint32_t)bc.lb;
+    if (queued) {
+        if (threadIdx.x == 0) next_lb = atomicAdd(&J.scal->ctr[CTR_ATT], 1u);
         __syncthreads();
-        const uint32_t g_end = seg_g0[s + 1];
-        for (uint32_t g = seg_g0[s] + wid; g < g_end; g += T_WAVES) {
-            const uint32_t e_begin = grp_e0[g], e_end = grp_e0[g + 1];
-            for (uint32_t base = e_begin & ~7u; base < e_end; base += GROUP_SPAN) {
-                const uint32_t e0 = base + 8u * lane;
-                uint32_t key[8];
-                float pv[8];
-                if (e0 + 8u <= nnz) {
-                    // plain loads: nontemporal ones measured 3% slower over the whole epoch
-                    const u4v k0 = *(BHT_GLOBAL const u4v *)(tkey + e0);
-                    const u4v k1 = *(BHT_GLOBAL const u4v *)(tkey + e0 + 4);
-                    const f4v v0 = *(BHT_GLOBAL const f4v *)(tval + e0);
-                    const f4v v1 = *(BHT_GLOBAL const f4v *)(tval + e0 + 4);
-                    key[0] = k0.x; key[1] = k0.y; key[2] = k0.z; key[3] = k0.w;
-                    key[4] = k1.x; key[5] = k1.y; key[6] = k1.z; key[7] = k1.w;
-                    pv[0] = v0.x; pv[1] = v0.y; pv[2] = v0.z; pv[3] = v0.w;
-                    pv[4] = v1.x; pv[5] = v1.y; pv[6] = v1.z; pv[7] = v1.w;
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const bool in = e0 + u < nnz;
-                        key[u] = in ? tkey[e0 + u] : 0u;
-                        pv[u] = in ? tval[e0 + u] : 0.0f;
-                    }
-                }
-                // edges of this group only; the others get no row
-                uint32_t row[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const bool ok = e0 + u >= e_begin && e0 + u < e_end;
-                    row[u] = ok ? key[u] >> 16 : NO_ROW;
-                    if (!ok) pv[u] = 0.0f;
-                }
-                if (stop_now) {
-#pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        pv[u] = pv[u] * inv_exag;
-                        if (row[u] != NO_ROW) tval[e0 + u] = pv[u];
-                    }
-                }
-                float f0[8], f1[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const float2 yi = lds_pos<D>(yrow, row[u] != NO_ROW ? row[u] : 0u);
-                    const float2 yj = lds_pos<D>(win, row[u] != NO_ROW ? key[u] & 0xffffu : 0u);
-                    const float d0 = yi.x - yj.x;
-                    const float d1 = D == 2 ? yi.y - yj.y : 0.0f;
-                    float dist = 0.0f;
-                    dist += d0 * d0;
-                    if (D == 2) dist += d1 * d1;
-                    const float f = pv[u] / (dist + 1.0f);
-                    f0[u] = f * d0;
-                    f1[u] = f * d1;
-                }
-                // runs of equal rows within the lane: the head run (from edge 0), the tail run
-                // (to edge 7), and rows that start and end inside the lane (added at once)
-                uint32_t head_row = row[0], cur_row = row[0];
-                float h0 = 0.0f, h1 = 0.0f, c0 = f0[0], c1 = f1[0];
-                bool multi = false;
-#pragma unroll
-                for (int u = 1; u < 8; u++) {
-                    if (row[u] == cur_row) {
-                        c0 += f0[u];
-                        c1 += f1[u];
+        lb = next_lb;
+    }
+    while (lb < nrb) {
+        const uint32_t row0 = lb * R;
+        const uint32_t nrows = row0 < n ? (n - row0 < R ? n - row0 : R) : 0u;
+        stage_points<D>(y, row0, nrows, yrow);
+        for (uint32_t k = threadIdx.x; k < nrows * D; k += T_THREADS) lds_view(acc)[k] = 0.0f;
+        const uint32_t s_begin = blk_seg[lb], s_end = blk_seg[lb + 1];
+        for (uint32_t s = s_begin; s < s_end; s++) {
+            const uint32_t t = seg_tile[s];
+            const uint32_t first = t * W, count = n - first < W ? n - first : W;
+            __syncthreads();  // the previous tile's reads are done
+            stage_points<D>(y, first, count, win);
+            __syncthreads();
+            const uint32_t g_end = seg_g0[s + 1];
+            for (uint32_t g = seg_g0[s] + wid; g < g_end; g += T_WAVES) {
+                const uint32_t e_begin = grp_e0[g], e_end = grp_e0[g + 1];
+                for (uint32_t base = e_begin & ~7u; base < e_end; base += GROUP_SPAN) {
+                    const uint32_t e0 = base + 8u * lane;
+                    uint32_t key[8];
+                    float pv[8];
+                    if (e0 + 8u <= nnz) {
+                        // plain loads: nontemporal ones measured 3% slower over the whole epoch
+                        const u4v k0 = *(BHT_GLOBAL const u4v *)(tkey + e0);
+                        const u4v k1 = *(BHT_GLOBAL const u4v *)(tkey + e0 + 4);
+                        const f4v v0 = *(BHT_GLOBAL const f4v *)(tval + e0);
+                        const f4v v1 = *(BHT_GLOBAL const f4v *)(tval + e0 + 4);
+                        key[0] = k0.x; key[1] = k0.y; key[2] = k0.z; key[3] = k0.w;
+                        key[4] = k1.x; key[5] = k1.y; key[6] = k1.z; key[7] = k1.w;
+                        pv[0] = v0.x; pv[1] = v0.y; pv[2] = v0.z; pv[3] = v0.w;
+                        pv[4] = v1.x; pv[5] = v1.y; pv[6] = v1.z; pv[7] = v1.w;
                     } else {
-                        if (!multi) {
-                            h0 = c0;
-                            h1 = c1;
-                            multi = true;
-                        } else if (cur_row != NO_ROW) {
-                            acc_add<D>(acc, cur_row, c0, c1);
+    #pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            const bool in = e0 + u < nnz;
+                            key[u] = in ? tkey[e0 + u] : 0u;
+                            pv[u] = in ? tval[e0 + u] : 0.0f;
                         }
-                        cur_row = row[u];
-                        c0 = f0[u];
-                        c1 = f1[u];
                     }
-                }
-                const uint32_t tail_row = cur_row;
-                // segmented inclusive scan of the tail sums over lanes (a segment = equal rows)
-                const uint32_t prev_tail = __shfl_up(tail_row, 1);
-                bool flag = lane == 0 || prev_tail != tail_row;
-                float s0 = c0, s1 = c1;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const float u0 = __shfl_up(s0, o), u1 = __shfl_up(s1, o);
-                    const bool uf = __shfl_up((int)flag, o) != 0;
-                    if (lane >= o) {
-                        if (!flag) {
-                            s0 = u0 + s0;
-                            s1 = u1 + s1;
+                    // edges of this group only; the others get no row
+                    uint32_t row[8];
+    #pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const bool ok = e0 + u >= e_begin && e0 + u < e_end;
+                        row[u] = ok ? key[u] >> 16 : NO_ROW;
+                        if (!ok) pv[u] = 0.0f;
+                    }
+                    if (stop_now) {
+    #pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            pv[u] = pv[u] * inv_exag;
+                            if (row[u] != NO_ROW) tval[e0 + u] = pv[u];
                         }
-                        flag = flag || uf;
                     }
+                    float f0[8], f1[8];
+    #pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const float2 yi = lds_pos<D>(yrow, row[u] != NO_ROW ? row[u] : 0u);
+                        const float2 yj = lds_pos<D>(win, row[u] != NO_ROW ? key[u] & 0xffffu : 0u);
+                        const float d0 = yi.x - yj.x;
+                        const float d1 = D == 2 ? yi.y - yj.y : 0.0f;
+                        float dist = 0.0f;
+                        dist += d0 * d0;
+                        if (D == 2) dist += d1 * d1;
+                        const float f = pv[u] / (dist + 1.0f);
+                        f0[u] = f * d0;
+                        f1[u] = f * d1;
+                    }
+                    // runs of equal rows within the lane: the head run (from edge 0), the tail run
+                    // (to edge 7), and rows that start and end inside the lane (added at once)
+                    uint32_t head_row = row[0], cur_row = row[0];
+                    float h0 = 0.0f, h1 = 0.0f, c0 = f0[0], c1 = f1[0];
+                    bool multi = false;
+    #pragma unroll
+                    for (int u = 1; u < 8; u++) {
+                        if (row[u] == cur_row) {
+                            c0 += f0[u];
+                            c1 += f1[u];
+                        } else {
+                            if (!multi) {
+                                h0 = c0;
+                                h1 = c1;
+                                multi = true;
+                            } else if (cur_row != NO_ROW) {
+                                acc_add<D>(acc, cur_row, c0, c1);
+                            }
+                            cur_row = row[u];
+                            c0 = f0[u];
+                            c1 = f1[u];
+                        }
+                    }
+                    const uint32_t tail_row = cur_row;
+                    // segmented inclusive scan of the tail sums over lanes (a segment = equal rows)
+                    const uint32_t prev_tail = __shfl_up(tail_row, 1);
+                    bool flag = lane == 0 || prev_tail != tail_row;
+                    float s0 = c0, s1 = c1;
+    #pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const float u0 = __shfl_up(s0, o), u1 = __shfl_up(s1, o);
+                        const bool uf = __shfl_up((int)flag, o) != 0;
+                        if (lane >= o) {
+                            if (!flag) {
+                                s0 = u0 + s0;
+                                s1 = u1 + s1;
+                            }
+                            flag = flag || uf;
+                        }
+                    }
+                    // the head row of a multi-run lane ends here: add what the lanes before hold
+                    const float p0 = __shfl_up(s0, 1), p1 = __shfl_up(s1, 1);
+                    if (multi && head_row != NO_ROW) {
+                        const bool joins = lane > 0 && prev_tail == head_row;
+                        acc_add<D>(acc, head_row, joins ? p0 + h0 : h0, joins ? p1 + h1 : h1);
+                    }
+                    // the tail row ends here unless the next lane's first edge continues it
+                    const uint32_t next_head = __shfl_down(head_row, 1);
+                    if (tail_row != NO_ROW && (lane == 63 || next_head != tail_row))
+                        acc_add<D>(acc, tail_row, s0, s1);
                 }
-                // the head row of a multi-run lane ends here: add what the lanes before hold
-                const float p0 = __shfl_up(s0, 1), p1 = __shfl_up(s1, 1);
-                if (multi && head_row != NO_ROW) {
-                    const bool joins = lane > 0 && prev_tail == head_row;
-                    acc_add<D>(acc, head_row, joins ? p0 + h0 : h0, joins ? p1 + h1 : h1);
-                }
-                // the tail row ends here unless the next lane's first edge continues it
-                const uint32_t next_head = __shfl_down(head_row, 1);
-                if (tail_row != NO_ROW && (lane == 63 || next_head != tail_row))
-                    acc_add<D>(acc, tail_row, s0, s1);
             }
         }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < nrows * D; k += T_THREADS)
+            out[(size_t)row0 * D + k] = lds_view(acc)[k];
+        __syncthreads();  // the sums are out before the next row block's rows overwrite them
+        if (queued) {
+            if (threadIdx.x == 0) next_lb = atomicAdd(&J.scal->ctr[CTR_ATT], 1u);
+            __syncthreads();
+            lb = next_lb;
+        } else {
+            lb += (uint32_t)bc.nb;
+        }
     }
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nrows * D; k += T_THREADS)
-        out[(size_t)row0 * D + k] = lds_view(acc)[k];
 }
 
 }  // namespace
@@ -462,7 +488,8 @@ void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, 
 }
 
 void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s) {
-    const dim3 g(L.nb[F_ATT]);
+    const bool queued = epoch && L.map[F_ATT] == nullptr;
+    const dim3 g(queued ? std::min(L.nb[F_ATT], ATT_QUEUE_BLOCKS) : L.nb[F_ATT]);
     if (L.dim == 2) {
         if (epoch)
             hipLaunchKernelGGL((k_attract_tiled<2, true>), g, dim3(T_THREADS), 0, s, L.jobs,
