@@ -806,7 +806,7 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
         if (fit) fitsne_repulsion(L, rs);
         else if (j->n > 0) bh_repulsion(L, rs);
         j->mark(rs);
-        attract_tiled(L, nullptr, true, j->stream);
+        attract_tiled(L, nullptr, true, j->stream, fork);
         j->mark(j->stream);
         if (fork) {
             BHT_HIP(hipEventRecord(j->ev_join, j->side));

@@ -231,7 +231,9 @@ const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s);
 
 // Attraction sums over the tiled layout (tiled.hip): epoch = true writes fattr (and applies the
 // stop-lying rescale to tval), epoch = false writes job 0's sums into pos.
-void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s);
+// beside: the job's repulsion runs concurrently on another stream (the attraction then leaves
+// part of the device to it).
+void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s, bool beside = false);
 // The gradient step from fattr, frep and Z (attract.hip).
 void update_points(const LaunchSet &L, hipStream_t s);
 void scale_values(float *vals, uint64_t nnz, float scale, hipStream_t s);

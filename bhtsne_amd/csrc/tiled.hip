@@ -38,10 +38,10 @@ constexpr int T_WAVES = T_THREADS / 64;
 // 512: 1473).
 constexpr int TILE_ROWS_MAX = 1024;
 constexpr int GROUP_SPAN = 512;  // edges per wave pass: 64 lanes x 8
-// A job's epoch launches at most this many attraction blocks, which take row blocks from a work
-// queue: 13/16 of the 512 that fit the device at once (two per CU), so that the repulsion beside
-// it always finds CUs (1M points: 416 blocks 1949 epochs/s, 512: 1872, 352: 1895, one block per
-// row block: 1906).
+// With the repulsion running beside it, a job's attraction launches at most this many blocks,
+// which take row blocks from a work queue: 13/16 of the 512 that fit the device at once (two per
+// CU), so that the repulsion always finds CUs (1M points: 416 blocks 1949 epochs/s, 512: 1872,
+// 352: 1895, one block per row block: 1906).  Alone on the device it keeps a block per row block.
 constexpr int ATT_QUEUE_BLOCKS = 416;
 // LDS per block: window + the block's row positions + the accumulators, under 80,064 bytes so
 // that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran at half speed)
@@ -226,7 +226,7 @@ __device__ __forceinline__ void acc_add(float *acc, uint32_t r, float v0, float 
 template <int D, bool EPOCH>
 __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                                                              const uint32_t *map,
-                                                             float *pos_out) {
+                                                             float *pos_out, int queued) {
     __shared__ __align__(16) float win[WIN_FLOATS];
     __shared__ __align__(16) float yrow[TILE_ROWS_MAX * D];
     __shared__ __align__(16) float acc[TILE_ROWS_MAX * D];
@@ -250,10 +250,9 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
     const bool stop_now = EPOCH && J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
     const float inv_exag = J.inv_exag;
 
-    // a job's epoch takes row blocks from the work queue (reset by the update kernel); otherwise
-    // (batches, the force export) each block has its own row block
+    // queued: the blocks take row blocks from the job's work queue (reset by the update kernel);
+    // otherwise each block has its own row block
     const uint32_t nrb = (n + R - 1) / R;
-    const bool queued = EPOCH && map == nullptr;
     uint32_t lb = (uint32_t)bc.lb;
     if (queued) {
         if (threadIdx.x == 0) next_lb = atomicAdd(&J.scal->ctr[CTR_ATT], 1u);
@@ -487,23 +486,23 @@ void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, 
     BHT_LAUNCH_CHECK();
 }
 
-void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s) {
-    const bool queued = epoch && L.map[F_ATT] == nullptr;
+void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s, bool beside) {
+    const int queued = epoch && beside && L.map[F_ATT] == nullptr ? 1 : 0;
     const dim3 g(queued ? std::min(L.nb[F_ATT], ATT_QUEUE_BLOCKS) : L.nb[F_ATT]);
     if (L.dim == 2) {
         if (epoch)
             hipLaunchKernelGGL((k_attract_tiled<2, true>), g, dim3(T_THREADS), 0, s, L.jobs,
-                               L.map[F_ATT], nullptr);
+                               L.map[F_ATT], nullptr, queued);
         else
             hipLaunchKernelGGL((k_attract_tiled<2, false>), g, dim3(T_THREADS), 0, s, L.jobs,
-                               L.map[F_ATT], pos);
+                               L.map[F_ATT], pos, 0);
     } else {
         if (epoch)
             hipLaunchKernelGGL((k_attract_tiled<1, true>), g, dim3(T_THREADS), 0, s, L.jobs,
-                               L.map[F_ATT], nullptr);
+                               L.map[F_ATT], nullptr, queued);
         else
             hipLaunchKernelGGL((k_attract_tiled<1, false>), g, dim3(T_THREADS), 0, s, L.jobs,
-                               L.map[F_ATT], pos);
+                               L.map[F_ATT], pos, 0);
     }
     BHT_LAUNCH_CHECK();
 }
