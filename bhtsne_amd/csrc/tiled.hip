@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                         pv[0] = v0.x; pv[1] = v0.y; pv[2] = v0.z; pv[3] = v0.w;
                         pv[4] = v1.x; pv[5] = v1.y; pv[6] = v1.z; pv[7] = v1.w;
                     } else {
-    #pragma unroll
+#pragma unroll
                         for (int u = 0; u < 8; u++) {
                             const bool in = e0 + u < nnz;
                             key[u] = in ? tkey[e0 + u] : 0u;
@@ -298,21 +298,21 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                     }
                     // edges of this group only; the others get no row
                     uint32_t row[8];
-    #pragma unroll
+#pragma unroll
                     for (int u = 0; u < 8; u++) {
                         const bool ok = e0 + u >= e_begin && e0 + u < e_end;
                         row[u] = ok ? key[u] >> 16 : NO_ROW;
                         if (!ok) pv[u] = 0.0f;
                     }
                     if (stop_now) {
-    #pragma unroll
+#pragma unroll
                         for (int u = 0; u < 8; u++) {
                             pv[u] = pv[u] * inv_exag;
                             if (row[u] != NO_ROW) tval[e0 + u] = pv[u];
                         }
                     }
                     float f0[8], f1[8];
-    #pragma unroll
+#pragma unroll
                     for (int u = 0; u < 8; u++) {
                         const float2 yi = lds_pos<D>(yrow, row[u] != NO_ROW ? row[u] : 0u);
                         const float2 yj = lds_pos<D>(win, row[u] != NO_ROW ? key[u] & 0xffffu : 0u);
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                     uint32_t head_row = row[0], cur_row = row[0];
                     float h0 = 0.0f, h1 = 0.0f, c0 = f0[0], c1 = f1[0];
                     bool multi = false;
-    #pragma unroll
+#pragma unroll
                     for (int u = 1; u < 8; u++) {
                         if (row[u] == cur_row) {
                             c0 += f0[u];
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                     const uint32_t prev_tail = __shfl_up(tail_row, 1);
                     bool flag = lane == 0 || prev_tail != tail_row;
                     float s0 = c0, s1 = c1;
-    #pragma unroll
+#pragma unroll
                     for (int o = 1; o < 64; o <<= 1) {
                         const float u0 = __shfl_up(s0, o), u1 = __shfl_up(s1, o);
                         const bool uf = __shfl_up((int)flag, o) != 0;
