@@ -39,10 +39,11 @@ constexpr int T_WAVES = T_THREADS / 64;
 constexpr int TILE_ROWS_MAX = 1024;
 constexpr int GROUP_SPAN = 512;  // edges per wave pass: 64 lanes x 8
 // With the repulsion running beside it, a job's attraction launches at most this many blocks,
-// which take row blocks from a work queue: 13/16 of the 512 that fit the device at once (two per
-// CU), so that the repulsion always finds CUs (1M points: 416 blocks 1949 epochs/s, 512: 1872,
-// 352: 1895, one block per row block: 1906).  Alone on the device it keeps a block per row block.
-constexpr int ATT_QUEUE_BLOCKS = 416;
+// which take row blocks from a work queue: 3/4 of the 512 that fit the device at once (two per
+// CU), so that the repulsion always finds CUs (1M points: 320 blocks 1827 epochs/s, 352: 1973,
+// 384: 1978, 416: 1956, one block per row block: 1947).  Alone on the device it keeps a block per
+// row block.
+constexpr int ATT_QUEUE_BLOCKS = 384;
 // LDS per block: window + the block's row positions + the accumulators, under 80,064 bytes so
 // that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran at half speed)
 constexpr int WIN_FLOATS = 11800;
@@ -215,6 +216,37 @@ __device__ void stage_points(BHT_GLOBAL const float *y, uint32_t first, uint32_t
     }
 }
 
+// A lane's 8 consecutive edges: packed keys and values.
+struct EdgeLoad {
+    uint32_t key[8];
+    float pv[8];
+};
+
+__device__ __forceinline__ EdgeLoad load_edges(BHT_GLOBAL const uint32_t *tkey,
+                                               BHT_GLOBAL const float *tval, uint32_t e0,
+                                               uint32_t nnz) {
+    EdgeLoad r;
+    if (e0 + 8u <= nnz) {
+        // plain loads: nontemporal ones measured 3% slower over the whole epoch
+        const u4v k0 = *(BHT_GLOBAL const u4v *)(tkey + e0);
+        const u4v k1 = *(BHT_GLOBAL const u4v *)(tkey + e0 + 4);
+        const f4v v0 = *(BHT_GLOBAL const f4v *)(tval + e0);
+        const f4v v1 = *(BHT_GLOBAL const f4v *)(tval + e0 + 4);
+        r.key[0] = k0.x; r.key[1] = k0.y; r.key[2] = k0.z; r.key[3] = k0.w;
+        r.key[4] = k1.x; r.key[5] = k1.y; r.key[6] = k1.z; r.key[7] = k1.w;
+        r.pv[0] = v0.x; r.pv[1] = v0.y; r.pv[2] = v0.z; r.pv[3] = v0.w;
+        r.pv[4] = v1.x; r.pv[5] = v1.y; r.pv[6] = v1.z; r.pv[7] = v1.w;
+    } else {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const bool in = e0 + u < nnz;
+            r.key[u] = in ? tkey[e0 + u] : 0u;
+            r.pv[u] = in ? tval[e0 + u] : 0.0f;
+        }
+    }
+    return r;
+}
+
 // Adds v to the accumulator of row r (exclusive to the calling lane).
 template <int D>
 __device__ __forceinline__ void acc_add(float *acc, uint32_t r, float v0, float v1) {
@@ -272,29 +304,39 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
             stage_points<D>(y, first, count, win);
             __syncthreads();
             const uint32_t g_end = seg_g0[s + 1];
-            for (uint32_t g = seg_g0[s] + wid; g < g_end; g += T_WAVES) {
-                const uint32_t e_begin = grp_e0[g], e_end = grp_e0[g + 1];
-                for (uint32_t base = e_begin & ~7u; base < e_end; base += GROUP_SPAN) {
+            // The wave's groups of this segment, 64 at a time: lane k holds the edge range of the
+            // wave's k-th group.  Each pass's edges are loaded while the pass before is summed.
+            for (uint32_t gc = seg_g0[s] + wid; gc < g_end; gc += 64u * T_WAVES) {
+                const uint32_t gk = gc + (uint32_t)lane * T_WAVES;
+                const uint32_t hb = gk < g_end ? grp_e0[gk] : 0u;
+                const uint32_t he = gk < g_end ? grp_e0[gk + 1] : 0u;
+                const uint32_t left = (g_end - gc + T_WAVES - 1) / T_WAVES;
+                const uint32_t ng = left < 64u ? left : 64u;
+                uint32_t k = 0;
+                uint32_t e_begin = (uint32_t)__builtin_amdgcn_readlane((int)hb, 0);
+                uint32_t e_end = (uint32_t)__builtin_amdgcn_readlane((int)he, 0);
+                uint32_t base = e_begin & ~7u;
+                EdgeLoad cur = load_edges(tkey, tval, base + 8u * lane, nnz);
+                while (true) {
+                    // the pass after this one: the group's next 512 edges or the next group
+                    uint32_t n_begin = e_begin, n_end = e_end, n_base = base + GROUP_SPAN, nk = k;
+                    bool more = n_base < e_end;
+                    if (!more && k + 1 < ng) {
+                        nk = k + 1;
+                        n_begin = (uint32_t)__builtin_amdgcn_readlane((int)hb, (int)nk);
+                        n_end = (uint32_t)__builtin_amdgcn_readlane((int)he, (int)nk);
+                        n_base = n_begin & ~7u;
+                        more = true;
+                    }
+                    EdgeLoad nxt = {};
+                    if (more) nxt = load_edges(tkey, tval, n_base + 8u * lane, nnz);
                     const uint32_t e0 = base + 8u * lane;
                     uint32_t key[8];
                     float pv[8];
-                    if (e0 + 8u <= nnz) {
-                        // plain loads: nontemporal ones measured 3% slower over the whole epoch
-                        const u4v k0 = *(BHT_GLOBAL const u4v *)(tkey + e0);
-                        const u4v k1 = *(BHT_GLOBAL const u4v *)(tkey + e0 + 4);
-                        const f4v v0 = *(BHT_GLOBAL const f4v *)(tval + e0);
-                        const f4v v1 = *(BHT_GLOBAL const f4v *)(tval + e0 + 4);
-                        key[0] = k0.x; key[1] = k0.y; key[2] = k0.z; key[3] = k0.w;
-                        key[4] = k1.x; key[5] = k1.y; key[6] = k1.z; key[7] = k1.w;
-                        pv[0] = v0.x; pv[1] = v0.y; pv[2] = v0.z; pv[3] = v0.w;
-                        pv[4] = v1.x; pv[5] = v1.y; pv[6] = v1.z; pv[7] = v1.w;
-                    } else {
 #pragma unroll
-                        for (int u = 0; u < 8; u++) {
-                            const bool in = e0 + u < nnz;
-                            key[u] = in ? tkey[e0 + u] : 0u;
-                            pv[u] = in ? tval[e0 + u] : 0.0f;
-                        }
+                    for (int u = 0; u < 8; u++) {
+                        key[u] = cur.key[u];
+                        pv[u] = cur.pv[u];
                     }
                     // edges of this group only; the others get no row
                     uint32_t row[8];
@@ -375,6 +417,12 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                     const uint32_t next_head = __shfl_down(head_row, 1);
                     if (tail_row != NO_ROW && (lane == 63 || next_head != tail_row))
                         acc_add<D>(acc, tail_row, s0, s1);
+                    if (!more) break;
+                    cur = nxt;
+                    base = n_base;
+                    e_begin = n_begin;
+                    e_end = n_end;
+                    k = nk;
                 }
             }
         }
