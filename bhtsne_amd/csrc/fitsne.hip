@@ -18,8 +18,13 @@ int point_blocks(int n) {
     return b < 1024 ? b : 1024;
 }
 
+// The spreading kernels take the box-sorted points in chunks of SPREAD_CHUNK, one wave per chunk
+// in SPREAD_ROUNDS rounds of 64.
+constexpr int SPREAD_ROUNDS = 4;
+constexpr int SPREAD_CHUNK = 64 * SPREAD_ROUNDS;
+
 int spread_blocks(int n) {
-    const uint64_t nchunks = ceil_div((uint64_t)(n > 0 ? n : 1), 64);
+    const uint64_t nchunks = ceil_div((uint64_t)(n > 0 ? n : 1), SPREAD_CHUNK);
     return (int)std::min<uint64_t>(ceil_div(nchunks, PT_THREADS / 64), 2048);
 }
 
@@ -314,12 +319,13 @@ __global__ void __launch_bounds__(256) k_zero_grid(const JobDev *jobs, const uin
 // ---- 3. box-segmented spreading (interpolation.rs:247-269) ----------------------------------
 //
 // Points sorted by box form runs; each box's 3^D nodes receive only that run's charges.  One wave
-// takes 64 sorted points: each lane writes its point's (3^D nodes x terms) contributions as a row
-// of an LDS slab, then lane t walks the rows summing term t, closing a sum at every run end
-// (wave-uniform masks of run ends).  Within a box the charges are therefore added in point order,
-// as the reference's sequential spreading adds them.  A run wholly inside a chunk is written
-// directly; a run crossing chunk boundaries leaves partials that k_spread_fixup adds in chunk
-// order.
+// takes a chunk of SPREAD_CHUNK sorted points in rounds of 64: each lane writes its point's
+// (3^D nodes x terms) contributions as a row of an LDS slab, then lane t walks the rows summing
+// term t, closing a sum at every run end (wave-uniform masks of run ends); a run still open at the
+// end of a round carries its sums into the next.  Within a chunk a box's charges are therefore
+// added in point order, as the reference's sequential spreading adds them.  A run wholly inside a
+// chunk is written directly; a run crossing chunk boundaries leaves partials that k_spread_fixup
+// adds in chunk order.
 
 constexpr int NV2 = 36;  // 9 nodes x 4 terms (D = 2)
 constexpr int NV1 = 9;   // 3 nodes x 3 terms (D = 1)
@@ -409,76 +415,95 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const
     float4 *grid = J.grid;
     float *part = J.part;
     const int n = J.n;
-    const int nchunks = (n + 63) / 64;
+    const int nchunks = (n + SPREAD_CHUNK - 1) / SPREAD_CHUNK;
     const Lagrange3 lb;
     const int lane = threadIdx.x & 63;
     const int wave0 = (bc.lb * PT_THREADS + threadIdx.x) >> 6;
     const int nwaves = (bc.nb * PT_THREADS) >> 6;
     float *part_first = part;
     float *part_last = part + (size_t)nchunks * NV;
-    // the chunk's contributions, one row per point; lane t then sums term t down the rows
+    // a round's contributions, one row per point; lane t then sums term t down the rows
     __shared__ float4 slab_mem[PT_THREADS / 64][64][SLAB_ROW4];
     float *slab = reinterpret_cast<float *>(slab_mem[threadIdx.x >> 6]);
     for (int c = wave0; c < nchunks; c += nwaves) {
-        const int s = c * 64 + lane;
-        const bool valid = s < n;
-        const uint32_t key = valid ? keys[s] : 0xffffffffu;
-        {
-            float v[NV];
-            if (valid) {
-                point_contrib<D>(y, vals[s], p, lb, v);
-            } else {
-#pragma unroll
-                for (int t = 0; t < NV; t++) v[t] = 0.0f;
-            }
-            float4 *row = reinterpret_cast<float4 *>(slab + lane * SLAB_ROW);
-#pragma unroll
-            for (int q = 0; q < (NV + 3) / 4; q++)
-                row[q] = make_float4(v[4 * q], 4 * q + 1 < NV ? v[4 * q + 1] : 0.0f,
-                                     4 * q + 2 < NV ? v[4 * q + 2] : 0.0f,
-                                     4 * q + 3 < NV ? v[4 * q + 3] : 0.0f);
-        }
-        // run structure of the chunk, as wave-uniform masks: a run ends at lane p (a tail) when
-        // the next sorted key differs or the chunk ends; per tail, whether the run began in this
-        // chunk and whether it continues past it
-        const uint32_t next = __shfl_down(key, 1, 64);
-        const bool tail = valid && (s == n - 1 || lane == 63 || next != key);
-        const uint32_t prev = __shfl_up(key, 1, 64);
-        const bool head = valid && (lane == 0 || prev != key);
-        const uint64_t heads = __ballot(head), tails = __ballot(tail);
-        bool starts = false, ends = false;
-        if (tail) {
-            const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-            const int my_head = 63 - __clzll(heads & upto);
-            const int s_head = c * 64 + my_head;
-            starts = s_head == 0 || keys[s_head - 1] != key;
-            ends = s == n - 1 || keys[s + 1] != key;
-        }
-        const uint64_t starts_m = __ballot(starts), ends_m = __ballot(ends);
-        __builtin_amdgcn_wave_barrier();
-        // lane t < NV: sequential sum of term t over each run (point order within a box)
-        float acc = 0.0f;
-        uint64_t rest = tails;
-        int row = 0;
-        while (rest) {
-            const int pt = __ffsll((long long)rest) - 1;  // uniform: next run end
-            rest &= rest - 1;
-            if (lane < NV)
-                for (; row <= pt; row++) acc += slab[row * SLAB_ROW + lane];
-            row = pt + 1;
-            const uint32_t rkey = __shfl(key, pt, 64);
-            const bool rs = (starts_m >> pt) & 1ull, re = (ends_m >> pt) & 1ull;
-            if (lane < NV) {
-                if (rs && re) {
-                    write_box_term<D>(grid, p, rkey, lane, acc);
+        const int c0 = c * SPREAD_CHUNK;
+        float acc = 0.0f;      // lane t < NV: term t of the run being summed
+        bool open = false;     // a run continues from the previous round
+        bool open_rs = false;  // and it started in this chunk
+        for (int q = 0; q < SPREAD_ROUNDS; q++) {
+            const int s0 = c0 + 64 * q;
+            if (s0 >= n) break;
+            const int s = s0 + lane;
+            const bool valid = s < n;
+            const uint32_t key = valid ? keys[s] : 0xffffffffu;
+            {
+                float v[NV];
+                if (valid) {
+                    point_contrib<D>(y, vals[s], p, lb, v);
                 } else {
-                    if (!rs) part_first[(size_t)c * NV + lane] = acc;
-                    if (!re) part_last[(size_t)c * NV + lane] = acc;
+#pragma unroll
+                    for (int t = 0; t < NV; t++) v[t] = 0.0f;
                 }
+                float4 *row = reinterpret_cast<float4 *>(slab + lane * SLAB_ROW);
+#pragma unroll
+                for (int w = 0; w < (NV + 3) / 4; w++)
+                    row[w] = make_float4(v[4 * w], 4 * w + 1 < NV ? v[4 * w + 1] : 0.0f,
+                                         4 * w + 2 < NV ? v[4 * w + 2] : 0.0f,
+                                         4 * w + 3 < NV ? v[4 * w + 3] : 0.0f);
             }
-            acc = 0.0f;
+            // run structure of the round, as wave-uniform masks.  A point's run continues past it
+            // when the next sorted key is the same; a run closes at a point where it does not
+            // continue, or at the chunk's last point.  Per point, whether its run began in this
+            // chunk (rs) and whether it ends here (re).
+            const uint32_t next = __shfl_down(key, 1, 64);
+            const uint32_t after = lane == 63 ? (s + 1 < n ? keys[s + 1] : 0xffffffffu) : next;
+            const bool cont = valid && s + 1 < n && after == key;
+            const bool last_round = q == SPREAD_ROUNDS - 1 || s0 + 64 >= n;
+            const bool tail = valid && (!cont || (lane == 63 && last_round));
+            const uint32_t prev = __shfl_up(key, 1, 64);
+            const bool head = valid && (lane == 0 || prev != key);
+            const uint64_t heads = __ballot(head), tails = __ballot(tail);
+            bool rs = false;
+            if (valid) {
+                const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+                const int h = 63 - __clzll(heads & upto);  // the run's first lane in this round
+                if (h > 0) rs = true;
+                else if (q > 0) rs = open ? open_rs : true;
+                else rs = c0 == 0 || keys[c0 - 1] != key;
+            }
+            const bool re = tail && !cont;
+            const uint64_t rs_m = __ballot(rs), re_m = __ballot(re);
+            __builtin_amdgcn_wave_barrier();
+            // lane t < NV: sequential sum of term t over each run (point order within a box)
+            uint64_t rest = tails;
+            int row = 0;
+            while (rest) {
+                const int pt = __ffsll((long long)rest) - 1;  // uniform: next run end
+                rest &= rest - 1;
+                if (lane < NV)
+                    for (; row <= pt; row++) acc += slab[row * SLAB_ROW + lane];
+                row = pt + 1;
+                const uint32_t rkey = __shfl(key, pt, 64);
+                const bool r_s = (rs_m >> pt) & 1ull, r_e = (re_m >> pt) & 1ull;
+                if (lane < NV) {
+                    if (r_s && r_e) {
+                        write_box_term<D>(grid, p, rkey, lane, acc);
+                    } else {
+                        if (!r_s) part_first[(size_t)c * NV + lane] = acc;
+                        if (!r_e) part_last[(size_t)c * NV + lane] = acc;
+                    }
+                }
+                acc = 0.0f;
+            }
+            // the round's last run continues into the next round: carry its sums
+            open = s0 + 63 < n && !((tails >> 63) & 1ull);
+            if (open) {
+                if (lane < NV)
+                    for (; row < 64; row++) acc += slab[row * SLAB_ROW + lane];
+                open_rs = (rs_m >> 63) & 1ull;
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -495,17 +520,17 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const JobDev *jobs,
     const uint32_t *keys = J.skeys;
     float4 *grid = J.grid;
     const int n = J.n;
-    const int nchunks = (n + 63) / 64;
+    const int nchunks = (n + SPREAD_CHUNK - 1) / SPREAD_CHUNK;
     const float *part_first = J.part;
     const float *part_last = J.part + (size_t)nchunks * NV;
     const int lane = threadIdx.x & 63;
     const int wave0 = (bc.lb * PT_THREADS + threadIdx.x) >> 6;
     const int nwaves = (bc.nb * PT_THREADS) >> 6;
     for (int c = wave0; c < nchunks; c += nwaves) {
-        const int last_s = min(c * 64 + 63, n - 1);
+        const int last_s = min(c * SPREAD_CHUNK + SPREAD_CHUNK - 1, n - 1);
         const uint32_t key = keys[last_s];
         if (!(last_s + 1 < n && keys[last_s + 1] == key)) continue;  // run ends in this chunk
-        const int first_s = c * 64;
+        const int first_s = c * SPREAD_CHUNK;
         const bool whole = keys[first_s] == key;
         if (whole && first_s > 0 && keys[first_s - 1] == key) continue;  // not the run's owner
         // first sorted position past the run, in [lo, hi]: a 64-way search, each lane probing one
@@ -520,7 +545,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const JobDev *jobs,
             hi = min(hi, lo + c_le * step);
             lo = lo + (c_le - 1) * step + 1;
         }
-        const int c_end = (lo - 1) / 64;  // chunk holding the run's last point
+        const int c_end = (lo - 1) / SPREAD_CHUNK;  // chunk holding the run's last point
         float tot[NV];
 #pragma unroll
         for (int t = 0; t < NV; t++) tot[t] = 0.0f;
