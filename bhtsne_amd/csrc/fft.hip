@@ -147,6 +147,13 @@ __device__ void fft_lds(float2 *buf, int L, const float2 *tw) {
 
 __device__ __forceinline__ int signed_lag(int c, int L) { return c <= L / 2 ? c : c - L; }
 
+// Column passes read each line with a stride of a whole row, so neighbouring columns share every
+// cache line.  Workgroups go to the 8 XCDs in turn; work item k of a launch (k = block + i * grid,
+// the grid a multiple of 8) is given line (k % 8) * per + k / 8, so that each XCD takes a
+// contiguous range of columns and reads their shared lines through its own L2.  Items past the
+// last line (from rounding `per` up) are skipped.
+__device__ __forceinline__ int xcd_line(int k, int per) { return (k & 7) * per + (k >> 3); }
+
 // ---- kernel spectrum -------------------------------------------------------------------------
 
 // Rows r1 = 2q, r2 = 2q + 1 of the squared Cauchy kernel (interpolation.rs:198-217), transformed
@@ -211,7 +218,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_kernel_cols(const JobDev *jobs,
     const float2 *tw = tw_all + p.tw_off;
     const float inv_total = 1.0f / (float)(L * L);
     const int npairs = (h + 1) / 2;
-    for (int q = bc.lb; q < npairs; q += bc.nb) {
+    const int per = (npairs + 7) / 8;
+    for (int k = bc.lb; k < 8 * per; k += bc.nb) {
+        const int q = xcd_line(k, per);
+        if (q >= npairs) continue;
         const int v1 = 2 * q, v2 = 2 * q + 1;
         for (int r = threadIdx.x; r < L; r += FFT_THREADS) {
             int rr = r <= L / 2 ? r : L - r;
@@ -272,7 +282,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const JobDev *jobs, c
     const GridPlan p = J.scal->plan;
     const int L = p.len, nodes = p.nodes, h = L / 2 + 1;
     const float2 *tw = tw_all + p.tw_off;
-    for (int line = bc.lb; line < 2 * L; line += bc.nb) {
+    const int per = (2 * L + 7) / 8;
+    for (int k = bc.lb; k < 8 * per; k += bc.nb) {
+        const int line = xcd_line(k, per);
+        if (line >= 2 * L) continue;
         const int g = line / L, c = line % L;
         float2 *col = spec + g * gstride + c;
         for (int r = threadIdx.x; r < L; r += FFT_THREADS)
