@@ -212,8 +212,13 @@ bool sort_group(const LaunchSet &L, int end_bit, hipStream_t s);
 // Key bits of the per-epoch sorts and whether their result lands in the alternate buffers.
 inline int box_key_bits(int dim) { return dim == 2 ? 20 : 10; }  // boxes^D <= 1000^D
 constexpr int MORTON_KEY_BITS = 64;
+// Digit width of a per-epoch sort: 2-D box keys (20 bits) take three passes of 7-bit digits, few
+// enough bins that a tile's keys land in runs; 1-D box keys one 10-bit pass, Morton codes 10-bit
+// passes.
+inline int group_sort_bits(int end_bit) { return end_bit == 20 ? 7 : GROUP_SORT_BITS; }
 inline bool sort_result_alt(int end_bit) {
-    return (((end_bit + GROUP_SORT_BITS - 1) / GROUP_SORT_BITS) & 1) != 0;
+    const int b = group_sort_bits(end_bit);
+    return (((end_bit + b - 1) / b) & 1) != 0;
 }
 
 // ---- FIt-SNE epoch (fitsne.hip, fft.hip) ---------------------------------------------------

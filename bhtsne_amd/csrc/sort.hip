@@ -229,13 +229,14 @@ __global__ void __launch_bounds__(256) k_rs_digit_scan(uint32_t *hist, uint32_t 
 // Exclusive prefix of the BINS digit totals of a segment into base[] (all threads of the block).
 template <int BINS>
 __device__ void digit_prefix(const uint32_t *totals, uint32_t *base, uint32_t *wsum) {
-    constexpr int PER = BINS / RS_THREADS;
-    static_assert(PER >= 1 && BINS % RS_THREADS == 0, "digits per thread");
+    constexpr int PER = BINS >= RS_THREADS ? BINS / RS_THREADS : 1;
+    static_assert(BINS < RS_THREADS || BINS % RS_THREADS == 0, "digits per thread");
+    const bool active = BINS >= RS_THREADS || (int)threadIdx.x < BINS;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t v[PER], sum = 0;
 #pragma unroll
     for (int i = 0; i < PER; i++) {
-        v[i] = totals[threadIdx.x * PER + i];
+        v[i] = active ? totals[threadIdx.x * PER + i] : 0u;
         sum += v[i];
     }
     uint32_t incl = sum;
@@ -248,6 +249,7 @@ __device__ void digit_prefix(const uint32_t *totals, uint32_t *base, uint32_t *w
     __syncthreads();
     uint32_t run = incl - sum;
     for (int q = 0; q < w; q++) run += wsum[q];
+    if (!active) return;
 #pragma unroll
     for (int i = 0; i < PER; i++) {
         base[threadIdx.x * PER + i] = run;
@@ -365,22 +367,27 @@ bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, 
                               1, nullptr, begin_bit, end_bit, temp, s);
 }
 
-// The per-epoch sorts: GROUP_SORT_BITS-bit digits; one segment (a job alone) or one segment per
-// member (a batch; the arena's tiles are ntiles * RS_TILE keys, temp group_sort_temp_bytes of
-// that).
-template <typename K>
-bool sort_group(const LaunchSet &L, int end_bit, hipStream_t s) {
+// The per-epoch sorts, group_sort_bits(end_bit)-bit digits; one segment (a job alone) or one
+// segment per member (a batch; the arena's tiles are ntiles * RS_TILE keys, temp
+// group_sort_temp_bytes of that).
+template <typename K, int BITS>
+static bool sort_group_bits(const LaunchSet &L, int end_bit, hipStream_t s) {
     if (L.segs) {
         if (L.sort_tiles == 0) return false;
-        return radix_passes<K, GROUP_SORT_BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt,
-                                                (uint64_t)L.sort_tiles * RS_TILE, L.sort_tiles,
-                                                L.segs, (unsigned)L.njobs, L.tile_seg, 0, end_bit,
-                                                L.sort_temp, s);
+        return radix_passes<K, BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt,
+                                     (uint64_t)L.sort_tiles * RS_TILE, L.sort_tiles, L.segs,
+                                     (unsigned)L.njobs, L.tile_seg, 0, end_bit, L.sort_temp, s);
     }
     if (L.sort_n == 0) return false;
-    return radix_passes<K, GROUP_SORT_BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt,
-                                            L.sort_n, ceil_div(L.sort_n, RS_TILE), nullptr, 1,
-                                            nullptr, 0, end_bit, L.sort_temp, s);
+    return radix_passes<K, BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt, L.sort_n,
+                                 ceil_div(L.sort_n, RS_TILE), nullptr, 1, nullptr, 0, end_bit,
+                                 L.sort_temp, s);
+}
+
+template <typename K>
+bool sort_group(const LaunchSet &L, int end_bit, hipStream_t s) {
+    if (group_sort_bits(end_bit) == 7) return sort_group_bits<K, 7>(L, end_bit, s);
+    return sort_group_bits<K, GROUP_SORT_BITS>(L, end_bit, s);
 }
 
 template bool radix_sort_pairs<uint32_t>(uint32_t *, uint32_t *, uint32_t *, uint32_t *, uint64_t,
