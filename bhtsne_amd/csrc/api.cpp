@@ -407,13 +407,13 @@ void fill_jobdev(bht_job *j) {
     d.stop_lying_epoch =
         (int)std::min<uint64_t>(j->params.stop_lying_epoch, (uint64_t)INT32_MAX - 2);
     if (d.method == M_FITSNE) {
-        const bool alt = sort_result_alt(box_key_bits(d.dim));
+        const bool alt = sort_result_alt(box_key_bits(d.dim), j->n >= SORT7_MIN_KEYS);
         d.keys = j->keys.as<uint32_t>();
         d.kidx = j->svals.as<uint32_t>();
         d.skeys = alt ? j->keys_alt.as<uint32_t>() : j->keys.as<uint32_t>();
         d.sidx = alt ? j->svals_alt.as<uint32_t>() : j->svals.as<uint32_t>();
     } else {
-        const bool alt = sort_result_alt(MORTON_KEY_BITS);
+        const bool alt = sort_result_alt(MORTON_KEY_BITS, false);
         d.codes = j->codes.as<uint64_t>();
         d.cidx = j->bidx.as<uint32_t>();
         d.scodes = alt ? j->codes_alt.as<uint64_t>() : j->codes.as<uint64_t>();
@@ -1014,7 +1014,7 @@ void batch_rebuild(bht_batch *b) {
         const bool bh = g.method == M_BH;
         const size_t ksize = bh ? 8 : 4;
         const int key_bits = bh ? MORTON_KEY_BITS : box_key_bits(g.dim);
-        const bool alt = sort_result_alt(key_bits);
+        const bool alt = sort_result_alt(key_bits, false);  // batches keep 10-bit digits
         // arena layout: one tile-aligned segment per member
         std::vector<SortSeg> segs(g.jobs.size());
         std::vector<uint32_t> tile_seg;

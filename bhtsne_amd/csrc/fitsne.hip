@@ -18,13 +18,14 @@ int point_blocks(int n) {
     return b < 1024 ? b : 1024;
 }
 
-// The spreading kernels take the box-sorted points in chunks of SPREAD_CHUNK, one wave per chunk
-// in SPREAD_ROUNDS rounds of 64.
-constexpr int SPREAD_ROUNDS = 4;
-constexpr int SPREAD_CHUNK = 64 * SPREAD_ROUNDS;
+// The spreading kernels take the box-sorted points in chunks of 64 * spread_rounds(n), one wave
+// per chunk in rounds of 64.  Large jobs take 4 rounds, so a box of many points leaves 4x fewer
+// chunk partials (1M points: the fixup 29 -> 15 us); small jobs keep one round per wave for the
+// parallelism (4 rounds cost C4's plain fits 8%).
+__host__ __device__ inline int spread_rounds(int n) { return n >= 65536 ? 4 : 1; }
 
 int spread_blocks(int n) {
-    const uint64_t nchunks = ceil_div((uint64_t)(n > 0 ? n : 1), SPREAD_CHUNK);
+    const uint64_t nchunks = ceil_div((uint64_t)(n > 0 ? n : 1), 64 * spread_rounds(n));
     return (int)std::min<uint64_t>(ceil_div(nchunks, PT_THREADS / 64), 2048);
 }
 
@@ -319,7 +320,7 @@ __global__ void __launch_bounds__(256) k_zero_grid(const JobDev *jobs, const uin
 // ---- 3. box-segmented spreading (interpolation.rs:247-269) ----------------------------------
 //
 // Points sorted by box form runs; each box's 3^D nodes receive only that run's charges.  One wave
-// takes a chunk of SPREAD_CHUNK sorted points in rounds of 64: each lane writes its point's
+// takes a chunk of 64 * spread_rounds(n) sorted points in rounds of 64: each lane writes its point's
 // (3^D nodes x terms) contributions as a row of an LDS slab, then lane t walks the rows summing
 // term t, closing a sum at every run end (wave-uniform masks of run ends); a run still open at the
 // end of a round carries its sums into the next.  Within a chunk a box's charges are therefore
@@ -415,7 +416,8 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const
     float4 *grid = J.grid;
     float *part = J.part;
     const int n = J.n;
-    const int nchunks = (n + SPREAD_CHUNK - 1) / SPREAD_CHUNK;
+    const int rounds = spread_rounds(n), chunk = 64 * rounds;
+    const int nchunks = (n + chunk - 1) / chunk;
     const Lagrange3 lb;
     const int lane = threadIdx.x & 63;
     const int wave0 = (bc.lb * PT_THREADS + threadIdx.x) >> 6;
@@ -426,11 +428,11 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const
     __shared__ float4 slab_mem[PT_THREADS / 64][64][SLAB_ROW4];
     float *slab = reinterpret_cast<float *>(slab_mem[threadIdx.x >> 6]);
     for (int c = wave0; c < nchunks; c += nwaves) {
-        const int c0 = c * SPREAD_CHUNK;
+        const int c0 = c * chunk;
         float acc = 0.0f;      // lane t < NV: term t of the run being summed
         bool open = false;     // a run continues from the previous round
         bool open_rs = false;  // and it started in this chunk
-        for (int q = 0; q < SPREAD_ROUNDS; q++) {
+        for (int q = 0; q < rounds; q++) {
             const int s0 = c0 + 64 * q;
             if (s0 >= n) break;
             const int s = s0 + lane;
@@ -458,7 +460,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const
             const uint32_t next = __shfl_down(key, 1, 64);
             const uint32_t after = lane == 63 ? (s + 1 < n ? keys[s + 1] : 0xffffffffu) : next;
             const bool cont = valid && s + 1 < n && after == key;
-            const bool last_round = q == SPREAD_ROUNDS - 1 || s0 + 64 >= n;
+            const bool last_round = q == rounds - 1 || s0 + 64 >= n;
             const bool tail = valid && (!cont || (lane == 63 && last_round));
             const uint32_t prev = __shfl_up(key, 1, 64);
             const bool head = valid && (lane == 0 || prev != key);
@@ -520,17 +522,18 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const JobDev *jobs,
     const uint32_t *keys = J.skeys;
     float4 *grid = J.grid;
     const int n = J.n;
-    const int nchunks = (n + SPREAD_CHUNK - 1) / SPREAD_CHUNK;
+    const int chunk = 64 * spread_rounds(n);
+    const int nchunks = (n + chunk - 1) / chunk;
     const float *part_first = J.part;
     const float *part_last = J.part + (size_t)nchunks * NV;
     const int lane = threadIdx.x & 63;
     const int wave0 = (bc.lb * PT_THREADS + threadIdx.x) >> 6;
     const int nwaves = (bc.nb * PT_THREADS) >> 6;
     for (int c = wave0; c < nchunks; c += nwaves) {
-        const int last_s = min(c * SPREAD_CHUNK + SPREAD_CHUNK - 1, n - 1);
+        const int last_s = min(c * chunk + chunk - 1, n - 1);
         const uint32_t key = keys[last_s];
         if (!(last_s + 1 < n && keys[last_s + 1] == key)) continue;  // run ends in this chunk
-        const int first_s = c * SPREAD_CHUNK;
+        const int first_s = c * chunk;
         const bool whole = keys[first_s] == key;
         if (whole && first_s > 0 && keys[first_s - 1] == key) continue;  // not the run's owner
         // first sorted position past the run, in [lo, hi]: a 64-way search, each lane probing one
@@ -545,7 +548,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const JobDev *jobs,
             hi = min(hi, lo + c_le * step);
             lo = lo + (c_le - 1) * step + 1;
         }
-        const int c_end = (lo - 1) / SPREAD_CHUNK;  // chunk holding the run's last point
+        const int c_end = (lo - 1) / chunk;  // chunk holding the run's last point
         float tot[NV];
 #pragma unroll
         for (int t = 0; t < NV; t++) tot[t] = 0.0f;

@@ -367,7 +367,7 @@ bool radix_sort_pairs(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, 
                               1, nullptr, begin_bit, end_bit, temp, s);
 }
 
-// The per-epoch sorts, group_sort_bits(end_bit)-bit digits; one segment (a job alone) or one
+// The per-epoch sorts, group_sort_bits(end_bit, large)-bit digits; one segment (a job alone) or one
 // segment per member (a batch; the arena's tiles are ntiles * RS_TILE keys, temp
 // group_sort_temp_bytes of that).
 template <typename K, int BITS>
@@ -386,7 +386,8 @@ static bool sort_group_bits(const LaunchSet &L, int end_bit, hipStream_t s) {
 
 template <typename K>
 bool sort_group(const LaunchSet &L, int end_bit, hipStream_t s) {
-    if (group_sort_bits(end_bit) == 7) return sort_group_bits<K, 7>(L, end_bit, s);
+    const bool large = L.segs == nullptr && L.sort_n >= SORT7_MIN_KEYS;
+    if (group_sort_bits(end_bit, large) == 7) return sort_group_bits<K, 7>(L, end_bit, s);
     return sort_group_bits<K, GROUP_SORT_BITS>(L, end_bit, s);
 }
 
