@@ -300,23 +300,30 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
         for (uint32_t s = s_begin; s < s_end; s++) {
             const uint32_t t = seg_tile[s];
             const uint32_t first = t * W, count = n - first < W ? n - first : W;
+            const uint32_t g_end = seg_g0[s + 1];
+            // The wave's groups of this segment, 64 at a time: lane k holds the edge range of the
+            // wave's k-th group.  Each pass's edges are loaded while the pass before is summed,
+            // and the first pass's while the tile is copied into LDS.
+            uint32_t gc = seg_g0[s] + wid;
+            uint32_t hb = 0, he = 0, ng = 0, k = 0, e_begin = 0, e_end = 0, base = 0;
+            EdgeLoad cur = {};
+            auto open_groups = [&]() {
+                const uint32_t gk = gc + (uint32_t)lane * T_WAVES;
+                hb = gk < g_end ? grp_e0[gk] : 0u;
+                he = gk < g_end ? grp_e0[gk + 1] : 0u;
+                const uint32_t left = (g_end - gc + T_WAVES - 1) / T_WAVES;
+                ng = left < 64u ? left : 64u;
+                k = 0;
+                e_begin = (uint32_t)__builtin_amdgcn_readlane((int)hb, 0);
+                e_end = (uint32_t)__builtin_amdgcn_readlane((int)he, 0);
+                base = e_begin & ~7u;
+                cur = load_edges(tkey, tval, base + 8u * lane, nnz);
+            };
+            if (gc < g_end) open_groups();
             __syncthreads();  // the previous tile's reads are done
             stage_points<D>(y, first, count, win);
             __syncthreads();
-            const uint32_t g_end = seg_g0[s + 1];
-            // The wave's groups of this segment, 64 at a time: lane k holds the edge range of the
-            // wave's k-th group.  Each pass's edges are loaded while the pass before is summed.
-            for (uint32_t gc = seg_g0[s] + wid; gc < g_end; gc += 64u * T_WAVES) {
-                const uint32_t gk = gc + (uint32_t)lane * T_WAVES;
-                const uint32_t hb = gk < g_end ? grp_e0[gk] : 0u;
-                const uint32_t he = gk < g_end ? grp_e0[gk + 1] : 0u;
-                const uint32_t left = (g_end - gc + T_WAVES - 1) / T_WAVES;
-                const uint32_t ng = left < 64u ? left : 64u;
-                uint32_t k = 0;
-                uint32_t e_begin = (uint32_t)__builtin_amdgcn_readlane((int)hb, 0);
-                uint32_t e_end = (uint32_t)__builtin_amdgcn_readlane((int)he, 0);
-                uint32_t base = e_begin & ~7u;
-                EdgeLoad cur = load_edges(tkey, tval, base + 8u * lane, nnz);
+            while (gc < g_end) {
                 while (true) {
                     // the pass after this one: the group's next 512 edges or the next group
                     uint32_t n_begin = e_begin, n_end = e_end, n_base = base + GROUP_SPAN, nk = k;
@@ -424,6 +431,8 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                     e_end = n_end;
                     k = nk;
                 }
+                gc += 64u * T_WAVES;
+                if (gc < g_end) open_groups();
             }
         }
         __syncthreads();
