@@ -145,6 +145,51 @@ __device__ void fft_lds(float2 *buf, int L, const float2 *tw) {
     while (rem % 7 == 0) { stockham_stage<7, INV>(buf, L, Ns, tw); Ns *= 7; rem /= 7; }
 }
 
+// One wave per line (lines up to WAVE_FFT_MAX points): the same Stockham stages, ping-ponging
+// between two LDS buffers of the wave, so a stage needs no block barrier.  Each output element is
+// computed with the same operations in the same order as in fft_lds.
+constexpr int WAVE_FFT_MAX = 1024;
+constexpr size_t WAVE_FFT_LDS = (FFT_THREADS / 64) * 2 * WAVE_FFT_MAX * sizeof(float2);
+
+template <int R, bool INV>
+__device__ __forceinline__ void wave_stage(const float2 *src, float2 *dst, int L, int Ns,
+                                           const float2 *tw, int lane) {
+    const int M = L / R;
+    const int tstep = L / (Ns * R);
+    for (int j = lane; j < M; j += 64) {
+        const int k0 = j % Ns;
+        float2 x[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            float2 a = src[j + k * M];
+            if (k > 0 && k0 > 0) {
+                float2 w = tw[k0 * k * tstep];
+                if (INV) w.y = -w.y;
+                a = cmul(a, w);
+            }
+            x[k] = a;
+        }
+        dft<R, INV>(x);
+        const int base = (j - k0) * R + k0;
+#pragma unroll
+        for (int q = 0; q < R; q++) dst[base + q * Ns] = x[q];
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Transforms the line in a; b is the wave's second buffer.  Returns the buffer holding the result.
+template <bool INV>
+__device__ float2 *fft_wave(float2 *a, float2 *b, int L, const float2 *tw, int lane) {
+    int Ns = 1, rem = L;
+    auto flip = [&]() { float2 *t = a; a = b; b = t; };
+    while (rem % 4 == 0) { wave_stage<4, INV>(a, b, L, Ns, tw, lane); flip(); Ns *= 4; rem /= 4; }
+    while (rem % 2 == 0) { wave_stage<2, INV>(a, b, L, Ns, tw, lane); flip(); Ns *= 2; rem /= 2; }
+    while (rem % 3 == 0) { wave_stage<3, INV>(a, b, L, Ns, tw, lane); flip(); Ns *= 3; rem /= 3; }
+    while (rem % 5 == 0) { wave_stage<5, INV>(a, b, L, Ns, tw, lane); flip(); Ns *= 5; rem /= 5; }
+    while (rem % 7 == 0) { wave_stage<7, INV>(a, b, L, Ns, tw, lane); flip(); Ns *= 7; rem /= 7; }
+    return a;
+}
+
 __device__ __forceinline__ int signed_lag(int c, int L) { return c <= L / 2 ? c : c - L; }
 
 // Column passes read each line with a stride of a whole row, so neighbouring columns share every
@@ -282,6 +327,35 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const JobDev *jobs, c
     const GridPlan p = J.scal->plan;
     const int L = p.len, nodes = p.nodes, h = L / 2 + 1;
     const float2 *tw = tw_all + p.tw_off;
+    if (L <= WAVE_FFT_MAX) {
+        // a wave per column line, four lines per work item
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        float2 *ba = buf + (size_t)w * 2 * WAVE_FFT_MAX, *bb = ba + WAVE_FFT_MAX;
+        const int nlines = 2 * L, nquads = (nlines + 3) / 4;
+        const int per = (nquads + 7) / 8;
+        for (int k = bc.lb; k < 8 * per; k += bc.nb) {
+            const int quad = xcd_line(k, per), line = 4 * quad + w;
+            if (quad >= nquads || line >= nlines) continue;
+            const int g = line / L, c = line % L;
+            float2 *col = spec + g * gstride + c;
+            for (int r = lane; r < L; r += 64)
+                ba[r] = r < nodes ? col[(size_t)r * L] : make_float2(0.f, 0.f);
+            __builtin_amdgcn_wave_barrier();
+            float2 *f = fft_wave<false>(ba, bb, L, tw, lane);
+            const int cc = c <= L / 2 ? c : L - c;
+            for (int u = lane; u < L; u += 64) {
+                const int uu = u <= L / 2 ? u : L - u;
+                const float kv = kspec[(size_t)uu * h + cc];
+                const float2 z = f[u];
+                f[u] = make_float2(z.x * kv, z.y * kv);
+            }
+            __builtin_amdgcn_wave_barrier();
+            const float2 *r2 = fft_wave<true>(f, f == ba ? bb : ba, L, tw, lane);
+            for (int r = lane; r < nodes; r += 64) col[(size_t)r * L] = r2[r];
+            __builtin_amdgcn_wave_barrier();
+        }
+        return;
+    }
     const int per = (2 * L + 7) / 8;
     for (int k = bc.lb; k < 8 * per; k += bc.nb) {
         const int line = xcd_line(k, per);
@@ -395,8 +469,8 @@ void fft_convolve(const LaunchSet &L, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_rows_fwd, dim3(L.nb[F_ROWS]), dim3(FFT_THREADS), lds, s, L.jobs,
                        L.map[F_ROWS], L.tw->table);
-    hipLaunchKernelGGL(k_cols_conv, dim3(L.nb[F_COLS]), dim3(FFT_THREADS), lds, s, L.jobs,
-                       L.map[F_COLS], L.tw->table);
+    hipLaunchKernelGGL(k_cols_conv, dim3(L.nb[F_COLS]), dim3(FFT_THREADS),
+                       std::max(lds, WAVE_FFT_LDS), s, L.jobs, L.map[F_COLS], L.tw->table);
     hipLaunchKernelGGL(k_rows_inv, dim3(L.nb[F_ROWS]), dim3(FFT_THREADS), lds, s, L.jobs,
                        L.map[F_ROWS], L.tw->table);
     BHT_LAUNCH_CHECK();
