@@ -39,11 +39,11 @@ constexpr int T_WAVES = T_THREADS / 64;
 constexpr int TILE_ROWS_MAX = 1024;
 constexpr int GROUP_SPAN = 512;  // edges per wave pass: 64 lanes x 8
 // With the repulsion running beside it, a job's attraction launches at most this many blocks,
-// which take row blocks from a work queue: 3/4 of the 512 that fit the device at once (two per
-// CU), so that the repulsion always finds CUs (1M points: 320 blocks 1827 epochs/s, 352: 1973,
-// 384: 1978, 416: 1956, one block per row block: 1947).  Alone on the device it keeps a block per
-// row block.
-constexpr int ATT_QUEUE_BLOCKS = 384;
+// which take row blocks from a work queue: 13/16 of the 512 that fit the device at once (two per
+// CU), so that the repulsion always finds CUs (1M points: 352 blocks 2041 epochs/s, 384: 2078,
+// 416: 2094; the best share moves with the repulsion's own speed).  Alone on the device it keeps a
+// block per row block.
+constexpr int ATT_QUEUE_BLOCKS = 416;
 // LDS per block: window + the block's row positions + the accumulators, under 80,064 bytes so
 // that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran at half speed)
 constexpr int WIN_FLOATS = 11800;
