@@ -275,6 +275,18 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *keys, const 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const TileCtx tc = tile_ctx<BINS>(n, ntiles, segs, tile_seg);
     // base[d] = segment base + keys of smaller digits + keys of digit d in earlier tiles
+    // every key of the segment has digit 0 in this pass (a high digit of small keys): the stable
+    // order is the input order, so the tile is copied
+    if (totals[tc.t0] == (uint32_t)(tc.end - tc.out_base)) {
+        for (int r = 0; r < RS_ITEMS; r++) {
+            const uint64_t idx = tc.first + (uint64_t)r * RS_THREADS + threadIdx.x;
+            if (idx < tc.end) {
+                keys_out[idx] = keys[idx];
+                vals_out[idx] = vals[idx];
+            }
+        }
+        return;
+    }
     digit_prefix<BINS>(totals + tc.t0, base, wsum);
     __syncthreads();
     for (int b = threadIdx.x; b < BINS; b += RS_THREADS)
