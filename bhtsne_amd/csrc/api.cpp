@@ -796,11 +796,15 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     const bool fork = j->n >= FORK_MIN_POINTS && !(j->params.flags & BHT_FLAG_SERIAL_EPOCH);
     hipStream_t rs = fork ? j->side : j->stream;
     for (uint64_t e = 0; e < count; e++) {
-        // recentre (+ FIt-SNE grid plan), then the repulsion and the attraction side by side
-        center_plan(L, fit, j->stream);
+        // recentre (+ FIt-SNE grid plan), then the repulsion and the attraction side by side; a
+        // forked epoch plans the grid on the repulsion's stream
         if (fork) {
+            center_points(L, j->stream);
             BHT_HIP(hipEventRecord(j->ev_fork, j->stream));
             BHT_HIP(hipStreamWaitEvent(j->side, j->ev_fork, 0));
+            plan_grid(L, fit, j->side);
+        } else {
+            center_plan(L, fit, j->stream);
         }
         j->mark(j->stream);
         if (fit) fitsne_repulsion(L, rs);

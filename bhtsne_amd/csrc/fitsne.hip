@@ -234,23 +234,33 @@ __global__ void __launch_bounds__(256) k_plan(const JobDev *jobs, const uint32_t
     scal->plan = p;
 }
 
-void center_plan(const LaunchSet &L, bool do_plan, hipStream_t s) {
-    const int *lo = L.tw ? L.tw->len_of_boxes : nullptr, *of = L.tw ? L.tw->off_of_boxes : nullptr;
-    const int plan = do_plan && L.tw != nullptr;
+void center_points(const LaunchSet &L, hipStream_t s) {
     hipLaunchKernelGGL(k_mean_finalize, dim3(L.nb[F_ONE]), dim3(FIN_THREADS), 0, s, L.jobs,
                        L.map[F_ONE]);
-    if (L.dim == 2) {
+    if (L.dim == 2)
         hipLaunchKernelGGL(k_center<2>, dim3(L.nb[F_PT]), dim3(PT_THREADS), 0, s, L.jobs,
                            L.map[F_PT]);
-        hipLaunchKernelGGL(k_plan<2>, dim3(L.nb[F_ONE]), dim3(256), 0, s, L.jobs, L.map[F_ONE], lo,
-                           of, plan);
-    } else {
+    else
         hipLaunchKernelGGL(k_center<1>, dim3(L.nb[F_PT]), dim3(PT_THREADS), 0, s, L.jobs,
                            L.map[F_PT]);
+    BHT_LAUNCH_CHECK();
+}
+
+void plan_grid(const LaunchSet &L, bool do_plan, hipStream_t s) {
+    const int *lo = L.tw ? L.tw->len_of_boxes : nullptr, *of = L.tw ? L.tw->off_of_boxes : nullptr;
+    const int plan = do_plan && L.tw != nullptr;
+    if (L.dim == 2)
+        hipLaunchKernelGGL(k_plan<2>, dim3(L.nb[F_ONE]), dim3(256), 0, s, L.jobs, L.map[F_ONE], lo,
+                           of, plan);
+    else
         hipLaunchKernelGGL(k_plan<1>, dim3(L.nb[F_ONE]), dim3(256), 0, s, L.jobs, L.map[F_ONE], lo,
                            of, plan);
-    }
     BHT_LAUNCH_CHECK();
+}
+
+void center_plan(const LaunchSet &L, bool do_plan, hipStream_t s) {
+    center_points(L, s);
+    plan_grid(L, do_plan, s);
 }
 
 // Z = sum of the per-block sums (minus the n self-interactions for FIt-SNE, interpolation.rs:331;

@@ -229,6 +229,10 @@ inline bool sort_result_alt(int end_bit, bool large) {
 // ---- FIt-SNE epoch (fitsne.hip, fft.hip) ---------------------------------------------------
 // Applies the pending recentring of the last update, then bounds and (do_plan) the grid plan.
 void center_plan(const LaunchSet &L, bool do_plan, hipStream_t s);
+// Its two halves: the recentring (mean + shift + per-block bounds), and the bounds' combination
+// with the grid plan (which only the repulsion reads).
+void center_points(const LaunchSet &L, hipStream_t s);
+void plan_grid(const LaunchSet &L, bool do_plan, hipStream_t s);
 // Z and 1/Z from the per-block partial sums (minus n for FIt-SNE).
 void finalize_z(const LaunchSet &L, int part_family, hipStream_t s);
 // after center_plan: frep, Z; returns the sorted point indices of job 0 (a spatial order)
