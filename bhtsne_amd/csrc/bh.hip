@@ -67,20 +67,28 @@ __global__ void __launch_bounds__(256) k_bh_codes(const JobDev *jobs, const uint
     }
 }
 
-// block-wide exclusive scan of one u32 per thread (BUILD_THREADS threads)
+// block-wide exclusive scan of one u32 per thread (BUILD_THREADS threads): a shuffle scan in
+// each wave, then every thread adds the totals of the waves before its own
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *buf, uint32_t &total) {
-    buf[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < BUILD_THREADS; off <<= 1) {
-        uint32_t t = threadIdx.x >= (unsigned)off ? buf[threadIdx.x - off] : 0u;
-        __syncthreads();
-        buf[threadIdx.x] += t;
-        __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(x, off, 64);
+        if (lane >= off) x += t;
     }
-    total = buf[BUILD_THREADS - 1];
-    uint32_t r = buf[threadIdx.x] - v;
+    if (lane == 63) buf[w] = x;
     __syncthreads();
-    return r;
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < BUILD_THREADS / 64; k++) {
+        const uint32_t c = buf[k];
+        pre += k < w ? c : 0u;
+        tot += c;
+    }
+    total = tot;
+    __syncthreads();
+    return pre + x - v;
 }
 
 __device__ __forceinline__ uint32_t lower_bound_digit(const uint64_t *codes, uint32_t lo,
@@ -96,10 +104,8 @@ __device__ __forceinline__ uint32_t lower_bound_digit(const uint64_t *codes, uin
 
 __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
                                                             const uint32_t *map) {
-    __shared__ uint32_t sbuf[BUILD_THREADS];
+    __shared__ uint32_t sbuf[BUILD_THREADS / 64];
     __shared__ uint32_t round_start[MAX_ROUNDS + 2];
-    __shared__ uint32_t s_nn;
-    __shared__ int s_rounds;
     const BlockCtx bc = block_ctx(map, jobs, F_ONE);
     const JobDev &J = jobs[bc.job];
     const float *y = job_y(J);
@@ -122,17 +128,18 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
         t.meta[0] = (uint32_t)BITS << 8;
         t.range[0] = 0;
         t.range[1] = (uint32_t)n;
-        s_nn = 1;
         round_start[0] = 0;
         round_start[1] = 1;
-        s_rounds = 1;
     }
     __syncthreads();
     if (n == 0) {
         if (tid == 0) t.counts[0] = 0;
         return;
     }
-    // breadth-first emission (arena.rs:197-259)
+    // breadth-first emission (arena.rs:197-259); every thread keeps the node count, which each
+    // chunk's scan total advances alike in all threads
+    uint32_t nn = 1;
+    int rounds = 1;
     for (int r = 0; r < MAX_ROUNDS; r++) {
         const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
         if (f0 == f1) break;
@@ -157,7 +164,7 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
             }
             uint32_t total;
             const uint32_t off = block_excl_scan(nchild, sbuf, total);
-            const uint32_t first = s_nn + off;
+            const uint32_t first = nn + off;
             if (i < f1 && internal) {
                 float4 nd = t.node[i];
                 nd.w = __uint_as_float(first);
@@ -174,17 +181,13 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
                     c++;
                 }
             }
-            __syncthreads();
-            if (tid == 0) s_nn += total;
-            __syncthreads();
+            nn += total;
         }
-        if (tid == 0) {
-            round_start[r + 2] = s_nn;
-            s_rounds = r + 1;
-        }
+        // the round's nodes and its end are visible to the whole block from here on
+        if (tid == 0) round_start[r + 2] = nn;
+        rounds = r + 1;
         __syncthreads();
     }
-    const int rounds = s_rounds;
     // centres of mass, deepest round first (arena.rs:261-309)
     for (int r = rounds - 1; r >= 0; r--) {
         const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
@@ -215,7 +218,7 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
         }
         __syncthreads();
     }
-    if (tid == 0) t.counts[0] = s_nn;
+    if (tid == 0) t.counts[0] = nn;
 }
 
 __global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const JobDev *jobs,
