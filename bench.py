@@ -25,11 +25,13 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "c3": dict(n=1_000_000, name="FIt-SNE 2D, N=1,000,000, D_in=50, 50-component GMM "
                "(centres N(0,36I), sigma 1, Dirichlet(1) weights, seed 2), perplexity 30 (k=90), "
-               "grid >= 50 boxes/axis", min_boxes=50),
+               "grid >= 50 boxes/axis", min_boxes=50, method="fitsne", epochs=750),
     "c2": dict(n=70_000, name="FIt-SNE 2D, N=70,000, D_in=20, 10-component GMM (centres "
-               "N(0,16I), axis std 1/sqrt(i+1), seed 1), perplexity 30", min_boxes=10),
-    "c1": dict(n=5_000, name="FIt-SNE 2D, N=5,000, D_in=50, 10-component GMM (seed 0), "
-               "perplexity 30", min_boxes=10),
+               "N(0,16I), axis std 1/sqrt(i+1), seed 1), perplexity 30", min_boxes=10,
+               method="fitsne", epochs=750),
+    "c1": dict(n=5_000, name="Barnes-Hut 2D theta 0.5, N=5,000, D_in=50, 10-component GMM "
+               "(centres N(0,25I), sigma 1, equal weights, seed 0), perplexity 30 (k=90)",
+               min_boxes=10, method="bh", epochs=1000),
     "c4": dict(name="batched: independent jobs seeded by job id, N log-uniform [500, 50,000], "
                "D_in in {16,32,64}, 5-20 component GMM (centres N(0,25I), sigma 1), D_out 2, "
                "FIt-SNE (even ids) / Barnes-Hut theta 0.5 (odd ids), perplexity 30"),
@@ -181,10 +183,10 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(aff, y_seed, state_late, min_boxes, n_epochs=3):
+def cpu_baseline(aff, y_seed, state_late, late_epoch, min_boxes, n_epochs=3, method="fitsne"):
     """The oracle (a C restatement of the reference's f32 loop, OpenMP over rows as rayon is)
-    timed on this host: n_epochs from the seed state and n_epochs from the GPU run's final state,
-    averaged per epoch."""
+    timed on this host: n_epochs from the seed state (epoch 0) and n_epochs from the GPU run's
+    state at epoch `late_epoch` (after stop-lying), averaged per epoch."""
     import oracle
     threads = oracle.set_threads(0)
     rows = np.ascontiguousarray(aff.rows, np.uint64)
@@ -192,23 +194,28 @@ def cpu_baseline(aff, y_seed, state_late, min_boxes, n_epochs=3):
     n = y_seed.shape[0]
     lr = oracle.auto_learning_rate(n, 12.0)
     times = []
-    for label, (y, uy, g), vals_scale in (
-            ("early", (y_seed.copy(), np.zeros_like(y_seed), np.ones_like(y_seed)), 12.0),
-            ("late", tuple(a.copy() for a in state_late), 1.0)):
+    for label, (y, uy, g), vals_scale, e0 in (
+            ("early", (y_seed.copy(), np.zeros_like(y_seed), np.ones_like(y_seed)), 12.0, 0),
+            ("late", tuple(a.copy() for a in state_late), 1.0, late_epoch)):
         vals = (np.ascontiguousarray(aff.values, np.float32) * np.float32(vals_scale))
         t0 = time.perf_counter()
-        oracle.run_epochs(y, uy, g, rows, cols, vals, method="fitsne", min_boxes=min_boxes,
-                          learning_rate=lr, e0=0 if label == "early" else 600,
-                          e1=n_epochs if label == "early" else 600 + n_epochs)
+        oracle.run_epochs(y, uy, g, rows, cols, vals, method=method, min_boxes=min_boxes,
+                          theta=0.5, learning_rate=lr, e0=e0, e1=e0 + n_epochs,
+                          momentum=0.5 if label == "early" else 0.8)
         dt = time.perf_counter() - t0
         times.append(dt / n_epochs)
-        log(f"cpu baseline {label}: {dt / n_epochs:.3f} s/epoch")
+        log(f"cpu baseline {label} (epochs {e0}..{e0 + n_epochs}): {dt / n_epochs:.3f} s/epoch")
     per_epoch = float(np.mean(times))
+    fft = ("; its FFT is a recursive mixed-radix complex transform in double precision with an "
+           "allocation per line, slower than the reference's rustfft/realfft, so this rate is a "
+           "lower bound of the reference's and the GPU/CPU ratio an upper bound"
+           if method == "fitsne" else "")
     return {"value": 1.0 / per_epoch, "unit": "epochs/s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle (C restatement of the reference loop, the same grid >= "
-                      f"{min_boxes} boxes) on this host: {n_epochs} epochs from the seed state + "
-                      f"{n_epochs} epochs from the GPU run's final state, mean s/epoch "
-                      f"(early {times[0]:.3f}, late {times[1]:.3f})"}
+            "sample": f"oracle (C restatement of the reference loop, "
+                      f"{'grid >= %d boxes' % min_boxes if method == 'fitsne' else 'theta 0.5'})"
+                      f" on this host: {n_epochs} epochs from the seed state (epoch 0) + "
+                      f"{n_epochs} epochs from the GPU run's state at epoch {late_epoch}, mean "
+                      f"s/epoch (early {times[0]:.3f}, late {times[1]:.3f}){fft}"}
 
 
 def main():
@@ -260,97 +267,147 @@ def main():
         return
 
     import bhtsne_amd as bt
-    from bhtsne_amd import datagen
+    from bhtsne_amd import datagen, metrics
 
     wl = WORKLOADS[args.workload]
+    method = wl["method"]
+    total_epochs = wl["epochs"]
+    steps = args.steps
+    seed = {"c1": 0, "c2": 1, "c3": 2}[args.workload]
     t0 = time.perf_counter()
     x = make_data(args.workload)
     n = x.shape[0]
-    log(f"data {x.shape} generated in {time.perf_counter() - t0:.1f}s sha256={datagen.sha256(x)[:16]}")
-    y_seed = datagen.seed_embedding(n, 2, 2)
+    log(f"data {x.shape} generated in {time.perf_counter() - t0:.1f}s "
+        f"sha256={datagen.sha256(x)[:16]}")
+    y_seed = datagen.seed_embedding(n, 2, seed)
 
-    t = bt.TSNE(x).perplexity(30.0).epochs(args.steps).min_boxes(wl["min_boxes"])
+    t = bt.TSNE(x).perplexity(30.0).epochs(total_epochs).min_boxes(wl["min_boxes"])
     if args.serial_epoch:
         t._set(flags=2)  # BHT_FLAG_SERIAL_EPOCH: repulsion and attraction in order, one stream
     # affinities: kNN + bandwidth search + symmetrisation on the GPU (not in the timed region)
     t0 = time.perf_counter()
-    t.initial_embedding(y_seed).prepare("fitsne")
+    t.initial_embedding(y_seed).prepare(method, 0.5)
     t.synchronize()
     t_aff = time.perf_counter() - t0
     aff = t.affinities()
     nnz = int(aff.columns.size)
     log(f"affinities built in {t_aff:.2f}s, nnz={nnz}")
 
-    # warmup run (discarded), then a fresh run from the seed state
+    # warmup run (discarded), then a fresh run from the seed state, timed with no events
     if args.warmup > 0:
         t.step(args.warmup)
         t.synchronize()
-    t.initial_embedding(y_seed).prepare("fitsne")
-    t.set_timing(True)
+    t.initial_embedding(y_seed).prepare(method, 0.5)
     t.synchronize()
     start = time.perf_counter()
-    t.step(args.steps)
+    t.step(steps)
     t.synchronize()
     elapsed = time.perf_counter() - start
-    rep_ms, upd_ms, timed = t.timing()
-    t.set_timing(False)
-    grid = t.grid_info()
-    state = t.get_state()
-    # gradient of the final state against the exact FP64 gradient (all-pairs repulsion and Z on
-    # the device, edge terms in FP64 on a 5,000-row sample)
-    from bhtsne_amd import metrics
+    eps = steps / elapsed
+    grid = t.grid_info() if method == "fitsne" else None
+    # the state after the timed epochs against the exact FP64 gradient (all-pairs repulsion and Z
+    # on the device, edge terms in FP64 on a 5,000-row sample)
     t0 = time.perf_counter()
     errs = metrics.error_report(t, sample=5000, seed=0)
-    log(f"vs FP64 exact (5k rows, final state): gradient rel L2 {errs['gradient']:.3e}, "
+    log(f"vs FP64 exact (5k rows, epoch {steps}): gradient rel L2 {errs['gradient']:.3e}, "
         f"repulsion F-/Z rel L2 {errs['repulsion']:.3e}, Z rel {errs['z']:.2e} "
         f"({time.perf_counter() - t0:.1f}s)")
     t.finish()
-    eps = args.steps / elapsed
-    log(f"{args.steps} epochs in {elapsed:.3f}s -> {eps:.1f} epochs/s; final grid {grid}")
+    kl_steps = t.kl_divergence_pair()
+    log(f"{steps} epochs in {elapsed:.4f}s -> {eps:.1f} epochs/s; grid {grid}; KL {kl_steps}")
+
+    # per-phase device time: the same epochs again from the seed state with HIP events on the
+    # job's streams, then the run continued (untimed) to its full length for the final KL and
+    # the late state of the CPU baseline
+    t.initial_embedding(y_seed).prepare(method, 0.5)
+    t.set_timing(True)
+    t.step(steps)
+    rep_ms, att_ms, timed = t.timing()
+    t.set_timing(False)
+    late_epoch = min(600, total_epochs)
+    if late_epoch > steps:
+        t.step(late_epoch - steps)
+    state = t.get_state()
+    t.step(total_epochs - max(late_epoch, steps))
+    t.synchronize()
+    final_grid = t.grid_info() if method == "fitsne" else None
+    t.finish()
+    kl_final = t.kl_divergence_pair()
+    log(f"full run ({total_epochs} epochs): KL {kl_final}, final grid {final_grid}")
+
+    # the reference's grid (>= 10 boxes) beside the 50-box floor, same timed epochs
+    ref_grid = None
+    if method == "fitsne" and wl["min_boxes"] != 10:
+        t._set(min_boxes=10)
+        t.initial_embedding(y_seed).prepare(method, 0.5)
+        t.synchronize()
+        s0 = time.perf_counter()
+        t.step(steps)
+        t.synchronize()
+        dt = time.perf_counter() - s0
+        ref_grid = {"epochs_per_s": round(steps / dt, 3), "min_boxes": 10,
+                    "grid": t.grid_info()}
+        t.finish()
+        t._set(min_boxes=wl["min_boxes"])
+        log(f"reference grid (>= 10 boxes): {steps / dt:.1f} epochs/s, {ref_grid['grid']}")
 
     # dominant kernel: the attraction over the column-tiled P; algorithmic bytes per launch are
     # the edges (packed row/column key + value, 8 B each), y read once and the sums written
     alg_bytes = 8 * nnz + 8 * n + 8 * n
-    upd_avg_s = upd_ms / 1e3 / max(timed, 1)
-    achieved = alg_bytes / upd_avg_s / 1e9
+    att_avg_s = att_ms / 1e3 / max(timed, 1)
+    achieved = alg_bytes / att_avg_s / 1e9
+    forked = n >= 32768 and not args.serial_epoch
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic("k_attract") if args.workload == "c3" else None,
                 "algorithmic_bytes": alg_bytes,
-                "kernel": "k_attract_tiled<2,true> (attraction sums over the tiled P; "
-                          "timed while the repulsion runs beside it)",
-                "avg_ms": round(upd_avg_s * 1e3, 4),
-                "repulsion_avg_ms": round(rep_ms / max(timed, 1), 4)}
+                "kernel": "k_attract_tiled<2,true> (attraction sums over the tiled P" +
+                          ("; timed while the repulsion runs beside it)" if forked else ")"),
+                "avg_ms": round(att_avg_s * 1e3, 4),
+                "repulsion_avg_ms": round(rep_ms / max(timed, 1), 4),
+                "timing": f"HIP events on the kernels' streams over a second pass of the same "
+                          f"{timed} epochs from the seed state"}
     if roofline["traffic"] is None:
         roofline["traffic_note"] = ("no counter pass of this kernel on record; see "
                                     "profiles/k_attract_c3_hbm_traffic.md")
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(aff, y_seed, state, wl["min_boxes"], args.cpu_epochs)
+        cpu = cpu_baseline(aff, y_seed, state, late_epoch, wl["min_boxes"], args.cpu_epochs,
+                           method)
     del aff, state, t
     batched = serving_line = None
     if args.workload == "c3" and not args.no_batched:
-        # the metric's second half, on reduced streams: 64 batched jobs against their plain fits,
+        # the metric's second half: the 256 batched jobs against plain fits of every 4th job,
         # and 32 requests at 24/s through the request loop against plain FIFO fits
         sub = argparse.Namespace(**vars(args))
-        sub.jobs, sub.plain_stride, sub.job_epochs = 64, 4, 1000
+        sub.jobs, sub.plain_stride, sub.job_epochs = 256, 4, 1000
         b = bench_batched(sub)
         batched = {k: b[k] for k in ("config", "batched", "plain", "vs_plain", "parity")}
         sub.requests, sub.rate, sub.step_epochs = 32, 24.0, 25
         v = bench_serving(sub)
         serving_line = {k: v[k] for k in ("config", "batched", "plain", "vs_plain", "parity")}
 
+    def kl_obj(pair):
+        return None if pair is None else {"reference_formula": round(pair[0], 6),
+                                          "true": round(pair[1], 6)}
+
     out = {"metric": METRIC, "value": round(eps, 3), "unit": "epochs/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
+           "steps": steps, "warmup": args.warmup,
+           "ms_per_step": round(1e3 * elapsed / steps, 4), "higher_is_better": True,
            "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": wl["name"], "n": n, "d_in": int(x.shape[1]), "d_out": 2,
-                      "perplexity": 30, "epochs": args.steps, "min_boxes": wl["min_boxes"],
+           "config": {"workload": wl["name"], "method": method, "n": n,
+                      "d_in": int(x.shape[1]), "d_out": 2, "perplexity": 30,
+                      "timed_epochs": f"0..{steps} from the seed state (run length "
+                                      f"{total_epochs})",
+                      "min_boxes": wl["min_boxes"] if method == "fitsne" else None,
                       "nnz": nnz, "affinity_build_s": round(t_aff, 3),
-                      "final_grid": grid, "parallelism": "single GPU",
+                      "grid_after_timed_epochs": grid, "parallelism": "single GPU",
+                      "kl_after_timed_epochs": kl_obj(kl_steps),
+                      "kl_full_run": kl_obj(kl_final), "final_grid": final_grid,
+                      "reference_grid": ref_grid,
                       "error_vs_fp64_exact": {
-                          "state": "final (epoch 750), 5000 sampled rows",
+                          "state": f"epoch {steps}, 5000 sampled rows",
                           "gradient_rel_l2": float(f"{errs['gradient']:.4e}"),
                           "repulsion_rel_l2": float(f"{errs['repulsion']:.4e}"),
                           "z_rel": float(f"{errs['z']:.4e}")}},

@@ -18,7 +18,8 @@ from . import _lib
 from ._lib import BhtError, InvalidInput, check, ptr
 
 __all__ = ["Context", "Neighbor", "SparseAffinities", "TSNE", "Batch", "BhtError",
-           "InvalidInput", "knn", "search_beta", "symmetrize", "default_context", "job_footprint"]
+           "InvalidInput", "knn", "search_beta", "symmetrize", "check_affinities",
+           "default_context", "job_footprint"]
 
 
 class Context:
@@ -261,6 +262,9 @@ class TSNE:
         rows = np.ascontiguousarray(aff.rows, np.uint64)
         cols = np.ascontiguousarray(aff.columns, np.uint32)
         vals = np.ascontiguousarray(aff.values, np.float32)
+        if vals.size != cols.size:
+            raise InvalidInput(_lib.BHT_ERR_INVALID, f"affinities have {cols.size} columns but "
+                                                     f"{vals.size} values")
         check(self._L.bht_job_set_affinities(self._h, ptr(rows), ptr(cols), ptr(vals), rows.size,
                                              cols.size, float(aff.perplexity)))
         return self
@@ -414,6 +418,19 @@ def job_footprint(n: int, d_in: int, d_out: int = 2, method: str = "fitsne", **p
 
 
 # ---- operators (the reference's internal steps) -------------------------------------------------
+
+def check_affinities(aff: SparseAffinities) -> None:
+    """The host-side structure check ``with_affinities`` makes (rows[0] == 0, non-decreasing
+    offsets, rows[n] == nnz, every column < n); raises ``InvalidInput``.  Needs no device."""
+    L = _lib.load()
+    rows = np.ascontiguousarray(aff.rows, np.uint64)
+    cols = np.ascontiguousarray(aff.columns, np.uint32)
+    vals = np.asarray(aff.values)
+    if vals.size != cols.size:
+        raise InvalidInput(_lib.BHT_ERR_INVALID, f"affinities have {cols.size} columns but "
+                                                 f"{vals.size} values")
+    check(L.bht_check_affinities(ptr(rows), ptr(cols), rows.size, cols.size))
+
 
 def knn(x, k: int, ctx: Optional[Context] = None, mode: str = "auto", stats: bool = False):
     """Exact kNN, self excluded, ascending (distance, index) (src/tsne/vptree.rs:274-319).

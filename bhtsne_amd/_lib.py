@@ -75,6 +75,7 @@ SIGNATURES = {
                                            C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
     "bht_job_set_affinities": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_uint64, C.c_uint64, C.c_float]),
+    "bht_check_affinities": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64]),
     "bht_job_prepare": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_float]),
     "bht_job_step": (C.c_int32, [C.c_void_p, C.c_uint64, EPOCH_CB, C.c_void_p]),
     "bht_job_finish": (C.c_int32, [C.c_void_p]),

@@ -14,10 +14,17 @@ constexpr int KPL = KMAX / 64;            // values per lane
 
 // ---- bandwidth search ------------------------------------------------------------------------
 //
-// One wave per row.  Every lane evaluates p_j = exp(-beta d_j^2) exactly as the reference (f32).
+// One wave per row.  Every lane evaluates p_j = exp(-beta d_j^2) with the reference's f32
+// argument; exp and log are evaluated in FP64 and rounded to f32, the correctly rounded f32
+// value.  The host libm's expf / logf (which the reference's f32 exp / ln call) return that value
+// for all but about 6 in 10,000 arguments, where they are one ulp off, and a one-ulp change of a
+// single p_j rarely moves the row's entropy, so the bisection takes the reference's steps.
 // mode FP64: the row sum and the entropy are reduced in FP64 across the wave.
 // mode F32_SEQ: lane 0 sums in f32 in the reference's left-to-right order from LDS.
 // beta, its brackets and the bisection steps are f32 as in the reference in both modes.
+__device__ __forceinline__ float exp_f32_rn(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float log_f32_rn(float x) { return (float)log((double)x); }
+
 template <int MODE>
 __global__ void __launch_bounds__(AF_THREADS) k_search_beta(const float *dist, float *p_out,
                                                             int64_t n, int k, float perplexity) {
@@ -36,7 +43,7 @@ __global__ void __launch_bounds__(AF_THREADS) k_search_beta(const float *dist, f
     bool found = false;
     float beta = 1.0f, min_beta = -FLT_MAX, max_beta = FLT_MAX;
     const float tolerance = 1e-5f;
-    const float log_perp = logf(perplexity);
+    const float log_perp = log_f32_rn(perplexity);
     float sum_f = 0.0f;
     double sum_d = 0.0;
     int iteration = 0;
@@ -45,7 +52,7 @@ __global__ void __launch_bounds__(AF_THREADS) k_search_beta(const float *dist, f
 #pragma unroll
         for (int t = 0; t < KPL; t++) {
             int j = lane + 64 * t;
-            p[t] = j < k ? expf(-beta * (d[t] * d[t])) : 0.0f;
+            p[t] = j < k ? exp_f32_rn(-beta * (d[t] * d[t])) : 0.0f;
         }
         if (MODE == 0) {
             double s = 0.0, e = 0.0;
@@ -75,7 +82,7 @@ __global__ void __launch_bounds__(AF_THREADS) k_search_beta(const float *dist, f
                 float entropy = 0.0f;
                 for (int j = 0; j < k; j++)
                     entropy = entropy + beta * sp[w][j] * (sd[w][j] * sd[w][j]);
-                entropy = entropy / s + logf(s);
+                entropy = entropy / s + log_f32_rn(s);
                 df = entropy - log_perp;
                 sf = s;
             }

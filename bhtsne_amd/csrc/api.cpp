@@ -245,6 +245,7 @@ struct bht_job {
     // per-phase timing with events (4 per timed epoch: fork, repulsion done, attraction done,
     // update done)
     bool timing = false;
+    bool timed_fork = false;  // the timed epochs forked (else every phase ran in order)
     std::vector<hipEvent_t> events;
     size_t events_used = 0;
 
@@ -619,6 +620,49 @@ void reorder_points(bht_job *j, const uint32_t *order) {
     j->vals2.reset();
 }
 
+// Structure of an injected CSR (the reference takes it unchecked and would panic on the first
+// out-of-bounds index; here a bad offset or column would be a device memory fault).
+void check_csr(const uint64_t *rows, const uint32_t *cols, uint64_t n_rows_plus_one,
+               uint64_t nnz) {
+    if (n_rows_plus_one == 0) throw Error(ERR_INVALID, "rows must hold n + 1 offsets");
+    if (nnz >= (1ull << 32)) throw Error(ERR_INVALID, "nnz must be below 2^32");
+    if (rows[0] != 0) throw Error(ERR_INVALID, "rows[0] must be 0");
+    if (rows[n_rows_plus_one - 1] != nnz) throw Error(ERR_INVALID, "rows[n] must equal nnz");
+    for (uint64_t i = 1; i < n_rows_plus_one; i++)
+        if (rows[i] < rows[i - 1])
+            throw Error(ERR_INVALID,
+                        "rows must be non-decreasing (row " + std::to_string(i - 1) + ")");
+    const uint64_t n = n_rows_plus_one - 1;
+    for (uint64_t e = 0; e < nnz; e++)
+        if (cols[e] >= n)
+            throw Error(ERR_INVALID, "column " + std::to_string(cols[e]) + " at entry " +
+                                         std::to_string(e) + " is out of range (n = " +
+                                         std::to_string(n) + ")");
+}
+
+// Back to sample order: the current embedding, uy and gains (engine point i is sample perm[i]).
+// The CSR is left to the caller, who replaces it.
+void unpermute_state(bht_job *j) {
+    if (!j->permuted) return;
+    ensure_order_buffers(j);
+    hipStream_t s = j->stream;
+    const int n = (int)j->n, D = (int)j->dim();
+    if (j->y[j->cur].p) {
+        scatter_rows(j->ycur(), j->perm.as<uint32_t>(), n, D, j->y[1 - j->cur].as<float>(), s);
+        std::swap(j->y[0].p, j->y[1].p);
+        std::swap(j->y[0].bytes, j->y[1].bytes);
+    }
+    for (DevBuf *b : {&j->uy, &j->gains}) {
+        if (!b->p) continue;
+        j->ptmp.ensure(j->ctx, j->n * D * 4);
+        scatter_rows(b->as<float>(), j->perm.as<uint32_t>(), n, D, j->ptmp.as<float>(), s);
+        std::swap(b->p, j->ptmp.p);
+        std::swap(b->bytes, j->ptmp.bytes);
+    }
+    BHT_HIP(hipStreamSynchronize(s));
+    j->permuted = false;
+}
+
 // Large jobs (y outgrows an XCD's L2) are renumbered once, before the first epoch.
 constexpr uint64_t REORDER_MIN_POINTS = 32768;
 constexpr uint64_t FORK_MIN_POINTS = 32768;
@@ -696,9 +740,22 @@ void affinities_from_knn(bht_job *j, uint32_t *d_idx, float *d_dist, uint32_t k)
     reset_order(j);
 }
 
+// Neighbours per row the device kernels handle: the kNN from sample vectors keeps a row's
+// candidates in one wave's LDS buffer, the bandwidth search a row's values in one wave.
+constexpr uint32_t MAX_K_DATA = 192;
+constexpr uint32_t MAX_K_NEIGHBORS = 256;
+
+void check_k(uint32_t k, uint32_t cap, const char *what) {
+    if (k > cap)
+        throw Error(ERR_INVALID, "k = " + std::to_string(k) + " neighbours per row (" + what +
+                                     "): at most " + std::to_string(cap) +
+                                     " neighbours per row are supported");
+}
+
 void build_affinities_from_data(bht_job *j) {
     if (!j->have_data) throw Error(ERR_STATE, "no sample vectors: call bht_job_set_data_f32");
     uint32_t k = n_neighbors_of(j);
+    check_k(k, MAX_K_DATA, "3 * perplexity, from sample vectors");
     bht_ctx *c = j->ctx;
     DevBuf idx(c), dist(c), temp(c);
     idx.ensure(c, (size_t)j->n * k * 4);
@@ -718,6 +775,8 @@ void prepare_run(bht_job *j, int method, float theta, bool cached) {
     else ensure_bh_workspace(j);
     const uint64_t len = j->n * j->dim();
     hipStream_t s = j->stream;
+    // P and the data may have been written by host copies or on another stream
+    l2_sync(s);
     // normalise P with the exaggeration folded in (mod.rs:284-293)
     EpochScalars *sc = j->scal.as<EpochScalars>();
     BHT_HIP(hipMemsetAsync(sc, 0, sizeof(EpochScalars), s));
@@ -752,6 +811,7 @@ void prepare_run(bht_job *j, int method, float theta, bool cached) {
     std::vector<float> ones(len, 1.0f);
     BHT_HIP(hipMemcpyAsync(j->gains.p, ones.data(), len * 4, hipMemcpyHostToDevice, s));
     BHT_HIP(hipMemsetAsync(sc, 0, sizeof(EpochScalars), s));
+    l2_sync(s);  // after the seed's host copies
     BHT_HIP(hipStreamSynchronize(s));
     j->method = method;
     j->theta = theta;
@@ -795,6 +855,7 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     // run the same kernels in order on one stream.
     const bool fork = j->n >= FORK_MIN_POINTS && !(j->params.flags & BHT_FLAG_SERIAL_EPOCH);
     hipStream_t rs = fork ? j->side : j->stream;
+    if (j->timing) j->timed_fork = fork;
     for (uint64_t e = 0; e < count; e++) {
         // recentre (+ FIt-SNE grid plan), then the repulsion and the attraction side by side; a
         // forked epoch plans the grid on the repulsion's stream
@@ -802,6 +863,8 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
             center_points(L, j->stream);
             BHT_HIP(hipEventRecord(j->ev_fork, j->stream));
             BHT_HIP(hipStreamWaitEvent(j->side, j->ev_fork, 0));
+            // the repulsion reads the recentred y and the bounds' partials written on `stream`
+            l2_sync(j->side);
             plan_grid(L, fit, j->side);
         } else {
             center_plan(L, fit, j->stream);
@@ -815,6 +878,8 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
         if (fork) {
             BHT_HIP(hipEventRecord(j->ev_join, j->side));
             BHT_HIP(hipStreamWaitEvent(j->stream, j->ev_join, 0));
+            // the update reads F- and Z written on `side`
+            l2_sync(j->stream);
         }
         update_points(L, j->stream);
         j->mark(j->stream);
@@ -831,6 +896,8 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
 
 void finish_run(bht_job *j) {
     if (!j->prepared) throw Error(ERR_STATE, "no prepared run");
+    // a batch member's epochs ran on the batch's stream
+    l2_sync(j->stream);
     recenter(j);
     // The epochs read the tiled copy of P and rescale it (tval) at stop-lying; the CSR values
     // (exports, KL) are rescaled here once the schedule has stopped lying.  If the run ended
@@ -848,13 +915,15 @@ void finish_run(bht_job *j) {
     j->prepared = false;
 }
 
-// Whether the P values on the device still carry the early exaggeration.  During a run the epoch
-// after stop_lying_epoch applies the rescale as it reads P, so until that epoch has run the device
-// values are exaggerated although the reference's schedule has already rescaled them.
+// Whether the epoch's P (what the next epoch's forces use) is the exported affinities times the
+// early exaggeration.  During a run the epoch after stop_lying_epoch applies the rescale as it
+// reads P, so until that epoch has run the device values are exaggerated although the reference's
+// schedule has already rescaled them; with stop_lying_epoch == 0 the export divides P by the
+// exaggeration once more (lib.rs:1024-1029).
 bool device_p_exaggerated(const bht_job *j) {
     const uint64_t sl = j->params.stop_lying_epoch;
-    if (j->prepared) return sl != 0 && j->epoch <= sl + 1;
-    return sl != 0 && !j->stop_lying_fired;
+    if (j->prepared) return sl == 0 || j->epoch <= sl + 1;
+    return !j->stop_lying_fired;
 }
 
 bool has_cached_affinities(const bht_job *j) {  // lib.rs:1192-1196
@@ -897,6 +966,7 @@ void fit_with_neighbors(bht_job *j, int method, float theta, const uint64_t *idx
     if (!ok)
         throw Error(ERR_INVALID,
                     "error: every neighbors row must have the same length, greater than zero.");
+    check_k((uint32_t)std::min<uint64_t>(k, UINT32_MAX), MAX_K_NEIGHBORS, "caller neighbours");
     // cached_affinities_match_neighbors (lib.rs:1201-1220)
     if (j->have_p && j->have_cached_perplexity &&
         j->cached_perplexity == j->params.perplexity) {
@@ -1191,7 +1261,11 @@ uint64_t batch_step(bht_batch *b, uint64_t max_epochs) {
     }
     if (k > 0) {
         const bool rebuilt = b->dirty;
-        if (b->dirty) batch_rebuild(b);
+        if (b->dirty) {
+            batch_rebuild(b);
+            // new members were prepared on their own streams
+            l2_sync(b->stream);
+        }
         for (uint64_t e = 0; e < k; e++) batch_launch_epoch(b, !rebuilt || k >= 64);
         b->epochs_launched += k;
         for (bht_job *j : b->members)
@@ -1408,11 +1482,13 @@ bht_status bht_job_get_affinities(const bht_job *job, uint64_t *rows, uint32_t *
         if (cols && job->nnz) std::memcpy(cols, c.data(), job->nnz * 4);
         if (vals) {
             if (job->nnz) std::memcpy(vals, v.data(), job->nnz * 4);
-            // lib.rs:1023-1029: the export holds P without the exaggeration.  The device CSR values
-            // keep it for the whole of a run that has a stop-lying epoch (the epochs rescale
-            // their tiled copy) and lose it at the end of the run once stop-lying has fired.
-            const uint64_t sl = job->params.stop_lying_epoch;
-            const bool exaggerated = job->prepared ? sl != 0 : sl != 0 && !job->stop_lying_fired;
+            // lib.rs:1023-1029: affinities() divides the stored P by the exaggeration unless
+            // stop-lying has fired, and stop_lying_epoch == 0 never sets the flag (the values it
+            // rescaled up front are divided once more).  During a run the device CSR values keep
+            // the exaggeration (the epochs rescale their tiled copy; the CSR follows at the end
+            // of the run), and stop_lying_fired is set exactly when the stored P lost it, so a
+            // prepared run always exports the device values divided by the exaggeration.
+            const bool exaggerated = job->prepared || !job->stop_lying_fired;
             if (exaggerated) {
                 const float scale = 1.0f / job->params.early_exaggeration;
                 for (uint64_t i = 0; i < job->nnz; i++) vals[i] = vals[i] * scale;
@@ -1427,8 +1503,12 @@ bht_status bht_job_set_affinities(bht_job *job, const uint64_t *rows, const uint
                                   float perplexity) {
     if (!job || !rows || (nnz && (!cols || !vals))) return fail(BHT_ERR_INVALID, "null argument");
     return guarded([&] {
-        if (nnz >= (1ull << 32)) throw Error(ERR_INVALID, "nnz must be below 2^32");
-        if (rows[n_rows_plus_one - 1] != nnz) throw Error(ERR_INVALID, "rows[n] must equal nnz");
+        // The kernels index y by these offsets and columns without bounds checks, so the CSR is
+        // checked on the host before anything is uploaded.
+        check_csr(rows, cols, n_rows_plus_one, nnz);
+        // A renumbered job holds its optimiser state in engine order: bring it back to sample
+        // order first, since the new CSR is in sample order (the reference keeps y as it is).
+        if (job->permuted) unpermute_state(job);
         std::vector<uint32_t> r(n_rows_plus_one);
         for (uint64_t i = 0; i < n_rows_plus_one; i++) r[i] = (uint32_t)rows[i];
         job->rows.ensure(job->ctx, n_rows_plus_one * 4);
@@ -1444,8 +1524,16 @@ bht_status bht_job_set_affinities(bht_job *job, const uint64_t *rows, const uint
         job->have_cached_perplexity = true;
         job->have_p = n_rows_plus_one == job->n + 1 && n_rows_plus_one > 1;
         if (job->have_p) reset_order(job);
+        job->prepared = false;  // a run prepared on the old P is not continued
+        l2_sync(job->stream);
         BHT_HIP(hipStreamSynchronize(job->stream));
     });
+}
+
+bht_status bht_check_affinities(const uint64_t *rows, const uint32_t *cols,
+                                uint64_t n_rows_plus_one, uint64_t nnz) {
+    if (!rows || (nnz && !cols)) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] { check_csr(rows, cols, n_rows_plus_one, nnz); });
 }
 
 bht_status bht_job_prepare(bht_job *job, uint32_t method, float theta) {
@@ -1518,6 +1606,7 @@ bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, cons
         // the state is taken as given: no recentring of an earlier update is pending
         EpochScalars *sc = job->scal.as<EpochScalars>();
         BHT_HIP(hipMemsetAsync(&sc->mean_pending, 0, sizeof(int), job->stream));
+        l2_sync(job->stream);
         BHT_HIP(hipStreamSynchronize(job->stream));
     });
 }
@@ -1563,7 +1652,10 @@ bht_status bht_job_get_timing(bht_job *job, double *out3) {
         for (size_t e = 0; e < epochs; e++) {
             float a = 0.f, b = 0.f;
             BHT_HIP(hipEventElapsedTime(&a, job->events[4 * e], job->events[4 * e + 1]));
-            BHT_HIP(hipEventElapsedTime(&b, job->events[4 * e], job->events[4 * e + 2]));
+            // forked: the attraction is the first launch on its stream after the fork mark;
+            // in order: it follows the repulsion
+            BHT_HIP(hipEventElapsedTime(&b, job->events[4 * e + (job->timed_fork ? 0 : 1)],
+                                        job->events[4 * e + 2]));
             rep += a;
             att += b;
         }

@@ -21,12 +21,16 @@ struct GridPlan {
     float qscale;          // power of two >= max |coordinate|: balances the packed FFT pairs
 };
 
-struct EpochScalars {
-    float mean[2];         // pending recentring shift (applied by the next centring kernel)
+// Two cache-line groups: what the repulsion writes (on the second stream of a forked epoch) and
+// what the rest of the epoch writes, so that the two streams never write into one 128-byte line.
+struct alignas(128) EpochScalars {
+    // -- written by the repulsion's kernels (grid plan, bounds, Z)
     float amin[2], amax[2];// per-axis bounding box after centring (Barnes-Hut)
     GridPlan plan;
     double zsum;           // sum of the per-point Z terms (FP64)
     float z, inv_z;        // Z and 1/Z of this epoch
+    // -- written by the centring, attraction and update kernels and at set-up
+    alignas(128) float mean[2]; // pending recentring shift (applied by the next centring kernel)
     int epoch;             // index of the epoch being executed
     int pad0;
     double psum;           // FP64 sum of P (normalisation)
@@ -191,6 +195,11 @@ struct LaunchSet {
     const uint32_t *tile_seg = nullptr; // segment of each tile
     uint32_t sort_tiles = 0;            // tiles of the segmented sort
 };
+
+// ---- cross-stream cache maintenance (sync.hip) -----------------------------------------------
+// Writes back and invalidates the L2 of every XCD, in stream order on s: the first launch on a
+// stream that consumes data another stream (or a host copy) produced.
+void l2_sync(hipStream_t s);
 
 // ---- scan / sort (sort.hip) ----------------------------------------------------------------
 size_t scan_temp_bytes(uint64_t n);

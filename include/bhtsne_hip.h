@@ -67,7 +67,10 @@ typedef struct bht_params {
     float momentum;                 /* 0.5 */
     float final_momentum;           /* 0.8 */
     float early_exaggeration;       /* 12 */
-    float perplexity;               /* 20 */
+    float perplexity;               /* 20; k = (int)(3 * perplexity) neighbours per row, at most
+                                       192 from sample vectors (perplexity < 64.34) and 256
+                                       from caller neighbours: larger k is rejected with
+                                       BHT_ERR_INVALID ("... neighbours per row ...") */
     uint32_t min_boxes;             /* FIt-SNE boxes per axis lower bound; 10 is the reference's */
     uint32_t max_boxes;             /* FIt-SNE grid capacity per axis, <= 1000 (the reference's) */
     uint32_t flags;                 /* BHT_FLAG_* */
@@ -125,12 +128,20 @@ bht_status bht_job_kl_divergence(bht_job *job, double *reference_kl, double *tru
 
 /* Affinity cache (src/lib.rs:1019-1051): two-phase.  With rows == NULL, *nnz receives the number
  * of entries; otherwise rows (n+1, usize offsets), cols and vals (nnz each) are filled.  Values are
- * un-exaggerated unless stop-lying already fired, as affinities() returns them. */
+ * divided by early_exaggeration unless stop-lying fired, as affinities() returns them (so with
+ * stop_lying_epoch == 0, which never sets the flag, they are divided once more).
+ * set_affinities checks the CSR on the host before any upload (rows[0] == 0, non-decreasing,
+ * rows[n] == nnz, every column < n) and rejects a malformed one with BHT_ERR_INVALID. */
 bht_status bht_job_get_affinities(const bht_job *job, uint64_t *rows, uint32_t *cols, float *vals,
                                   uint64_t *nnz, float *perplexity);
 bht_status bht_job_set_affinities(bht_job *job, const uint64_t *rows, const uint32_t *cols,
                                   const float *vals, uint64_t n_rows_plus_one, uint64_t nnz,
                                   float perplexity);
+/* The structure check set_affinities makes, on its own (host only; no device or context needed).
+ * No counterpart in the reference, whose with_affinities (src/lib.rs:1043-1051) takes the CSR as
+ * it is; for callers that want to vet an affinity graph before handing it over. */
+bht_status bht_check_affinities(const uint64_t *rows, const uint32_t *cols,
+                                uint64_t n_rows_plus_one, uint64_t nnz);
 
 /* ---- stepping interface (the epoch loop of src/lib.rs:961-1014 in pieces) ---------------------
  * prepare: P from data / cached affinities normalised and exaggerated, optimiser reset, embedding
@@ -149,18 +160,21 @@ bht_status bht_job_synchronize(bht_job *job);
 bht_status bht_job_forces(bht_job *job, float *attractive, float *repulsive, float *inverse_z);
 bht_status bht_job_get_state(const bht_job *job, float *y, float *uy, float *gains);
 bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, const float *gains);
-/* Position of the prepared run: the next epoch index, and whether P is still exaggerated (the
- * epoch's P is then the exported affinities times early_exaggeration). */
+/* Position of the prepared run: the next epoch index, and whether the next epoch's P is the
+ * exported affinities times early_exaggeration (while P is still exaggerated, and throughout a
+ * run with stop_lying_epoch == 0, whose export divides P by the exaggeration once more as
+ * affinities() does, src/lib.rs:1024-1029). */
 bht_status bht_job_run_state(const bht_job *job, uint64_t *next_epoch, int *exaggerated);
 /* FIt-SNE grid of the last repulsion: boxes per axis, FFT length, joint min and max. */
 bht_status bht_job_grid_info(const bht_job *job, uint32_t *boxes, uint32_t *fft_len, float *lo,
                              float *hi);
 
-/* Per-phase device time of the epochs run while timing is enabled, measured with HIP events.
- * After the recentring an epoch runs the repulsion and the attraction concurrently (two streams)
- * and then the update.  out[0] = repulsion ms (box sort .. Z, on the second stream), out[1] =
- * attraction ms (the attraction kernel), both from the fork; out[2] = epochs timed.  Enabling
- * resets the counters. */
+/* Per-phase device time of the epochs run while timing is enabled, measured with HIP events
+ * (the events add a little time to each epoch; time throughput with timing off).  After the
+ * recentring an epoch runs the repulsion and the attraction, concurrently on two streams for a
+ * job of 32768 points or more, else in order, and then the update.  out[0] = repulsion ms (box
+ * sort .. Z), out[1] = attraction ms (the attraction kernel alone), summed over the timed
+ * epochs; out[2] = epochs timed.  Enabling resets the counters. */
 bht_status bht_job_set_timing(bht_job *job, int enable);
 bht_status bht_job_get_timing(bht_job *job, double *out3);
 

@@ -306,6 +306,19 @@ def run_epochs(y, uy, gains, rows, cols, vals, *, method="fitsne", theta=0.5, mi
     return mom.value
 
 
+def sparse_kl(y, rows, cols, vals, inverse_z):
+    """sparse_kl_divergence (mod.rs:738-775) with a given 1/Z: (reference-compatible KL in f32,
+    true KL in double)."""
+    y = _f32(y)
+    n, D = y.shape
+    r = C.c_float(0.0)
+    t = C.c_double(0.0)
+    lib().oracle_sparse_kl(y, n, D, np.ascontiguousarray(rows, np.uint64),
+                           np.ascontiguousarray(cols, np.uint32), _f32(vals), float(inverse_z),
+                           C.byref(r), C.byref(t))
+    return r.value, t.value
+
+
 def kl_divergence(y, rows, cols, vals, method="fitsne", theta=0.5, min_boxes=10):
     """(reference-compatible KL, true KL) as kl_divergence() (lib.rs:448-477, mod.rs:635-775)."""
     y = _f32(y)

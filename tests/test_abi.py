@@ -66,3 +66,45 @@ def test_ctx_create_without_gpu_reports_an_error(lib):
     h = ctypes.c_void_p()
     st = L.bht_ctx_create(0, 0, ctypes.byref(h))
     assert st != 0 and L.bht_last_error()
+
+
+def _csr():
+    import numpy as np
+    rows = np.array([0, 2, 3, 5], np.uint64)
+    cols = np.array([1, 2, 0, 0, 1], np.uint32)
+    return rows, cols
+
+
+def test_check_affinities_accepts_a_valid_graph(lib):
+    import bhtsne_amd as bt
+    import numpy as np
+    rows, cols = _csr()
+    bt.check_affinities(bt.SparseAffinities(rows, cols, np.ones(5, np.float32), 3.0))
+
+
+@pytest.mark.parametrize("case", ["empty_rows", "first_offset", "decreasing", "last_offset",
+                                  "column_range", "value_count"])
+def test_check_affinities_rejects_malformed_graphs(lib, case):
+    # the checks with_affinities makes on the host before any upload (no device involved)
+    import bhtsne_amd as bt
+    import numpy as np
+    rows, cols = _csr()
+    vals = np.ones(5, np.float32)
+    if case == "empty_rows":
+        rows = np.zeros(0, np.uint64)
+        cols = cols[:0]
+        vals = vals[:0]
+    elif case == "first_offset":
+        rows = rows.copy()
+        rows[0] = 1
+    elif case == "decreasing":
+        rows = np.array([0, 3, 2, 5], np.uint64)
+    elif case == "last_offset":
+        rows = np.array([0, 2, 3, 4], np.uint64)
+    elif case == "column_range":
+        cols = cols.copy()
+        cols[3] = 3
+    else:
+        vals = vals[:4]
+    with pytest.raises(bt.InvalidInput):
+        bt.check_affinities(bt.SparseAffinities(rows, cols, vals, 3.0))

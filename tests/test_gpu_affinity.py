@@ -68,10 +68,9 @@ def test_search_beta_sequential_mode(bt):
     rel = np.abs(got - ref) / ref.max(axis=1, keepdims=True)
     print("f32-seq max rel", rel.max(), "rows > 1e-5:", int((rel.max(1) > 1e-5).sum()),
           flush=True)
-    # exp/log differ from the host libm by an ulp at most; a row whose entropy lands within that
-    # of the 1e-5 stopping tolerance can stop one bisection step apart
-    assert np.mean(rel.max(1) < 1e-5) > 0.99
-    assert rel.max() < 1e-4
+    # exp and log are correctly rounded f32 on the device (the host libm is one ulp off on ~6 in
+    # 10,000 arguments), sums in the reference's order: every row within 1e-5
+    assert rel.max() < 1e-5
 
 
 def test_search_beta_fp64_mode(bt):
@@ -81,8 +80,10 @@ def test_search_beta_fp64_mode(bt):
     got = bt.search_beta(dist, 30.0, mode=0)
     rel = np.abs(got - ref) / ref.max(axis=1, keepdims=True)
     print("fp64 max rel", rel.max(), "rows > 1e-5:", int((rel.max(1) > 1e-5).sum()), flush=True)
-    # FP64 sums can end the bisection one step apart from the f32 loop near the 1e-5 entropy
-    # tolerance; those rows stay within the bisection's own resolution
+    # FP64 sums (the default, as the FP64 row sums are specified) differ from the f32 loop's by
+    # f32 rounding, so a row whose entropy ends within that of the 1e-5 stopping tolerance stops
+    # one bisection step apart (measured: about 7% of rows); those rows stay within the
+    # bisection's own resolution
     assert np.mean(rel.max(1) < 1e-5) > 0.9
     assert np.mean(rel.max(1) < 1e-4) == 1.0
 

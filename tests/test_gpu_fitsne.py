@@ -215,3 +215,31 @@ def test_one_dimensional_embedding(bt):
     assert _rel(pos, rpos) < 1e-5
     assert _rel(neg, rneg) < 1e-4
     assert abs(inv - rinv) / abs(rinv) < 1e-5
+
+
+def _oracle_run(x_vals, rows, cols, y0, epochs, sl, ms):
+    pv = x_vals.copy()
+    oracle.normalize_p(pv, 12.0)
+    y, uy, g = y0.copy(), np.zeros_like(y0), np.ones_like(y0)
+    oracle.run_epochs(y, uy, g, rows, cols, pv, learning_rate=oracle.auto_learning_rate(
+        y0.shape[0], 12), stop_lying_epoch=sl, momentum_switch_epoch=ms, e0=0, e1=epochs)
+    return y
+
+
+def test_schedule_epoch_tracks_oracle(bt, problem):
+    # stop-lying and the momentum switch at epoch 5 (they fire after epoch 5's update,
+    # src/lib.rs:1162-1174): the GPU's trajectory over 12 epochs follows the oracle's with the
+    # same schedule, and is far from the oracle's with the schedule one epoch early or late
+    x, rows, cols, vals = problem
+    n = x.shape[0]
+    y0 = (np.random.default_rng(7).standard_normal((n, 2)) * 1e-2).astype(np.float32)
+    t = _job(bt, x, rows, cols, vals, y0, stop_lying_epoch=5, momentum_switch_epoch=5)
+    t.step(12)
+    gy, _, _ = t.get_state()
+    e_same = _rel(gy, _oracle_run(vals, rows, cols, y0, 12, 5, 5))
+    e_early = _rel(gy, _oracle_run(vals, rows, cols, y0, 12, 4, 4))
+    e_late = _rel(gy, _oracle_run(vals, rows, cols, y0, 12, 6, 6))
+    print(f"12 epochs, schedule at 5: same {e_same:.2e}, one early {e_early:.2e}, one late "
+          f"{e_late:.2e}", flush=True)
+    assert e_same < 1e-4
+    assert e_early > 100 * e_same and e_late > 100 * e_same

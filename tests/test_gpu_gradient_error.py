@@ -52,8 +52,16 @@ def test_gradient_error_c1_barnes_hut(bt, epochs):
     err, ref = _errors(t, "bh")
     print(f"C1 Barnes-Hut theta 0.5, N={n}, epoch {epochs}: gradient rel L2 error vs FP64 exact "
           f"GPU {err:.3e}, oracle {ref:.3e}", flush=True)
-    # equal errors, up to the f32 noise of the two evaluations (their forces agree to ~1e-5)
-    assert abs(err - ref) <= 0.02 * ref + 1e-4
+    _check(err, ref)
+
+
+def _check(err, ref):
+    # the GPU gradient is no further from the exact FP64 gradient than the reference algorithm
+    # evaluated in f32 (the oracle) is; where the approximation's own error dominates the f32
+    # noise of either evaluation (above 1e-3), the two errors are equal to 2%
+    assert err <= 1.02 * ref
+    if ref > 1e-3:
+        assert abs(err - ref) <= 0.02 * ref
 
 
 @pytest.mark.parametrize("epochs", [50, 300])
@@ -66,5 +74,4 @@ def test_gradient_error_c2_fitsne(bt, epochs):
     err, ref = _errors(t, "fitsne")
     print(f"C2 FIt-SNE, N={n}, epoch {epochs}: gradient rel L2 error vs FP64 exact (5k rows) "
           f"GPU {err:.3e}, oracle {ref:.3e}", flush=True)
-    # equal errors, up to the f32 noise of the two evaluations (their forces agree to ~1e-5)
-    assert abs(err - ref) <= 0.02 * ref + 1e-4
+    _check(err, ref)
