@@ -43,6 +43,8 @@ def lib():
     sig = {
         "oracle_set_threads": (C.c_int, [C.c_int]),
         "oracle_knn": (None, [f32p, C.c_int64, C.c_int, C.c_int, u32p, f32p]),
+        "oracle_knn_rows": (None, [f32p, C.c_int64, C.c_int, C.c_int, i64p, C.c_int64, u32p,
+                                   f32p]),
         "oracle_search_beta_row": (C.c_int, [f32p, f32p, C.c_int, C.c_float]),
         "oracle_search_beta_row_f64": (C.c_int, [f64p, f64p, C.c_int, C.c_double]),
         "oracle_search_beta": (None, [f32p, f32p, C.c_int64, C.c_int, C.c_float]),
@@ -63,6 +65,7 @@ def lib():
         "oracle_fitsne_repulsion": (None, [f32p, C.c_int64, C.c_int, C.c_int, f32p,
                                            C.POINTER(C.c_float), f64p]),
         "oracle_set_fft_f32": (None, [C.c_int]),
+        "oracle_set_cr_explog": (None, [C.c_int]),
         "oracle_fitsne_repulsion_f64": (None, [f32p, C.c_int64, C.c_int, C.c_int, f64p,
                                                C.POINTER(C.c_double)]),
         "oracle_morton_encode2": (C.c_uint64, [C.c_uint32, C.c_uint32]),
@@ -110,6 +113,17 @@ def knn(x: np.ndarray, k: int):
     idx = np.zeros((n, k), np.uint32)
     dist = np.zeros((n, k), np.float32)
     lib().oracle_knn(x, n, d, k, idx, dist)
+    return idx, dist
+
+
+def knn_rows(x: np.ndarray, k: int, sel):
+    """Brute-force kNN of the rows `sel` only (same order and arithmetic as knn)."""
+    x = _f32(x)
+    n, d = x.shape
+    sel = np.ascontiguousarray(sel, np.int64)
+    idx = np.zeros((len(sel), k), np.uint32)
+    dist = np.zeros((len(sel), k), np.float32)
+    lib().oracle_knn_rows(x, n, d, k, sel, len(sel), idx, dist)
     return idx, dist
 
 
@@ -193,6 +207,17 @@ def fitsne_repulsion(y, min_boxes=10):
     info = np.zeros(4, np.float64)
     lib().oracle_fitsne_repulsion(y, n, D, int(min_boxes), neg, C.byref(z), info)
     return neg, z.value, dict(boxes=int(info[0]), fft_len=int(info[1]), min=info[2], max=info[3])
+
+
+class cr_explog:
+    """Context manager: the bandwidth search's exp / ln correctly rounded (as the device computes
+    them) instead of the host libm's expf / logf (as the reference calls them)."""
+
+    def __enter__(self):
+        lib().oracle_set_cr_explog(1)
+
+    def __exit__(self, *exc):
+        lib().oracle_set_cr_explog(0)
 
 
 class f32_fft:

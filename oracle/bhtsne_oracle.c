@@ -66,8 +66,65 @@ static int pair_less(float da, uint32_t ia, float db, uint32_t ib) {
     return da < db || (da == db && ia < ib);
 }
 
-/* k nearest neighbours of every row, self excluded, ascending (distance, index).  Equivalent to
- * the reference's "search k+1, drop self" (vptree.rs:305-318) under a (distance, index) order. */
+/* k nearest neighbours of row i, self excluded, ascending (distance, index), into idx / dist
+ * (k each); hd / hi are k-entry scratch.  Equivalent to the reference's "search k+1, drop self"
+ * (vptree.rs:305-318) under a (distance, index) order. */
+static void knn_row(const float *x, int64_t n, int d, int k, int64_t i, float *hd, uint32_t *hi,
+                    uint32_t *idx, float *dist) {
+    int cnt = 0;
+    /* max-heap on (dist, idx) of the best k so far */
+    for (int64_t j = 0; j < n; j++) {
+        if (j == i) continue;
+        float dj = euclid_f32(x + (size_t)j * d, x + (size_t)i * d, d);
+        if (cnt < k) {
+            int c = cnt++;
+            hd[c] = dj;
+            hi[c] = (uint32_t)j;
+            while (c > 0) {
+                int p = (c - 1) / 2;
+                if (pair_less(hd[p], hi[p], hd[c], hi[c])) {
+                    float tf = hd[p]; hd[p] = hd[c]; hd[c] = tf;
+                    uint32_t ti = hi[p]; hi[p] = hi[c]; hi[c] = ti;
+                    c = p;
+                } else break;
+            }
+        } else if (pair_less(dj, (uint32_t)j, hd[0], hi[0])) {
+            hd[0] = dj;
+            hi[0] = (uint32_t)j;
+            int c = 0;
+            for (;;) {
+                int l = 2 * c + 1, r = l + 1, m = c;
+                if (l < k && pair_less(hd[m], hi[m], hd[l], hi[l])) m = l;
+                if (r < k && pair_less(hd[m], hi[m], hd[r], hi[r])) m = r;
+                if (m == c) break;
+                float tf = hd[m]; hd[m] = hd[c]; hd[c] = tf;
+                uint32_t ti = hi[m]; hi[m] = hi[c]; hi[c] = ti;
+                c = m;
+            }
+        }
+    }
+    /* heap-sort ascending */
+    for (int end = cnt - 1; end > 0; end--) {
+        float tf = hd[0]; hd[0] = hd[end]; hd[end] = tf;
+        uint32_t ti = hi[0]; hi[0] = hi[end]; hi[end] = ti;
+        int c = 0;
+        for (;;) {
+            int l = 2 * c + 1, r = l + 1, m = c;
+            if (l < end && pair_less(hd[m], hi[m], hd[l], hi[l])) m = l;
+            if (r < end && pair_less(hd[m], hi[m], hd[r], hi[r])) m = r;
+            if (m == c) break;
+            float t2 = hd[m]; hd[m] = hd[c]; hd[c] = t2;
+            uint32_t u2 = hi[m]; hi[m] = hi[c]; hi[c] = u2;
+            c = m;
+        }
+    }
+    for (int t = 0; t < k; t++) {
+        idx[t] = t < cnt ? hi[t] : 0u;
+        dist[t] = t < cnt ? hd[t] : 0.0f;
+    }
+}
+
+/* k nearest neighbours of every row (brute force). */
 EXPORT void oracle_knn(const float *x, int64_t n, int d, int k, uint32_t *idx_out,
                        float *dist_out) {
 #pragma omp parallel
@@ -75,59 +132,24 @@ EXPORT void oracle_knn(const float *x, int64_t n, int d, int k, uint32_t *idx_ou
         float *hd = (float *)malloc(sizeof(float) * (size_t)k);
         uint32_t *hi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)k);
 #pragma omp for schedule(dynamic, 16)
-        for (int64_t i = 0; i < n; i++) {
-            int cnt = 0;
-            /* max-heap on (dist, idx) of the best k so far */
-            for (int64_t j = 0; j < n; j++) {
-                if (j == i) continue;
-                float dj = euclid_f32(x + (size_t)j * d, x + (size_t)i * d, d);
-                if (cnt < k) {
-                    int c = cnt++;
-                    hd[c] = dj;
-                    hi[c] = (uint32_t)j;
-                    while (c > 0) {
-                        int p = (c - 1) / 2;
-                        if (pair_less(hd[p], hi[p], hd[c], hi[c])) {
-                            float tf = hd[p]; hd[p] = hd[c]; hd[c] = tf;
-                            uint32_t ti = hi[p]; hi[p] = hi[c]; hi[c] = ti;
-                            c = p;
-                        } else break;
-                    }
-                } else if (pair_less(dj, (uint32_t)j, hd[0], hi[0])) {
-                    hd[0] = dj;
-                    hi[0] = (uint32_t)j;
-                    int c = 0;
-                    for (;;) {
-                        int l = 2 * c + 1, r = l + 1, m = c;
-                        if (l < k && pair_less(hd[m], hi[m], hd[l], hi[l])) m = l;
-                        if (r < k && pair_less(hd[m], hi[m], hd[r], hi[r])) m = r;
-                        if (m == c) break;
-                        float tf = hd[m]; hd[m] = hd[c]; hd[c] = tf;
-                        uint32_t ti = hi[m]; hi[m] = hi[c]; hi[c] = ti;
-                        c = m;
-                    }
-                }
-            }
-            /* heap-sort ascending */
-            for (int end = cnt - 1; end > 0; end--) {
-                float tf = hd[0]; hd[0] = hd[end]; hd[end] = tf;
-                uint32_t ti = hi[0]; hi[0] = hi[end]; hi[end] = ti;
-                int c = 0;
-                for (;;) {
-                    int l = 2 * c + 1, r = l + 1, m = c;
-                    if (l < end && pair_less(hd[m], hi[m], hd[l], hi[l])) m = l;
-                    if (r < end && pair_less(hd[m], hi[m], hd[r], hi[r])) m = r;
-                    if (m == c) break;
-                    float t2 = hd[m]; hd[m] = hd[c]; hd[c] = t2;
-                    uint32_t u2 = hi[m]; hi[m] = hi[c]; hi[c] = u2;
-                    c = m;
-                }
-            }
-            for (int t = 0; t < k; t++) {
-                idx_out[(size_t)i * k + t] = t < cnt ? hi[t] : 0u;
-                dist_out[(size_t)i * k + t] = t < cnt ? hd[t] : 0.0f;
-            }
-        }
+        for (int64_t i = 0; i < n; i++)
+            knn_row(x, n, d, k, i, hd, hi, idx_out + (size_t)i * k, dist_out + (size_t)i * k);
+        free(hd);
+        free(hi);
+    }
+}
+
+/* The same for the m rows sel[0..m) only (sampled checks at sizes where all rows would take too
+ * long); output row t is row sel[t]. */
+EXPORT void oracle_knn_rows(const float *x, int64_t n, int d, int k, const int64_t *sel,
+                            int64_t m, uint32_t *idx_out, float *dist_out) {
+#pragma omp parallel
+    {
+        float *hd = (float *)malloc(sizeof(float) * (size_t)k);
+        uint32_t *hi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)k);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t t = 0; t < m; t++)
+            knn_row(x, n, d, k, sel[t], hd, hi, idx_out + (size_t)t * k, dist_out + (size_t)t * k);
         free(hd);
         free(hi);
     }
@@ -138,6 +160,16 @@ EXPORT void oracle_knn(const float *x, int64_t n, int d, int k, uint32_t *idx_ou
 /* ------------------------------------------------------------------------------------------ */
 
 /* One row, f32 exactly as written in the reference. Returns the number of bisection steps. */
+/* exp and ln of the bandwidth search: the host libm's expf / logf (what the reference's f32 exp()
+ * and ln() call), or, when cr_explog is set, the correctly rounded values ((float) of the double
+ * routine), which differ from the libm's by one ulp on ~6 in 10,000 arguments.  The second form
+ * is what the device computes, so it pins the device kernel bit for bit; the first is the
+ * reference. */
+static int g_cr_explog = 0;
+EXPORT void oracle_set_cr_explog(int on) { g_cr_explog = on; }
+static float sb_exp(float x) { return g_cr_explog ? (float)exp((double)x) : expf(x); }
+static float sb_log(float x) { return g_cr_explog ? (float)log((double)x) : logf(x); }
+
 EXPORT int oracle_search_beta_row(float *p, const float *d, int k, float perplexity) {
     int found = 0;
     float beta = 1.0f;
@@ -147,16 +179,16 @@ EXPORT int oracle_search_beta_row(float *p, const float *d, int k, float perplex
     int iteration = 0;
     float sum = 0.0f;
     const float two = 2.0f;
-    const float log_perp = logf(perplexity);
+    const float log_perp = sb_log(perplexity);
 
     while (!found && iteration < 200) {
-        for (int j = 0; j < k; j++) p[j] = expf(-beta * (d[j] * d[j]));
+        for (int j = 0; j < k; j++) p[j] = sb_exp(-beta * (d[j] * d[j]));
         sum = 0.0f;
         for (int j = 0; j < k; j++) sum = sum + p[j];
         sum = sum + FLT_MIN;
         float entropy = 0.0f;
         for (int j = 0; j < k; j++) entropy = entropy + beta * p[j] * (d[j] * d[j]);
-        entropy = entropy / sum + logf(sum);
+        entropy = entropy / sum + sb_log(sum);
         float diff = entropy - log_perp;
         if (diff < tolerance && -diff < tolerance) {
             found = 1;

@@ -68,9 +68,15 @@ def test_search_beta_sequential_mode(bt):
     rel = np.abs(got - ref) / ref.max(axis=1, keepdims=True)
     print("f32-seq max rel", rel.max(), "rows > 1e-5:", int((rel.max(1) > 1e-5).sum()),
           flush=True)
-    # exp and log are correctly rounded f32 on the device (the host libm is one ulp off on ~6 in
-    # 10,000 arguments), sums in the reference's order: every row within 1e-5
-    assert rel.max() < 1e-5
+    # the device evaluates exp and ln correctly rounded: with the oracle doing the same, every
+    # value is equal bit for bit (sums in the reference's order, the same bisection)
+    with oracle.cr_explog():
+        cr = oracle.search_beta(dist, 30.0)
+    assert np.array_equal(got, cr)
+    # the host libm's expf / logf are one ulp off on ~6 in 10,000 arguments; that moves the
+    # bisection of about one row in 2,000 by a step (measured on 2,000 rows of the 1M workload)
+    assert np.mean(rel.max(1) < 1e-5) >= 0.999
+    assert rel.max() < 2e-4
 
 
 def test_search_beta_fp64_mode(bt):
