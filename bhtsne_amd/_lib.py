@@ -98,6 +98,9 @@ SIGNATURES = {
     "bht_job_run_state": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
     "bht_exact_repulsion_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                             C.c_void_p, C.POINTER(C.c_double)]),
+    "bht_exact_repulsion_f64_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64,
+                                                    C.c_uint32, C.c_void_p, C.c_void_p,
+                                                    C.c_void_p]),
     "bht_batch_create": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "bht_batch_destroy": (None, [C.c_void_p]),
     "bht_batch_add": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float]),

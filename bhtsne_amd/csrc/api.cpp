@@ -1333,6 +1333,7 @@ bht_status bht_ctx_create(int device, uint64_t hbm_budget_bytes, bht_ctx **out) 
         BHT_HIP(hipSetDevice(device));
         BHT_HIP(hipStreamCreate(&ctx->stream));
         build_twiddles(ctx.get());
+        exact_scratch_init(device);
         *out = ctx.release();
     });
 }
@@ -1744,6 +1745,20 @@ bht_status bht_exact_repulsion_f64(bht_ctx *ctx, const float *y, uint64_t n, uin
         double z = 0.0;
         for (double v : zp) z += v;
         *z_out = z;
+    });
+}
+
+bht_status bht_exact_repulsion_f64_noalloc(bht_ctx *ctx, const float *y_dev, uint64_t n,
+                                           uint32_t d, double *neg_dev, double *z_dev,
+                                           void *stream) {
+    if (!ctx || !y_dev || !neg_dev || !z_dev) return fail(BHT_ERR_INVALID, "null argument");
+    if (d != 1 && d != 2) return fail(BHT_ERR_INVALID, "d must be 1 or 2");
+    if (n == 0 || n > exact_noalloc_max_points())
+        return fail(BHT_ERR_INVALID, "n must be in [1, 16777216] without allocation");
+    return guarded([&] {
+        BHT_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+        exact_repulsion_f64_noalloc(ctx->device, y_dev, (int64_t)n, (int)d, neg_dev, z_dev, s);
     });
 }
 

@@ -7,6 +7,9 @@
 // One thread per point; the other points stream through LDS in tiles of 256.  Every point's
 // terms are summed in tile order and Z is combined from per-block partials on the host in block
 // order, so the result is deterministic.
+#include <map>
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace bht {
@@ -51,6 +54,60 @@ __global__ void __launch_bounds__(EX_THREADS) k_exact_rep(const float *y, int n,
 }
 
 int exact_repulsion_blocks(int64_t n) { return (int)ceil_div((uint64_t)(n > 0 ? n : 1), EX_THREADS); }
+
+// ---- the variant that allocates nothing ---------------------------------------------------------
+//
+// Its per-block partials of Z live in `scratch`, a file-scope device array (each GPU holds its
+// own copy for the life of the process), instead of a buffer the call allocates.  In host code
+// the array's name only stands for the symbol: its device address is looked up once per device
+// (exact_scratch_init, with that device current) and only that address is handed to kernels.
+// Every call on one GPU shares the array, so calls that overlap on one GPU race; every partial is
+// written before it is read, so nothing needs clearing between calls.
+constexpr int NOALLOC_MAX_BLOCKS = 65536;  // 512 KiB: n <= 65536 * 256 = 16,777,216 points
+static __device__ __align__(256) double scratch[NOALLOC_MAX_BLOCKS];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_of_device;
+
+// Z = the partials summed in block order by one thread (the order the allocating entry point
+// sums them in on the host, so both give the same bits).
+__global__ void k_sum_partials(const double *part, int nparts, double *z_out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double z = 0.0;
+    for (int b = 0; b < nparts; b++) z += part[b];
+    *z_out = z;
+}
+}  // namespace
+
+uint64_t exact_noalloc_max_points() { return (uint64_t)NOALLOC_MAX_BLOCKS * EX_THREADS; }
+
+void exact_scratch_init(int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_of_device.count(device)) return;
+    BHT_HIP(hipSetDevice(device));
+    void *ptr = nullptr;
+    BHT_HIP(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_of_device[device] = (double *)ptr;
+}
+
+void exact_repulsion_f64_noalloc(int device, const float *y, int64_t n, int dim, double *neg,
+                                 double *z_out, hipStream_t s) {
+    double *part = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        auto it = g_scratch_of_device.find(device);
+        if (it == g_scratch_of_device.end())
+            throw Error(ERR_STATE, "the device's scratch was not set up (bht_ctx_create)");
+        part = it->second;
+    }
+    const int nb = exact_repulsion_blocks(n);
+    if (nb > NOALLOC_MAX_BLOCKS)
+        throw Error(ERR_INVALID, "at most 16777216 points without allocation");
+    exact_repulsion_f64(y, n, dim, neg, part, s);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, part, nb, z_out);
+    BHT_LAUNCH_CHECK();
+}
 
 void exact_repulsion_f64(const float *y, int64_t n, int dim, double *neg, double *zpart,
                          hipStream_t s) {

@@ -271,5 +271,12 @@ void sum_values_f64(const float *vals, uint64_t nnz, EpochScalars *scal, double 
 int exact_repulsion_blocks(int64_t n);
 void exact_repulsion_f64(const float *y, int64_t n, int dim, double *neg, double *zpart,
                          hipStream_t s);
+// The same with its partials in the file's static device scratch (no allocation): y, neg and
+// z_out are device pointers; Z (one double) is written to z_out on the device.
+// exact_scratch_init(device) looks the scratch up once per device (at context creation).
+void exact_scratch_init(int device);
+uint64_t exact_noalloc_max_points();
+void exact_repulsion_f64_noalloc(int device, const float *y, int64_t n, int dim, double *neg,
+                                 double *z_out, hipStream_t s);
 
 }  // namespace bht

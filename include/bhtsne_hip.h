@@ -198,6 +198,17 @@ bht_status bht_knn_f32_ex(bht_ctx *ctx, const float *x, uint64_t n, uint32_t d_i
  * y: n * d f32 (d = 1 or 2); neg_out: n * d doubles. */
 bht_status bht_exact_repulsion_f64(bht_ctx *ctx, const float *y, uint64_t n, uint32_t d,
                                    double *neg_out, double *z_out);
+/* The same computation without any allocation, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  y_dev (n * d f32), neg_dev (n * d doubles) and z_dev (one double) are DEVICE
+ * pointers on the context's device; the call only enqueues work on `stream` (a hipStream_t; NULL:
+ * the context's stream) and returns.  The per-block partials of Z live in a static device array
+ * of 512 KiB that each GPU holds for the life of the process, so n <= 16,777,216 (larger n is
+ * rejected with BHT_ERR_INVALID), and calls that overlap on one GPU race.  The results equal
+ * bht_exact_repulsion_f64's bit for bit. */
+bht_status bht_exact_repulsion_f64_noalloc(bht_ctx *ctx, const float *y_dev, uint64_t n,
+                                           uint32_t d, double *neg_dev, double *z_dev,
+                                           void *stream);
 /* Per-row bandwidth search: conditional P (n * k) from neighbour distances (n * k). */
 bht_status bht_search_beta_f32(bht_ctx *ctx, const float *dist, uint64_t n, uint32_t k,
                                float perplexity, uint32_t mode, float *p_out);
