@@ -338,8 +338,7 @@ class TSNE:
 class Batch:
     """Many independent TSNE jobs advanced together: every epoch kernel is one launch for all
     members of a method / d_out, replayed from a captured graph.  Each member's result equals its
-    fit alone (``TSNE.fit_sne`` / ``barnes_hut``) bit for bit when that fit runs its epochs in
-    order on one stream (``BHT_FLAG_SERIAL_EPOCH``, as the batch does).  Jobs join with ``add``
+    fit alone (``TSNE.fit_sne`` / ``barnes_hut``) bit for bit.  Jobs join with ``add``
     (which builds their affinities unless cached) and leave when their ``epochs`` are done."""
 
     def __init__(self, ctx: Optional[Context] = None, graph: bool = True):

@@ -82,9 +82,9 @@ typedef struct bht_params {
  * sample order).  This flag keeps the input order. */
 #define BHT_FLAG_NO_REORDER 1u
 /* Jobs of 32768 points or more run each epoch's repulsion on a second stream beside the
- * attraction.  This flag keeps every epoch in order on the job's stream: the same kernels, run to
- * run bit-reproducible (runs with the two streams have differed in the last bits now and then),
- * and the order a batch runs its members in. */
+ * attraction (with the cache maintenance the handover between streams needs, the results are
+ * the same bits as in order).  This flag keeps every epoch in order on the job's stream, the
+ * order a batch runs its members in. */
 #define BHT_FLAG_SERIAL_EPOCH 2u
 
 void bht_params_default(bht_params *p);
