@@ -243,6 +243,9 @@ def main():
                     help="batched workload: plain launches instead of the captured epoch graph")
     ap.add_argument("--step-epochs", type=int, default=25,
                     help="serving workload: epochs between admissions")
+    ap.add_argument("--short", action="store_true",
+                    help="only the timed epochs (for profiler runs): no error report, per-phase "
+                         "pass, full run, reference-grid run, CPU baseline or batched lines")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -305,6 +308,11 @@ def main():
     elapsed = time.perf_counter() - start
     eps = steps / elapsed
     grid = t.grid_info() if method == "fitsne" else None
+    if args.short:
+        log(f"{steps} epochs in {elapsed:.4f}s -> {eps:.1f} epochs/s; grid {grid}")
+        print(json.dumps({"metric": METRIC, "value": round(eps, 3), "unit": "epochs/s",
+                          "steps": steps, "short": True}), flush=True)
+        return
     # the state after the timed epochs against the exact FP64 gradient (all-pairs repulsion and Z
     # on the device, edge terms in FP64 on a 5,000-row sample)
     t0 = time.perf_counter()
