@@ -48,6 +48,9 @@ constexpr int ATT_QUEUE_BLOCKS = 416;
 // that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran at half speed)
 constexpr int WIN_FLOATS = 11800;
 constexpr uint32_t NO_ROW = 0xffffu;
+// the accumulators of a block's rows as (x, y) pairs (y unused for D = 1), then a dummy pair
+// per lane (adds that change no row)
+constexpr uint32_t ACC_DUMMY = TILE_ROWS_MAX * 2;
 
 typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -247,12 +250,66 @@ __device__ __forceinline__ EdgeLoad load_edges(BHT_GLOBAL const uint32_t *tkey,
     return r;
 }
 
-// Adds v to the accumulator of row r (exclusive to the calling lane).
+// acc[at[u]] += (v0[u], v1[u]) for the lane's eight adds: all reads, then all writes (the eight
+// addresses are distinct rows or the lane's dummy slot; accumulators are stored as pairs).
 template <int D>
-__device__ __forceinline__ void acc_add(float *acc, uint32_t r, float v0, float v1) {
-    BHT_LDS float *a = lds_view(acc);
-    a[r * D] += v0;
-    if (D == 2) a[r * D + 1] += v1;
+__device__ __forceinline__ void acc_add8(float *acc, const uint32_t *at, const float *v0,
+                                         const float *v1) {
+    BHT_LDS f2v *a = (BHT_LDS f2v *)lds_view(acc);
+    f2v cur[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) cur[u] = a[at[u] >> 1];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        cur[u].x = cur[u].x + v0[u];
+        if (D == 2) cur[u].y = cur[u].y + v1[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[at[u] >> 1] = cur[u];
+}
+
+// DPP controls (wave64): shifts within rows of 16 lanes, broadcasts of a row's last lane to the
+// rows after it, and whole-wave shifts by one lane.
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114,
+              DPP_ROW_SHR8 = 0x118, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143,
+              DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f(float old, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL,
+                                                      ROW_MASK, 0xf, false));
+}
+
+// One step of a segmented inclusive scan (sums restart where flag is set): each lane combines
+// the value DPP moves to it from the lane `CTRL` names (none: -0.0 and no flag, which change
+// nothing) with its own.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void seg_step(float &s0, float &s1, int &flag) {
+    const float t0 = dpp_f<CTRL, ROW_MASK>(-0.0f, s0);
+    const float t1 = dpp_f<CTRL, ROW_MASK>(-0.0f, s1);
+    const int tf = __builtin_amdgcn_update_dpp(0, flag, CTRL, ROW_MASK, 0xf, false);
+    if (!flag) {
+        s0 = t0 + s0;
+        s1 = t1 + s1;
+    }
+    flag |= tf;
+}
+
+// Segmented inclusive scan of (s0, s1) over the wave: within rows of 16 lanes, then across rows.
+// It stops as soon as every lane's segment is complete.
+__device__ __forceinline__ void seg_scan(float &s0, float &s1, int &flag) {
+    if (__builtin_amdgcn_ballot_w64(flag == 0) == 0) return;
+    seg_step<DPP_ROW_SHR1, 0xf>(s0, s1, flag);
+    if (__builtin_amdgcn_ballot_w64(flag == 0) == 0) return;
+    seg_step<DPP_ROW_SHR2, 0xf>(s0, s1, flag);
+    if (__builtin_amdgcn_ballot_w64(flag == 0) == 0) return;
+    seg_step<DPP_ROW_SHR4, 0xf>(s0, s1, flag);
+    if (__builtin_amdgcn_ballot_w64(flag == 0) == 0) return;
+    seg_step<DPP_ROW_SHR8, 0xf>(s0, s1, flag);
+    if (__builtin_amdgcn_ballot_w64(flag == 0) == 0) return;
+    seg_step<DPP_ROW_BCAST15, 0xa>(s0, s1, flag);
+    if (__builtin_amdgcn_ballot_w64(flag == 0) == 0) return;
+    seg_step<DPP_ROW_BCAST31, 0xc>(s0, s1, flag);
 }
 
 template <int D, bool EPOCH>
@@ -261,7 +318,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                                                              float *pos_out, int queued) {
     __shared__ __align__(16) float win[WIN_FLOATS];
     __shared__ __align__(16) float yrow[TILE_ROWS_MAX * D];
-    __shared__ __align__(16) float acc[TILE_ROWS_MAX * D];
+    __shared__ __align__(16) float acc[TILE_ROWS_MAX * 2 + 64 * 2];  // (x, y) pairs
     __shared__ uint32_t next_lb;
     const BlockCtx bc = block_ctx(map, jobs, F_ATT);
     const JobDev &J = jobs[bc.job];
@@ -295,7 +352,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
         const uint32_t row0 = lb * R;
         const uint32_t nrows = row0 < n ? (n - row0 < R ? n - row0 : R) : 0u;
         stage_points<D>(y, row0, nrows, yrow);
-        for (uint32_t k = threadIdx.x; k < nrows * D; k += T_THREADS) lds_view(acc)[k] = 0.0f;
+        for (uint32_t k = threadIdx.x; k < nrows * 2; k += T_THREADS) lds_view(acc)[k] = 0.0f;
         const uint32_t s_begin = blk_seg[lb], s_end = blk_seg[lb + 1];
         for (uint32_t s = s_begin; s < s_end; s++) {
             const uint32_t t = seg_tile[s];
@@ -338,20 +395,16 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                     EdgeLoad nxt = {};
                     if (more) nxt = load_edges(tkey, tval, n_base + 8u * lane, nnz);
                     const uint32_t e0 = base + 8u * lane;
-                    uint32_t key[8];
+                    // edges of this group only; the others get no row and no weight
+                    const uint32_t span = e_end - e_begin;
+                    uint32_t row[8], col[8];
                     float pv[8];
 #pragma unroll
                     for (int u = 0; u < 8; u++) {
-                        key[u] = cur.key[u];
-                        pv[u] = cur.pv[u];
-                    }
-                    // edges of this group only; the others get no row
-                    uint32_t row[8];
-#pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const bool ok = e0 + u >= e_begin && e0 + u < e_end;
-                        row[u] = ok ? key[u] >> 16 : NO_ROW;
-                        if (!ok) pv[u] = 0.0f;
+                        const bool ok = e0 + (uint32_t)u - e_begin < span;
+                        row[u] = ok ? cur.key[u] >> 16 : NO_ROW;
+                        col[u] = ok ? cur.key[u] & 0xffffu : 0u;
+                        pv[u] = ok ? cur.pv[u] : 0.0f;
                     }
                     if (stop_now) {
 #pragma unroll
@@ -360,70 +413,60 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                             if (row[u] != NO_ROW) tval[e0 + u] = pv[u];
                         }
                     }
+                    // p q (y_i - y_j), q = 1 / (1 + |y_i - y_j|^2) (hardware reciprocal)
                     float f0[8], f1[8];
 #pragma unroll
                     for (int u = 0; u < 8; u++) {
                         const float2 yi = lds_pos<D>(yrow, row[u] != NO_ROW ? row[u] : 0u);
-                        const float2 yj = lds_pos<D>(win, row[u] != NO_ROW ? key[u] & 0xffffu : 0u);
+                        const float2 yj = lds_pos<D>(win, col[u]);
                         const float d0 = yi.x - yj.x;
                         const float d1 = D == 2 ? yi.y - yj.y : 0.0f;
-                        float dist = 0.0f;
-                        dist += d0 * d0;
-                        if (D == 2) dist += d1 * d1;
-                        const float f = pv[u] / (dist + 1.0f);
+                        float dist = d0 * d0;
+                        if (D == 2) dist = dist + d1 * d1;
+                        const float f = pv[u] * __builtin_amdgcn_rcpf(dist + 1.0f);
                         f0[u] = f * d0;
                         f1[u] = f * d1;
                     }
-                    // runs of equal rows within the lane: the head run (from edge 0), the tail run
-                    // (to edge 7), and rows that start and end inside the lane (added at once)
-                    uint32_t head_row = row[0], cur_row = row[0];
-                    float h0 = 0.0f, h1 = 0.0f, c0 = f0[0], c1 = f1[0];
-                    bool multi = false;
+                    // running sums of the lane's runs of equal rows (rows never decrease)
 #pragma unroll
                     for (int u = 1; u < 8; u++) {
-                        if (row[u] == cur_row) {
-                            c0 += f0[u];
-                            c1 += f1[u];
-                        } else {
-                            if (!multi) {
-                                h0 = c0;
-                                h1 = c1;
-                                multi = true;
-                            } else if (cur_row != NO_ROW) {
-                                acc_add<D>(acc, cur_row, c0, c1);
-                            }
-                            cur_row = row[u];
-                            c0 = f0[u];
-                            c1 = f1[u];
-                        }
+                        const bool same = row[u] == row[u - 1];
+                        f0[u] = same ? f0[u - 1] + f0[u] : f0[u];
+                        f1[u] = same ? f1[u - 1] + f1[u] : f1[u];
                     }
-                    const uint32_t tail_row = cur_row;
-                    // segmented inclusive scan of the tail sums over lanes (a segment = equal rows)
-                    const uint32_t prev_tail = __shfl_up(tail_row, 1);
-                    bool flag = lane == 0 || prev_tail != tail_row;
-                    float s0 = c0, s1 = c1;
+                    // the lanes' tail runs joined by a segmented scan across the wave
+                    const uint32_t tail_row = row[7];
+                    const uint32_t prev_tail = (uint32_t)__builtin_amdgcn_update_dpp(
+                        (int)NO_ROW, (int)tail_row, DPP_WAVE_SHR1, 0xf, 0xf, false);
+                    int flag = (lane == 0 || prev_tail != tail_row || tail_row == NO_ROW) ? 1 : 0;
+                    float s0 = f0[7], s1 = f1[7];
+                    seg_scan(s0, s1, flag);
+                    // what the lanes before hold of this lane's head row
+                    const float p0 = dpp_f<DPP_WAVE_SHR1, 0xf>(-0.0f, s0);
+                    const float p1 = dpp_f<DPP_WAVE_SHR1, 0xf>(-0.0f, s1);
+                    const bool joins = lane > 0 && prev_tail == row[0] && row[0] != NO_ROW;
+                    const uint32_t next_head = (uint32_t)__builtin_amdgcn_update_dpp(
+                        (int)NO_ROW, (int)row[0], DPP_WAVE_SHL1, 0xf, 0xf, false);
+                    // every run that ends in this lane is added to its row once; the lane's other
+                    // seven adds go to its own dummy slot.  No two lanes of the wave touch the
+                    // same row here, and no other wave touches these rows in this segment, so
+                    // the read-add-write needs no atomics (what several waves leave in a dummy
+                    // slot is never read).
+                    const uint32_t own = ACC_DUMMY + (uint32_t)lane * 2;
+                    uint32_t at[8];
+                    float v0[8], v1[8];
 #pragma unroll
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const float u0 = __shfl_up(s0, o), u1 = __shfl_up(s1, o);
-                        const bool uf = __shfl_up((int)flag, o) != 0;
-                        if (lane >= o) {
-                            if (!flag) {
-                                s0 = u0 + s0;
-                                s1 = u1 + s1;
-                            }
-                            flag = flag || uf;
-                        }
+                    for (int u = 0; u < 7; u++) {
+                        const bool ends = row[u] != row[u + 1] && row[u] != NO_ROW;
+                        const bool head = joins && row[u] == row[0];
+                        at[u] = ends ? row[u] * 2 : own;
+                        v0[u] = head ? p0 + f0[u] : f0[u];
+                        v1[u] = head ? p1 + f1[u] : f1[u];
                     }
-                    // the head row of a multi-run lane ends here: add what the lanes before hold
-                    const float p0 = __shfl_up(s0, 1), p1 = __shfl_up(s1, 1);
-                    if (multi && head_row != NO_ROW) {
-                        const bool joins = lane > 0 && prev_tail == head_row;
-                        acc_add<D>(acc, head_row, joins ? p0 + h0 : h0, joins ? p1 + h1 : h1);
-                    }
-                    // the tail row ends here unless the next lane's first edge continues it
-                    const uint32_t next_head = __shfl_down(head_row, 1);
-                    if (tail_row != NO_ROW && (lane == 63 || next_head != tail_row))
-                        acc_add<D>(acc, tail_row, s0, s1);
+                    at[7] = tail_row != NO_ROW && next_head != tail_row ? tail_row * 2 : own;
+                    v0[7] = s0;
+                    v1[7] = s1;
+                    acc_add8<D>(acc, at, v0, v1);
                     if (!more) break;
                     cur = nxt;
                     base = n_base;
@@ -437,7 +480,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
         }
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < nrows * D; k += T_THREADS)
-            out[(size_t)row0 * D + k] = lds_view(acc)[k];
+            out[(size_t)row0 * D + k] = lds_view(acc)[D == 2 ? k : 2 * k];
         __syncthreads();  // the sums are out before the next row block's rows overwrite them
         if (queued) {
             if (threadIdx.x == 0) next_lb = atomicAdd(&J.scal->ctr[CTR_ATT], 1u);
