@@ -307,6 +307,7 @@ def main():
     t.synchronize()
     elapsed = time.perf_counter() - start
     eps = steps / elapsed
+    form = t.epoch_form()
     grid = t.grid_info() if method == "fitsne" else None
     if args.short:
         log(f"{steps} epochs in {elapsed:.4f}s -> {eps:.1f} epochs/s; grid {grid}")
@@ -342,6 +343,26 @@ def main():
     t.finish()
     kl_final = t.kl_divergence_pair()
     log(f"full run ({total_epochs} epochs): KL {kl_final}, final grid {final_grid}")
+
+    # the autotuner's epoch form against the built-in one (BHT_FLAG_NO_AUTOTUNE), same epochs
+    flags = t.params()["flags"]
+    t._set(flags=flags | 4)
+    t.initial_embedding(y_seed).prepare(method, 0.5)
+    t.synchronize()
+    s0 = time.perf_counter()
+    t.step(steps)
+    t.synchronize()
+    builtin_eps = steps / (time.perf_counter() - s0)
+    builtin_form = t.epoch_form()
+    t.finish()
+    t._set(flags=flags)
+    autotune = {"epoch_form": form, "builtin_form": builtin_form,
+                "builtin_epochs_per_s": round(builtin_eps, 3),
+                "gain": round(eps / builtin_eps, 4),
+                "meaning": "-1: epoch in order on one stream; k > 0: repulsion beside an "
+                           "attraction on k workgroups"}
+    log(f"autotuner: form {form} ({eps:.1f} epochs/s) against built-in {builtin_form} "
+        f"({builtin_eps:.1f} epochs/s)")
 
     # the reference's grid (>= 10 boxes) beside the 50-box floor, same timed epochs
     ref_grid = None
@@ -413,7 +434,7 @@ def main():
                       "grid_after_timed_epochs": grid, "parallelism": "single GPU",
                       "kl_after_timed_epochs": kl_obj(kl_steps),
                       "kl_full_run": kl_obj(kl_final), "final_grid": final_grid,
-                      "reference_grid": ref_grid,
+                      "reference_grid": ref_grid, "autotune": autotune,
                       "error_vs_fp64_exact": {
                           "state": f"epoch {steps}, 5000 sampled rows",
                           "gradient_rel_l2": float(f"{errs['gradient']:.4e}"),

@@ -325,6 +325,14 @@ class TSNE:
         check(self._L.bht_job_run_state(self._h, C.byref(e), C.byref(x)))
         return int(e.value), bool(x.value)
 
+    def epoch_form(self) -> int:
+        """How the last run's epochs ran (the autotuner's choice): -1 in order on one stream, 0
+        the repulsion beside an attraction with a block per row block, k > 0 beside an
+        attraction on k blocks."""
+        v = C.c_int32(0)
+        check(self._L.bht_job_epoch_form(self._h, C.byref(v)))
+        return int(v.value)
+
     def grid_info(self):
         b, L_ = C.c_uint32(), C.c_uint32()
         lo, hi = C.c_float(), C.c_float()

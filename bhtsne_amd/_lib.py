@@ -96,6 +96,7 @@ SIGNATURES = {
     "bht_knn_f32_ex": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     "bht_job_run_state": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
+    "bht_job_epoch_form": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32)]),
     "bht_exact_repulsion_f64": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                             C.c_void_p, C.POINTER(C.c_double)]),
     "bht_exact_repulsion_f64_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64,

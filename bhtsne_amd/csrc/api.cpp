@@ -68,6 +68,8 @@ int64_t next_smooth(int64_t n) {  // interpolation.rs:364-378
 
 // Allocation accounting is shared by the jobs and batches of a context, which a request loop may
 // drive from two host threads (one preparing arriving jobs, one stepping the batch).
+struct TuneCache;
+
 struct bht_ctx {
     int device = 0;
     uint64_t budget = 0;
@@ -75,6 +77,8 @@ struct bht_ctx {
     TwiddleSet tw;
     hipStream_t stream = nullptr;
     std::mutex mu;
+    std::unique_ptr<TuneCache> tune;  // the autotuner's choices per job size class
+    ~bht_ctx();
 
     void *alloc(size_t bytes) {
         if (bytes == 0) bytes = 16;
@@ -244,6 +248,7 @@ struct bht_job {
 
     // per-phase timing with events (4 per timed epoch: fork, repulsion done, attraction done,
     // update done)
+    int last_choice = -1;     // the last run's epoch form: attraction queue blocks, -1 in order
     bool timing = false;
     bool timed_fork = false;  // the timed epochs forked (else every phase ran in order)
     std::vector<hipEvent_t> events;
@@ -263,14 +268,16 @@ struct bht_job {
         if (side) hipStreamDestroy(side);
         if (stream) hipStreamDestroy(stream);
     }
-    void mark(hipStream_t s) {
+    // Records timing event `slot` (0: epoch start after the recentring, 1: repulsion done,
+    // 2: attraction done, 3: update done) of the epoch being enqueued.
+    void mark(hipStream_t s, int slot) {
         if (!timing) return;
-        if (events_used == events.size()) {
+        while (events.size() < events_used + 4) {
             hipEvent_t e;
             BHT_HIP(hipEventCreate(&e));
             events.push_back(e);
         }
-        BHT_HIP(hipEventRecord(events[events_used++], s));
+        BHT_HIP(hipEventRecord(events[events_used + slot], s));
     }
     uint32_t dim() const { return params.d_out; }
     float *ycur() const { return y[cur].as<float>(); }
@@ -844,6 +851,198 @@ void epoch_tail(bht_job *j, uint64_t epoch) {
     if (epoch == j->params.momentum_switch_epoch) j->params.momentum = j->params.final_momentum;
 }
 
+// One epoch's launches (src/lib.rs:961-1014: repulsion + attraction, gradient step, zero-mean):
+// the recentring, then the repulsion and the attraction (side by side on two streams when
+// `fork`, the attraction on `queue_blocks` blocks), then the update.
+void enqueue_epoch(bht_job *j, const LaunchSet &L, bool fit, bool fork, int queue_blocks) {
+    hipStream_t rs = fork ? j->side : j->stream;
+    if (fork) {
+        center_points(L, j->stream);
+        BHT_HIP(hipEventRecord(j->ev_fork, j->stream));
+        BHT_HIP(hipStreamWaitEvent(j->side, j->ev_fork, 0));
+        // the repulsion reads the recentred y and the bounds' partials written on `stream`
+        l2_sync(j->side);
+        plan_grid(L, fit, j->side);
+    } else {
+        center_plan(L, fit, j->stream);
+    }
+    j->mark(j->stream, 0);
+    if (fork) {
+        // the attraction is enqueued first: the host takes a while to enqueue the repulsion's
+        // chain of kernels, and the attraction must not wait for that
+        attract_tiled(L, nullptr, true, j->stream, queue_blocks);
+        j->mark(j->stream, 2);
+        if (fit) fitsne_repulsion(L, rs);
+        else if (j->n > 0) bh_repulsion(L, rs);
+        j->mark(rs, 1);
+        BHT_HIP(hipEventRecord(j->ev_join, j->side));
+        BHT_HIP(hipStreamWaitEvent(j->stream, j->ev_join, 0));
+        // the update reads F- and Z written on `side`
+        l2_sync(j->stream);
+    } else {
+        if (fit) fitsne_repulsion(L, rs);
+        else if (j->n > 0) bh_repulsion(L, rs);
+        j->mark(rs, 1);
+        attract_tiled(L, nullptr, true, j->stream, 0);
+        j->mark(j->stream, 2);
+    }
+    update_points(L, j->stream);
+    j->mark(j->stream, 3);
+    if (j->timing) j->events_used += 4;
+}
+
+// ---- autotuner ---------------------------------------------------------------------------------
+//
+// For a job large enough to run its repulsion beside its attraction, how many blocks the
+// attraction keeps (the rest of the device goes to the repulsion's chain of small kernels), or
+// whether the epoch is better run in order on one stream, depends on the job's size and on the
+// device.  None of these choices changes any result bit (the attraction's sums do not depend on
+// which block takes which row block, and the two streams compute what one stream computes), so
+// the choice is made by timing: the first run of a job in a size class (per context; method,
+// output dimension and half-octave of n) times a few epochs of each candidate from a snapshot of
+// the job's state, restores the snapshot, and keeps the fastest.
+
+struct TuneChoice {
+    int fork = 1;
+    int queue_blocks = 0;
+};
+
+}  // namespace
+
+struct TuneCache {
+    std::mutex mu;
+    std::map<uint64_t, TuneChoice> choices;
+};
+
+bht_ctx::~bht_ctx() = default;
+
+namespace {
+
+uint64_t tune_key(const bht_job *j) {
+    int lg = 0;  // floor(log2 n)
+    while (lg < 62 && (2ull << lg) <= j->n) lg++;
+    const uint64_t half = j->n >= (3ull << lg) / 2 ? 1 : 0;  // upper half of the octave
+    return ((uint64_t)j->method << 40) | ((uint64_t)j->dim() << 32) | (2ull * lg + half);
+}
+
+constexpr int TUNE_WARM = 1, TUNE_TIMED = 2;
+
+// Epoch-carried device state: both embedding buffers, uy, gains, the scalars (epoch counter,
+// pending recentring) and the update's partial sums (the next recentring reads them).
+struct StateSnapshot {
+    DevBuf y0, y1, uy, gains, scal, mpart;
+    int cur = 0;
+    uint64_t epoch = 0;
+};
+
+void snapshot_take(bht_job *j, StateSnapshot &sn) {
+    bht_ctx *c = j->ctx;
+    const size_t b = (size_t)j->n * j->dim() * 4;
+    auto copy = [&](DevBuf &dst, const DevBuf &src, size_t bytes) {
+        dst.ensure(c, bytes);
+        BHT_HIP(hipMemcpyAsync(dst.p, src.p, bytes, hipMemcpyDeviceToDevice, j->stream));
+    };
+    copy(sn.y0, j->y[0], b);
+    copy(sn.y1, j->y[1], b);
+    copy(sn.uy, j->uy, b);
+    copy(sn.gains, j->gains, b);
+    copy(sn.scal, j->scal, sizeof(EpochScalars));
+    copy(sn.mpart, j->mpart, 2 * (size_t)MAX_UPDATE_PARTS * sizeof(double));
+    sn.cur = j->cur;
+    sn.epoch = j->epoch;
+}
+
+void snapshot_restore(bht_job *j, StateSnapshot &sn) {
+    const size_t b = (size_t)j->n * j->dim() * 4;
+    auto copy = [&](DevBuf &dst, const DevBuf &src, size_t bytes) {
+        BHT_HIP(hipMemcpyAsync(dst.p, src.p, bytes, hipMemcpyDeviceToDevice, j->stream));
+    };
+    copy(j->y[0], sn.y0, b);
+    copy(j->y[1], sn.y1, b);
+    copy(j->uy, sn.uy, b);
+    copy(j->gains, sn.gains, b);
+    copy(j->scal, sn.scal, sizeof(EpochScalars));
+    copy(j->mpart, sn.mpart, 2 * (size_t)MAX_UPDATE_PARTS * sizeof(double));
+    l2_sync(j->stream);
+    BHT_HIP(hipStreamSynchronize(j->stream));
+    j->cur = sn.cur;
+    j->epoch = sn.epoch;
+}
+
+TuneChoice autotune(bht_job *j, const LaunchSet &L, bool fit) {
+    TuneChoice best;
+    best.queue_blocks = attract_default_queue_blocks();
+    // every candidate runs epochs [epoch, epoch + TUNE_WARM + TUNE_TIMED) from the snapshot;
+    // none may be the epoch after stop-lying (it rescales the tiled P in place)
+    const uint64_t sl = j->params.stop_lying_epoch;
+    const uint64_t trials_end = j->epoch + TUNE_WARM + TUNE_TIMED;
+    if (sl != 0 && j->epoch <= sl + 1 && sl + 1 < trials_end) return best;
+    std::vector<TuneChoice> cands;
+    const int nblk = L.nb[F_ATT];
+    for (int q : {256, 320, 352, 384, 416, 448, 512})
+        if (q < nblk) cands.push_back(TuneChoice{1, q});
+    cands.push_back(TuneChoice{1, 0});  // forked, one block per row block
+    cands.push_back(TuneChoice{0, 0});  // in order on one stream
+    StateSnapshot sn;
+    snapshot_take(j, sn);
+    BHT_HIP(hipStreamSynchronize(j->stream));
+    hipEvent_t t0, t1;
+    BHT_HIP(hipEventCreate(&t0));
+    BHT_HIP(hipEventCreate(&t1));
+    float best_ms = 0.0f;
+    const bool was_timing = j->timing;
+    j->timing = false;
+    try {
+        for (size_t c = 0; c < cands.size(); c++) {
+            for (int e = 0; e < TUNE_WARM; e++) {
+                enqueue_epoch(j, L, fit, cands[c].fork != 0, cands[c].queue_blocks);
+                j->cur = 1 - j->cur;
+            }
+            BHT_HIP(hipEventRecord(t0, j->stream));
+            for (int e = 0; e < TUNE_TIMED; e++) {
+                enqueue_epoch(j, L, fit, cands[c].fork != 0, cands[c].queue_blocks);
+                j->cur = 1 - j->cur;
+            }
+            BHT_HIP(hipEventRecord(t1, j->stream));
+            BHT_HIP(hipEventSynchronize(t1));
+            float ms = 0.0f;
+            BHT_HIP(hipEventElapsedTime(&ms, t0, t1));
+            if (c == 0 || ms < best_ms) {
+                best_ms = ms;
+                best = cands[c];
+            }
+            snapshot_restore(j, sn);
+        }
+    } catch (...) {
+        j->timing = was_timing;
+        hipEventDestroy(t0);
+        hipEventDestroy(t1);
+        throw;
+    }
+    j->timing = was_timing;
+    BHT_HIP(hipEventDestroy(t0));
+    BHT_HIP(hipEventDestroy(t1));
+    return best;
+}
+
+TuneChoice epoch_choice(bht_job *j, const LaunchSet &L, bool fit) {
+    TuneChoice ch;
+    ch.fork = j->n >= FORK_MIN_POINTS && !(j->params.flags & BHT_FLAG_SERIAL_EPOCH) ? 1 : 0;
+    ch.queue_blocks = ch.fork ? attract_default_queue_blocks() : 0;
+    if (!ch.fork || (j->params.flags & BHT_FLAG_NO_AUTOTUNE)) return ch;
+    TuneCache &tc = *j->ctx->tune;
+    const uint64_t key = tune_key(j);
+    {
+        std::lock_guard<std::mutex> g(tc.mu);
+        auto it = tc.choices.find(key);
+        if (it != tc.choices.end()) return it->second;
+    }
+    ch = autotune(j, L, fit);
+    std::lock_guard<std::mutex> g(tc.mu);
+    tc.choices[key] = ch;
+    return ch;
+}
+
 void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     if (!j->prepared) throw Error(ERR_STATE, "bht_job_prepare must precede bht_job_step");
     if (j->in_batch) throw Error(ERR_STATE, "the job is advanced by its batch");
@@ -852,37 +1051,12 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     const LaunchSet L = single_set(j);
     const bool fit = j->method == FIT_FITSNE;
     // Small jobs are bound by launch latency, which the fork's events would only add to; they
-    // run the same kernels in order on one stream.
-    const bool fork = j->n >= FORK_MIN_POINTS && !(j->params.flags & BHT_FLAG_SERIAL_EPOCH);
-    hipStream_t rs = fork ? j->side : j->stream;
-    if (j->timing) j->timed_fork = fork;
+    // run the same kernels in order on one stream.  Large ones use the autotuner's choice.
+    const TuneChoice ch = count > 0 ? epoch_choice(j, L, fit) : TuneChoice{0, 0};
+    j->last_choice = ch.fork ? ch.queue_blocks : -1;
+    if (j->timing) j->timed_fork = ch.fork != 0;
     for (uint64_t e = 0; e < count; e++) {
-        // recentre (+ FIt-SNE grid plan), then the repulsion and the attraction side by side; a
-        // forked epoch plans the grid on the repulsion's stream
-        if (fork) {
-            center_points(L, j->stream);
-            BHT_HIP(hipEventRecord(j->ev_fork, j->stream));
-            BHT_HIP(hipStreamWaitEvent(j->side, j->ev_fork, 0));
-            // the repulsion reads the recentred y and the bounds' partials written on `stream`
-            l2_sync(j->side);
-            plan_grid(L, fit, j->side);
-        } else {
-            center_plan(L, fit, j->stream);
-        }
-        j->mark(j->stream);
-        if (fit) fitsne_repulsion(L, rs);
-        else if (j->n > 0) bh_repulsion(L, rs);
-        j->mark(rs);
-        attract_tiled(L, nullptr, true, j->stream, fork);
-        j->mark(j->stream);
-        if (fork) {
-            BHT_HIP(hipEventRecord(j->ev_join, j->side));
-            BHT_HIP(hipStreamWaitEvent(j->stream, j->ev_join, 0));
-            // the update reads F- and Z written on `side`
-            l2_sync(j->stream);
-        }
-        update_points(L, j->stream);
-        j->mark(j->stream);
+        enqueue_epoch(j, L, fit, ch.fork != 0, ch.queue_blocks);
         j->cur = 1 - j->cur;
         const uint64_t epoch = j->epoch++;
         epoch_tail(j, epoch);
@@ -1328,6 +1502,7 @@ bht_status bht_ctx_create(int device, uint64_t hbm_budget_bytes, bht_ctx **out) 
         if (device < 0 || device >= count)
             throw Error(ERR_INVALID, "device index out of range");
         auto ctx = std::make_unique<bht_ctx>();
+        ctx->tune = std::make_unique<TuneCache>();
         ctx->device = device;
         ctx->budget = hbm_budget_bytes;
         BHT_HIP(hipSetDevice(device));
@@ -1610,6 +1785,12 @@ bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, cons
         l2_sync(job->stream);
         BHT_HIP(hipStreamSynchronize(job->stream));
     });
+}
+
+bht_status bht_job_epoch_form(const bht_job *job, int32_t *attraction_blocks) {
+    if (!job || !attraction_blocks) return fail(BHT_ERR_INVALID, "null argument");
+    *attraction_blocks = job->last_choice;
+    return BHT_OK;
 }
 
 bht_status bht_job_run_state(const bht_job *job, uint64_t *next_epoch, int *exaggerated) {

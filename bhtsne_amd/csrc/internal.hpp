@@ -254,9 +254,12 @@ const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s);
 
 // Attraction sums over the tiled layout (tiled.hip): epoch = true writes fattr (and applies the
 // stop-lying rescale to tval), epoch = false writes job 0's sums into pos.
-// beside: the job's repulsion runs concurrently on another stream (the attraction then leaves
-// part of the device to it).
-void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s, bool beside = false);
+// queue_blocks > 0 (single job with its repulsion running concurrently on another stream): that
+// many blocks take the row blocks from a work queue, leaving the rest of the device to the
+// repulsion; 0: one block per row block.  The sums are the same either way.
+void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s,
+                   int queue_blocks = 0);
+int attract_default_queue_blocks();
 // The gradient step from fattr, frep and Z (attract.hip).
 void update_points(const LaunchSet &L, hipStream_t s);
 void scale_values(float *vals, uint64_t nnz, float scale, hipStream_t s);

@@ -39,11 +39,11 @@ constexpr int T_WAVES = T_THREADS / 64;
 constexpr int TILE_ROWS_MAX = 1024;
 constexpr int GROUP_SPAN = 512;  // edges per wave pass: 64 lanes x 8
 // With the repulsion running beside it, a job's attraction launches at most this many blocks,
-// which take row blocks from a work queue: 13/16 of the 512 that fit the device at once (two per
-// CU), so that the repulsion always finds CUs (1M points: 352 blocks 2041 epochs/s, 384: 2078,
-// 416: 2094; the best share moves with the repulsion's own speed).  Alone on the device it keeps a
-// block per row block.
-constexpr int ATT_QUEUE_BLOCKS = 416;
+// which take row blocks from a work queue, so that the repulsion always finds CUs; 384 of the 512
+// that fit the device at once (two per CU) is the best share at 1M points.  The best share moves
+// with the two sides' speeds, so the driver's autotuner picks it per job size; this is the value
+// it starts from.  Alone on the device the attraction keeps a block per row block.
+constexpr int ATT_QUEUE_BLOCKS = 384;
 // LDS per block: window + the block's row positions + the accumulators, under 80,064 bytes so
 // that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran at half speed)
 constexpr int WIN_FLOATS = 11800;
@@ -586,9 +586,11 @@ void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, 
     BHT_LAUNCH_CHECK();
 }
 
-void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s, bool beside) {
-    const int queued = epoch && beside && L.map[F_ATT] == nullptr ? 1 : 0;
-    const dim3 g(queued ? std::min(L.nb[F_ATT], ATT_QUEUE_BLOCKS) : L.nb[F_ATT]);
+int attract_default_queue_blocks() { return ATT_QUEUE_BLOCKS; }
+
+void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s, int queue_blocks) {
+    const int queued = epoch && queue_blocks > 0 && L.map[F_ATT] == nullptr ? 1 : 0;
+    const dim3 g(queued ? std::min(L.nb[F_ATT], queue_blocks) : L.nb[F_ATT]);
     if (L.dim == 2) {
         if (epoch)
             hipLaunchKernelGGL((k_attract_tiled<2, true>), g, dim3(T_THREADS), 0, s, L.jobs,

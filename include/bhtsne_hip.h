@@ -86,6 +86,12 @@ typedef struct bht_params {
  * the same bits as in order).  This flag keeps every epoch in order on the job's stream, the
  * order a batch runs its members in. */
 #define BHT_FLAG_SERIAL_EPOCH 2u
+/* A job of 32768 points or more takes the form of its epochs (the two streams or one, and how
+ * many workgroups the attraction keeps beside the repulsion) from a per-context autotuner, which
+ * times the candidates on the first run of each job size class (a few epochs from a snapshot of
+ * the job's state, which is then restored; every candidate gives the same result bits).  This
+ * flag uses the built-in form instead. */
+#define BHT_FLAG_NO_AUTOTUNE 4u
 
 void bht_params_default(bht_params *p);
 
@@ -165,6 +171,10 @@ bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, cons
  * run with stop_lying_epoch == 0, whose export divides P by the exaggeration once more as
  * affinities() does, src/lib.rs:1024-1029). */
 bht_status bht_job_run_state(const bht_job *job, uint64_t *next_epoch, int *exaggerated);
+/* The form the last run's epochs took (no counterpart in the reference): -1 in order on one
+ * stream, 0 the repulsion beside the attraction with a workgroup per row block, k > 0 beside an
+ * attraction on k workgroups taking row blocks from a queue. */
+bht_status bht_job_epoch_form(const bht_job *job, int32_t *attraction_blocks);
 /* FIt-SNE grid of the last repulsion: boxes per axis, FFT length, joint min and max. */
 bht_status bht_job_grid_info(const bht_job *job, uint32_t *boxes, uint32_t *fft_len, float *lo,
                              float *hi);

@@ -12,6 +12,7 @@ from bhtsne_amd import datagen
 pytestmark = pytest.mark.gpu
 
 SERIAL = 2  # BHT_FLAG_SERIAL_EPOCH
+NO_AUTOTUNE = 4  # BHT_FLAG_NO_AUTOTUNE
 
 
 @pytest.fixture(scope="module")
@@ -42,10 +43,28 @@ def test_forked_fit_repeats_and_equals_serial(bt, request_id, epochs):
     a, ka = _fit(bt, x, y0, aff, 0, epochs, d_out)
     b, kb = _fit(bt, x, y0, aff, 0, epochs, d_out)
     c, kc = _fit(bt, x, y0, aff, SERIAL, epochs, d_out)
+    d, kd = _fit(bt, x, y0, aff, NO_AUTOTUNE, epochs, d_out)
     diff_ab = int((a != b).sum())
     diff_ac = int((a != c).sum())
+    diff_ad = int((a != d).sum())
     print(f"forked vs forked: {diff_ab} differing values; forked vs serial: {diff_ac}; "
-          f"KL {ka} {kb} {kc}", flush=True)
+          f"autotuned vs built-in form: {diff_ad}; KL {ka} {kb} {kc} {kd}", flush=True)
     assert np.all(np.isfinite(a))
     assert diff_ab == 0 and ka == kb
     assert diff_ac == 0 and ka == kc
+    assert diff_ad == 0 and ka == kd
+
+
+def test_autotuner_choice_is_reported_and_cached(bt):
+    x, _ = datagen.gmm(40000, 16, 8, 16.0, 5)
+    ctx = bt.Context(0)
+    forms = []
+    for _ in range(2):
+        t = bt.TSNE(x, ctx=ctx).perplexity(30.0).epochs(5)
+        t.initial_embedding(datagen.seed_embedding(40000, 2, 5)).fit_sne()
+        forms.append(t.epoch_form())
+    print(f"autotuned epoch form at n=40000: {forms}", flush=True)
+    assert forms[0] == forms[1] and forms[0] >= -1
+    t = bt.TSNE(x[:1000], ctx=ctx).perplexity(30.0).epochs(3)
+    t.fit_sne()
+    assert t.epoch_form() == -1  # small jobs run in order on one stream
