@@ -136,7 +136,7 @@ def bench_serving(args):
     serving.serve([serving.Request(-1, reqs[0].x[:2000], epochs=20)], ctx, want_kl=False)
     res_b = serving.serve(reqs, ctx, step_epochs=args.step_epochs)
     sb = serving.latency_summary(res_b)
-    log(f"batched: {sb}")
+    log(f"batched: {sb}; loop {serving.serve.last_stats}")
     res_p = serving.serve_plain(reqs, ctx)
     sp = serving.latency_summary(res_p)
     log(f"plain: {sp}")

@@ -80,6 +80,16 @@ struct bht_ctx {
     std::unique_ptr<TuneCache> tune;  // the autotuner's choices per job size class
     ~bht_ctx();
 
+    // Freed blocks are kept for reuse instead of being returned with hipFree, which waits for
+    // the whole device: a request loop frees a preparing job's temporaries while a batch runs,
+    // and each free would stall on the batch's queued epochs.  Live bytes (what the budget and
+    // the admission accounting see) are the requested sizes; cached blocks are trimmed when
+    // they would exceed CACHE_LIMIT or, with a budget, live + cached bytes would exceed it.
+    static constexpr uint64_t CACHE_LIMIT = 16ull << 30;
+    std::multimap<size_t, void *> cached;         // capacity -> free block
+    std::map<void *, size_t> capacity;            // live block -> capacity
+    uint64_t cached_bytes = 0;
+
     void *alloc(size_t bytes) {
         if (bytes == 0) bytes = 16;
         {
@@ -88,22 +98,60 @@ struct bht_ctx {
                 throw Error(ERR_OOM, "HBM budget exceeded: " + std::to_string(in_use + bytes) +
                                          " bytes requested of " + std::to_string(budget));
             in_use += bytes;
+            // the smallest cached block that fits without wasting more than its request
+            auto it = cached.lower_bound(bytes);
+            if (it != cached.end() && it->first <= 2 * bytes + (1u << 20)) {
+                void *p = it->second;
+                capacity[p] = it->first;
+                cached_bytes -= it->first;
+                cached.erase(it);
+                return p;
+            }
         }
         void *p = nullptr;
         hipError_t e = hipSetDevice(device);
         if (e == hipSuccess) e = hipMalloc(&p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            trim(0);
+            e = hipMalloc(&p, bytes);
+        }
+        std::lock_guard<std::mutex> g(mu);
         if (e != hipSuccess) {
-            std::lock_guard<std::mutex> g(mu);
             in_use -= bytes;
             BHT_HIP(e);
         }
+        capacity[p] = bytes;
         return p;
     }
     void release(void *p, size_t bytes) {
         if (!p) return;
-        hipFree(p);
-        std::lock_guard<std::mutex> g(mu);
-        in_use -= bytes ? bytes : 16;
+        bool over;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            in_use -= bytes ? bytes : 16;
+            auto it = capacity.find(p);
+            const size_t cap = it != capacity.end() ? it->second : (bytes ? bytes : 16);
+            if (it != capacity.end()) capacity.erase(it);
+            cached.emplace(cap, p);
+            cached_bytes += cap;
+            over = cached_bytes > CACHE_LIMIT || (budget && in_use + cached_bytes > budget);
+        }
+        if (over) trim(budget ? budget - std::min(budget, used()) : CACHE_LIMIT / 2);
+    }
+    // Returns cached blocks to the device (largest first) until at most `keep` bytes stay cached.
+    void trim(uint64_t keep) {
+        std::vector<void *> out;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            while (cached_bytes > keep && !cached.empty()) {
+                auto it = std::prev(cached.end());
+                cached_bytes -= it->first;
+                out.push_back(it->second);
+                cached.erase(it);
+            }
+        }
+        for (void *p : out) hipFree(p);
     }
     uint64_t used() {
         std::lock_guard<std::mutex> g(mu);
@@ -1168,8 +1216,9 @@ void fit_with_neighbors(bht_job *j, int method, float theta, const uint64_t *idx
     ddist.ensure(c, n_rows * k * 4);
     std::vector<uint32_t> hidx(n_rows * k);
     for (uint64_t e = 0; e < n_rows * k; e++) hidx[e] = (uint32_t)idx[e];
-    BHT_HIP(hipMemcpy(didx.p, hidx.data(), hidx.size() * 4, hipMemcpyHostToDevice));
-    BHT_HIP(hipMemcpy(ddist.p, dist, n_rows * k * 4, hipMemcpyHostToDevice));
+    BHT_HIP(hipMemcpyAsync(didx.p, hidx.data(), hidx.size() * 4, hipMemcpyHostToDevice,
+                           j->stream));
+    BHT_HIP(hipMemcpyAsync(ddist.p, dist, n_rows * k * 4, hipMemcpyHostToDevice, j->stream));
     affinities_from_knn(j, didx.as<uint32_t>(), ddist.as<float>(), (uint32_t)k);
     fit(j, method, theta, cb, user, false);
 }
@@ -1283,11 +1332,11 @@ void batch_rebuild(bht_batch *b) {
         g.sort_temp.ensure(c, group_sort_temp_bytes(arena));
         g.segs.ensure(c, segs.size() * sizeof(SortSeg));
         g.tile_seg.ensure(c, std::max<size_t>(tile_seg.size(), 1) * 4);
-        BHT_HIP(hipMemcpy(g.segs.p, segs.data(), segs.size() * sizeof(SortSeg),
-                          hipMemcpyHostToDevice));
+        BHT_HIP(hipMemcpyAsync(g.segs.p, segs.data(), segs.size() * sizeof(SortSeg),
+                               hipMemcpyHostToDevice, b->stream));
         if (!tile_seg.empty())
-            BHT_HIP(hipMemcpy(g.tile_seg.p, tile_seg.data(), tile_seg.size() * 4,
-                              hipMemcpyHostToDevice));
+            BHT_HIP(hipMemcpyAsync(g.tile_seg.p, tile_seg.data(), tile_seg.size() * 4,
+                                   hipMemcpyHostToDevice, b->stream));
         // table entries: batch grid sizes, sort fields into the arena
         int len_cap = 1;
         for (size_t q = 0; q < g.jobs.size(); q++) {
@@ -1324,7 +1373,10 @@ void batch_rebuild(bht_batch *b) {
             count[f] = (int)(maps.size() - off[f]);
         }
         g.maps.ensure(c, std::max<size_t>(maps.size(), 1) * 4);
-        BHT_HIP(hipMemcpy(g.maps.p, maps.data(), maps.size() * 4, hipMemcpyHostToDevice));
+        BHT_HIP(hipMemcpyAsync(g.maps.p, maps.data(), maps.size() * 4, hipMemcpyHostToDevice,
+                               b->stream));
+        // the host vectors are released at the end of this iteration
+        BHT_HIP(hipStreamSynchronize(b->stream));
         LaunchSet &L = g.L;
         for (int f = 0; f < NFAM; f++) {
             L.map[f] = g.maps.as<uint32_t>() + off[f];
@@ -1346,8 +1398,9 @@ void batch_rebuild(bht_batch *b) {
     }
     b->table.ensure(c, std::max<size_t>(table.size(), 1) * sizeof(JobDev));
     if (!table.empty())
-        BHT_HIP(hipMemcpy(b->table.p, table.data(), table.size() * sizeof(JobDev),
-                          hipMemcpyHostToDevice));
+        BHT_HIP(hipMemcpyAsync(b->table.p, table.data(), table.size() * sizeof(JobDev),
+                               hipMemcpyHostToDevice, b->stream));
+    BHT_HIP(hipStreamSynchronize(b->stream));
     for (auto &g : b->groups) g->L.jobs = b->table.as<JobDev>();
     b->dirty = false;
 }
@@ -1520,6 +1573,7 @@ void bht_ctx_destroy(bht_ctx *ctx) {
     ctx->release(ctx->tw.len_of_boxes, (MAX_BOXES + 1) * sizeof(int));
     ctx->release(ctx->tw.off_of_boxes, (MAX_BOXES + 1) * sizeof(int));
     if (ctx->stream) hipStreamDestroy(ctx->stream);
+    ctx->trim(0);
     delete ctx;
 }
 
@@ -1564,7 +1618,9 @@ bht_status bht_job_set_data_f32(bht_job *job, const float *x, uint64_t n, uint32
         if (n != job->n) throw Error(ERR_INVALID, "data rows must equal n_samples");
         if (d_in == 0) throw Error(ERR_INVALID, "d_in must be positive");
         job->x.ensure(job->ctx, (size_t)n * d_in * 4);
-        BHT_HIP(hipMemcpy(job->x.p, x, (size_t)n * d_in * 4, hipMemcpyHostToDevice));
+        BHT_HIP(hipMemcpyAsync(job->x.p, x, (size_t)n * d_in * 4, hipMemcpyHostToDevice,
+                               job->stream));
+        BHT_HIP(hipStreamSynchronize(job->stream));
         job->d_in = d_in;
         job->have_data = true;
     });
@@ -1690,9 +1746,10 @@ bht_status bht_job_set_affinities(bht_job *job, const uint64_t *rows, const uint
         job->rows.ensure(job->ctx, n_rows_plus_one * 4);
         job->cols.ensure(job->ctx, nnz * 4);
         job->vals.ensure(job->ctx, nnz * 4);
-        BHT_HIP(hipMemcpy(job->rows.p, r.data(), r.size() * 4, hipMemcpyHostToDevice));
-        BHT_HIP(hipMemcpy(job->cols.p, cols, nnz * 4, hipMemcpyHostToDevice));
-        BHT_HIP(hipMemcpy(job->vals.p, vals, nnz * 4, hipMemcpyHostToDevice));
+        BHT_HIP(hipMemcpyAsync(job->rows.p, r.data(), r.size() * 4, hipMemcpyHostToDevice,
+                               job->stream));
+        BHT_HIP(hipMemcpyAsync(job->cols.p, cols, nnz * 4, hipMemcpyHostToDevice, job->stream));
+        BHT_HIP(hipMemcpyAsync(job->vals.p, vals, nnz * 4, hipMemcpyHostToDevice, job->stream));
         job->nnz = nnz;
         // with_affinities (lib.rs:1043-1051): adopt the perplexity; the cache key is rows.len()
         job->params.perplexity = perplexity;

@@ -44,6 +44,7 @@ class Result:
     arrival: float
     admitted: float = 0.0
     finished: float = 0.0
+    prepare_s: float = 0.0        # affinities + optimiser state on the preparing thread
     embedding: Optional[np.ndarray] = None
     kl: Optional[float] = None
 
@@ -79,6 +80,7 @@ def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: i
     reserved = {"bytes": 0}
     lock = threading.Condition()
     errors = []
+    stats = {"step_s": 0.0, "steps": 0}
     t0 = time.perf_counter()
 
     def preparer():
@@ -93,8 +95,10 @@ def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: i
                     while budget and reserved["bytes"] + need > budget and reserved["bytes"] > 0:
                         lock.wait(0.05)
                     reserved["bytes"] += need
+                p0 = time.perf_counter()
                 job = _make_job(r, ctx)
                 job.prepare(r.method, r.theta)
+                results[r.rid].prepare_s = time.perf_counter() - p0
                 ready.put((r, job, need))
         except Exception as e:  # surfaced by the stepping loop
             errors.append(e)
@@ -125,7 +129,10 @@ def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: i
         if errors:
             raise errors[0]
         if len(batch):
+            s0 = time.perf_counter()
             batch.step(step_epochs)
+            stats["step_s"] += time.perf_counter() - s0
+            stats["steps"] += 1
             for job in batch.take_finished():
                 r, _, need = live.pop(job._h.value)
                 _collect(job, results[r.rid], want_kl, t0)
@@ -136,6 +143,8 @@ def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: i
     if errors:
         raise errors[0]
     batch.close()
+    serve.last_stats = dict(stats, span_s=time.perf_counter() - t0,
+                            prepare_s=sum(r.prepare_s for r in results.values()))
     return [results[r.rid] for r in requests]
 
 
