@@ -349,7 +349,8 @@ class Batch:
     fit alone (``TSNE.fit_sne`` / ``barnes_hut``) bit for bit.  Jobs join with ``add``
     (which builds their affinities unless cached) and leave when their ``epochs`` are done."""
 
-    def __init__(self, ctx: Optional[Context] = None, graph: bool = True):
+    def __init__(self, ctx: Optional[Context] = None, graph: bool = True,
+                 concurrent: bool = False):
         self._ctx = ctx or default_context()
         self._L = _lib.load()
         h = C.c_void_p()
@@ -358,6 +359,8 @@ class Batch:
         self._members = {}  # handle value -> TSNE (kept alive while a member)
         if not graph:
             check(self._L.bht_batch_set_graph(self._h, 0))
+        if concurrent:
+            check(self._L.bht_batch_set_concurrent(self._h, 1))
 
     def add(self, job: TSNE, method: str = "fitsne", theta: float = 0.5):
         m = _lib.METHOD_FITSNE if method == "fitsne" else _lib.METHOD_BARNES_HUT

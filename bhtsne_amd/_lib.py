@@ -110,6 +110,7 @@ SIGNATURES = {
                                             C.POINTER(C.c_uint64)]),
     "bht_batch_size": (C.c_uint64, [C.c_void_p]),
     "bht_batch_set_graph": (C.c_int32, [C.c_void_p, C.c_int]),
+    "bht_batch_set_concurrent": (C.c_int32, [C.c_void_p, C.c_int]),
     "bht_batch_synchronize": (C.c_int32, [C.c_void_p]),
     "bht_job_footprint": (C.c_int32, [C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32,
                                       C.POINTER(C.c_uint64)]),

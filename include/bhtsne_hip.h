@@ -253,6 +253,10 @@ bht_status bht_batch_take_finished(bht_batch *batch, bht_job **out, uint64_t cap
 uint64_t bht_batch_size(const bht_batch *batch);
 /* Graph replay of the epoch (default on) or plain launches. */
 bht_status bht_batch_set_graph(bht_batch *batch, int enable);
+/* The groups' repulsions on a second stream beside their attractions, or every kernel in order
+ * on one stream (the default: a batch's grouped launches already fill the device); the results
+ * are the same bits either way. */
+bht_status bht_batch_set_concurrent(bht_batch *batch, int enable);
 bht_status bht_batch_synchronize(bht_batch *batch);
 /* Upper bound of the device bytes one job of n samples (d_in features) needs: what a request loop
  * admits against the context's HBM budget. */

@@ -68,9 +68,11 @@ def alone(bt):
     return [_alone(bt, s) for s in SPECS]
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_batch_equals_alone(bt, alone, graph):
-    b = bt.Batch(graph=graph)
+@pytest.mark.parametrize("graph,concurrent", [(True, False), (False, False), (True, True)])
+def test_batch_equals_alone(bt, alone, graph, concurrent):
+    # with the repulsions beside the attractions on a second stream or every kernel in order,
+    # replayed from a graph or launched plainly: each member is its fit alone, bit for bit
+    b = bt.Batch(graph=graph, concurrent=concurrent)
     jobs = []
     for s in SPECS:
         t = _job(bt, s)
