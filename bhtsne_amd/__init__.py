@@ -8,7 +8,9 @@ same rejection messages (raised as ``InvalidInput``).  Every numeric step runs i
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 from typing import Callable, NamedTuple, Optional, Sequence
 
@@ -22,6 +24,26 @@ __all__ = ["Context", "Neighbor", "SparseAffinities", "TSNE", "Batch", "BhtError
            "default_context", "job_footprint"]
 
 
+# Live handles, released in dependency order (batches, jobs, contexts) when the interpreter exits,
+# while the library and the HIP runtime are still fully up: objects left to the interpreter's
+# final teardown would otherwise be released in no particular order.
+_live = weakref.WeakSet()
+
+
+def _release_all():
+    objs = list(_live)
+    for kind in ("Batch", "TSNE", "Context"):
+        for o in objs:
+            if type(o).__name__ == kind:
+                try:
+                    o.close()
+                except Exception:
+                    pass
+
+
+atexit.register(_release_all)
+
+
 class Context:
     """One device context (twiddle tables, HBM budget accounting)."""
 
@@ -31,6 +53,7 @@ class Context:
         check(L.bht_ctx_create(int(device), int(hbm_budget), C.byref(h)))
         self._h = h
         self._L = L
+        _live.add(self)
 
     @property
     def handle(self):
@@ -111,12 +134,17 @@ class TSNE:
             check(self._L.bht_job_set_data_f32(self._h, ptr(x), self._n, x.shape[1]))
         self._callback = None
         self._fitted = False
+        _live.add(self)
+
+    def close(self):
+        """Releases the job's device memory (the object is unusable afterwards)."""
+        if getattr(self, "_h", None):
+            self._L.bht_job_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:
-            if self._h:
-                self._L.bht_job_destroy(self._h)
-                self._h = None
+            self.close()
         except Exception:
             pass
 
@@ -357,6 +385,7 @@ class Batch:
         check(self._L.bht_batch_create(self._ctx.handle, C.byref(h)))
         self._h = h
         self._members = {}  # handle value -> TSNE (kept alive while a member)
+        _live.add(self)
         if not graph:
             check(self._L.bht_batch_set_graph(self._h, 0))
         if concurrent:

@@ -78,6 +78,10 @@ struct bht_ctx {
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::unique_ptr<TuneCache> tune;  // the autotuner's choices per job size class
+    // Jobs and batches hold their context: destroying a context that still has them only marks
+    // it, and the last of them frees it, so that destruction in any order is safe.
+    int users = 0;
+    bool closed = false;
     ~bht_ctx();
 
     // Freed blocks are kept for reuse instead of being returned with hipFree, which waits for
@@ -265,6 +269,20 @@ struct bht_job {
     DevBuf keys, svals, keys_alt, svals_alt, grid, spec, rker, kspec, part, dpart, mpart, fpart,
         sort_temp, scal;
     int nodes_cap = 0, len_cap = 0;
+    // A large 2-D FIt-SNE job sizes its FFT launches' LDS for the lengths its grid has reached
+    // (reported by the grid plan into host_len, pinned host memory), so that they fit a CU beside
+    // the attraction; lines that outgrow a launch use fft_fb.  The host keeps at most
+    // 16 epochs queued so that the reported length is recent.
+    int *host_len = nullptr;
+    DevBuf fft_fb;
+    // the box sort's final buffers (2-D jobs sorted in 7-bit passes, whose pass count the grid
+    // plan decides on the device)
+    DevBuf keys_fin, svals_fin;
+    // the device holds per-block bounds of the current embedding buffer (from the last update or
+    // centring), so that the next FIt-SNE epoch can leave the recentring to its readers
+    bool lazy_ready = false;
+    hipEvent_t ahead[4] = {};
+    bool ahead_rec[4] = {};
     // Barnes-Hut workspace
     DevBuf codes, codes_alt, bidx, bidx_alt, bsort_temp, bnode, bmeta, brange, bhw2, bcount;
     // the job's entry of the device job table (host copy + device copy)
@@ -308,9 +326,14 @@ struct bht_job {
         BHT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         BHT_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         BHT_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        for (auto &e : ahead) BHT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     ~bht_job() {
         for (auto e : events) hipEventDestroy(e);
+        for (auto e : ahead)
+            if (e) hipEventDestroy(e);
+        if (stream) hipStreamSynchronize(stream);
+        if (host_len) hipHostFree(host_len);
         if (ev_fork) hipEventDestroy(ev_fork);
         if (ev_join) hipEventDestroy(ev_join);
         if (side) hipStreamDestroy(side);
@@ -359,6 +382,9 @@ void ensure_state_buffers(bht_job *j) {
     j->fpart.ensure(j->ctx, 4 * 1024 * sizeof(float));
 }
 
+// Jobs from this size on run their epochs forked and size their FFT launches dynamically.
+constexpr uint64_t DYN_FFT_MIN_POINTS = 32768;
+
 void ensure_fitsne_workspace(bht_job *j) {
     ensure_state_buffers(j);
     bht_ctx *c = j->ctx;
@@ -391,6 +417,22 @@ void ensure_fitsne_workspace(bht_job *j) {
     d.rker = j->rker.as<float>();
     d.kspec = j->kspec.as<float>();
     d.part = j->part.as<float>();
+    if (j->dim() == 2 && n >= DYN_FFT_MIN_POINTS) {
+        j->fft_fb.ensure(c, (size_t)FFT_FB_BLOCKS * 2 * j->len_cap * sizeof(float2));
+        if (!j->host_len) {
+            BHT_HIP(hipHostMalloc((void **)&j->host_len, 64, hipHostMallocDefault));
+            *j->host_len = 0;
+        }
+        d.fft_fb = j->fft_fb.as<float2>();
+        d.host_len = j->host_len;
+        if (n >= SORT7_MIN_KEYS) {
+            j->keys_fin.ensure(c, n * 4);
+            j->svals_fin.ensure(c, n * 4);
+        }
+    } else {
+        d.fft_fb = nullptr;
+        d.host_len = nullptr;
+    }
 }
 
 void ensure_bh_workspace(bht_job *j) {
@@ -423,6 +465,13 @@ float resolve_learning_rate(const bht_job *j) {  // lib.rs:652-659
 
 // Fills the job's table entry from its buffers and parameters.  A job alone sorts in its own
 // buffers; a batch points the sort fields at its arena afterwards.
+// Whether the job's box sort takes its pass count from the device's grid plan (a 2-D FIt-SNE job
+// large enough for 7-bit passes and with the final buffers; batches sort with fixed passes).
+bool device_sort_passes(const bht_job *j) {
+    return j->method == FIT_FITSNE && j->dim() == 2 && j->n >= SORT7_MIN_KEYS &&
+           j->keys_fin.p != nullptr;
+}
+
 void fill_jobdev(bht_job *j) {
     JobDev &d = j->hd;
     d.n = (int)j->n;
@@ -468,6 +517,10 @@ void fill_jobdev(bht_job *j) {
         d.kidx = j->svals.as<uint32_t>();
         d.skeys = alt ? j->keys_alt.as<uint32_t>() : j->keys.as<uint32_t>();
         d.sidx = alt ? j->svals_alt.as<uint32_t>() : j->svals.as<uint32_t>();
+        if (device_sort_passes(j)) {
+            d.skeys = j->keys_fin.as<uint32_t>();
+            d.sidx = j->svals_fin.as<uint32_t>();
+        }
     } else {
         const bool alt = sort_result_alt(MORTON_KEY_BITS, false);
         d.codes = j->codes.as<uint64_t>();
@@ -520,7 +573,7 @@ LaunchSet single_set(bht_job *j) {
     L.dim = (int)j->dim();
     L.njobs = 1;
     L.tw = &j->ctx->tw;
-    L.fft_lds = (size_t)j->len_cap * sizeof(float2);
+    L.fft_cap = j->len_cap;
     if (j->method == FIT_BH) {
         L.keys = j->codes.p;
         L.keys_alt = j->codes_alt.p;
@@ -533,6 +586,11 @@ LaunchSet single_set(bht_job *j) {
         L.vals = j->svals.as<uint32_t>();
         L.vals_alt = j->svals_alt.as<uint32_t>();
         L.sort_temp = j->sort_temp.p;
+        if (device_sort_passes(j)) {
+            L.sort_npass = &j->scal.as<EpochScalars>()->sort_passes;
+            L.keys_fin = j->keys_fin.p;
+            L.vals_fin = j->svals_fin.as<uint32_t>();
+        }
     }
     L.sort_n = j->n;
     return L;
@@ -540,7 +598,10 @@ LaunchSet single_set(bht_job *j) {
 
 // Applies the recentring left pending by the last update (zero_mean, mod.rs:534-568) and writes
 // the per-axis bounds.
-void recenter(bht_job *j) { center_plan(single_set(j), false, j->stream); }
+void recenter(bht_job *j) {
+    center_plan(single_set(j), false, j->stream);
+    j->lazy_ready = false;
+}
 
 // ---- point order -----------------------------------------------------------------------------
 
@@ -871,6 +932,7 @@ void prepare_run(bht_job *j, int method, float theta, bool cached) {
     j->method = method;
     j->theta = theta;
     j->epoch = 0;
+    j->lazy_ready = false;
     j->run_momentum = j->params.momentum;
     j->prepared = true;
     // the tiled copy of P is built once the points have their final numbering
@@ -899,19 +961,37 @@ void epoch_tail(bht_job *j, uint64_t epoch) {
     if (epoch == j->params.momentum_switch_epoch) j->params.momentum = j->params.final_momentum;
 }
 
+// FFT line points the epoch's FFT launches hold in LDS: a quarter more than the latest length the
+// job's grid plan reported (rounded to 64), at most the grid capacity; the whole capacity until the
+// plan has reported, and for jobs without the fallback buffer.
+int fft_cap_for(const bht_job *j) {
+    if (!j->host_len || !j->hd.fft_fb) return j->len_cap;
+    if (j->params.flags & BHT_FLAG_FFT_GLOBAL_LINES) return 64;
+    const int seen = __atomic_load_n(j->host_len, __ATOMIC_ACQUIRE);
+    if (seen <= 0) return j->len_cap;
+    const int want = (seen + seen / 4 + 63) / 64 * 64;
+    return std::min(want, j->len_cap);
+}
+
 // One epoch's launches (src/lib.rs:961-1014: repulsion + attraction, gradient step, zero-mean):
 // the recentring, then the repulsion and the attraction (side by side on two streams when
 // `fork`, the attraction on `queue_blocks` blocks), then the update.
-void enqueue_epoch(bht_job *j, const LaunchSet &L, bool fit, bool fork, int queue_blocks) {
+void enqueue_epoch(bht_job *j, const LaunchSet &L0, bool fit, bool fork, int queue_blocks) {
+    LaunchSet L = L0;
+    if (fit) L.fft_cap = fft_cap_for(j);
     hipStream_t rs = fork ? j->side : j->stream;
+    // A FIt-SNE epoch after an update leaves the recentring to the readers of the embedding (one
+    // head kernel: the shift and the grid plan); otherwise the embedding is centred in place.
+    const bool lazy = fit && j->lazy_ready;
+    if (lazy) epoch_head(L, j->stream);
     if (fork) {
-        center_points(L, j->stream);
+        if (!lazy) center_points(L, j->stream);
         BHT_HIP(hipEventRecord(j->ev_fork, j->stream));
         BHT_HIP(hipStreamWaitEvent(j->side, j->ev_fork, 0));
-        // the repulsion reads the recentred y and the bounds' partials written on `stream`
+        // the repulsion reads y, the scalars and the bounds' partials written on `stream`
         l2_sync(j->side);
-        plan_grid(L, fit, j->side);
-    } else {
+        if (!lazy) plan_grid(L, fit, j->side);
+    } else if (!lazy) {
         center_plan(L, fit, j->stream);
     }
     j->mark(j->stream, 0);
@@ -935,6 +1015,7 @@ void enqueue_epoch(bht_job *j, const LaunchSet &L, bool fit, bool fork, int queu
         j->mark(j->stream, 2);
     }
     update_points(L, j->stream);
+    j->lazy_ready = fit;
     j->mark(j->stream, 3);
     if (j->timing) j->events_used += 4;
 }
@@ -978,9 +1059,10 @@ constexpr int TUNE_WARM = 1, TUNE_TIMED = 2;
 // Epoch-carried device state: both embedding buffers, uy, gains, the scalars (epoch counter,
 // pending recentring) and the update's partial sums (the next recentring reads them).
 struct StateSnapshot {
-    DevBuf y0, y1, uy, gains, scal, mpart;
+    DevBuf y0, y1, uy, gains, scal, mpart, fpart;
     int cur = 0;
     uint64_t epoch = 0;
+    bool lazy_ready = false;
 };
 
 void snapshot_take(bht_job *j, StateSnapshot &sn) {
@@ -996,8 +1078,10 @@ void snapshot_take(bht_job *j, StateSnapshot &sn) {
     copy(sn.gains, j->gains, b);
     copy(sn.scal, j->scal, sizeof(EpochScalars));
     copy(sn.mpart, j->mpart, 2 * (size_t)MAX_UPDATE_PARTS * sizeof(double));
+    copy(sn.fpart, j->fpart, (size_t)MAX_UPDATE_PARTS * sizeof(float4));
     sn.cur = j->cur;
     sn.epoch = j->epoch;
+    sn.lazy_ready = j->lazy_ready;
 }
 
 void snapshot_restore(bht_job *j, StateSnapshot &sn) {
@@ -1011,10 +1095,12 @@ void snapshot_restore(bht_job *j, StateSnapshot &sn) {
     copy(j->gains, sn.gains, b);
     copy(j->scal, sn.scal, sizeof(EpochScalars));
     copy(j->mpart, sn.mpart, 2 * (size_t)MAX_UPDATE_PARTS * sizeof(double));
+    copy(j->fpart, sn.fpart, (size_t)MAX_UPDATE_PARTS * sizeof(float4));
     l2_sync(j->stream);
     BHT_HIP(hipStreamSynchronize(j->stream));
     j->cur = sn.cur;
     j->epoch = sn.epoch;
+    j->lazy_ready = sn.lazy_ready;
 }
 
 TuneChoice autotune(bht_job *j, const LaunchSet &L, bool fit) {
@@ -1046,6 +1132,8 @@ TuneChoice autotune(bht_job *j, const LaunchSet &L, bool fit) {
                 enqueue_epoch(j, L, fit, cands[c].fork != 0, cands[c].queue_blocks);
                 j->cur = 1 - j->cur;
             }
+            // the timed epochs size their FFT launches by the grid the warm epoch reported
+            BHT_HIP(hipStreamSynchronize(j->stream));
             BHT_HIP(hipEventRecord(t0, j->stream));
             for (int e = 0; e < TUNE_TIMED; e++) {
                 enqueue_epoch(j, L, fit, cands[c].fork != 0, cands[c].queue_blocks);
@@ -1103,8 +1191,18 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     const TuneChoice ch = count > 0 ? epoch_choice(j, L, fit) : TuneChoice{0, 0};
     j->last_choice = ch.fork ? ch.queue_blocks : -1;
     if (j->timing) j->timed_fork = ch.fork != 0;
+    const bool bounded = fit && j->host_len != nullptr;
+    for (auto &r : j->ahead_rec) r = false;
     for (uint64_t e = 0; e < count; e++) {
+        // at most 16 epochs in flight (every 4th epoch records an event)
+        const int slot = (int)((e / 4) % 4);
+        if (bounded && e % 4 == 0 && j->ahead_rec[slot])
+            BHT_HIP(hipEventSynchronize(j->ahead[slot]));
         enqueue_epoch(j, L, fit, ch.fork != 0, ch.queue_blocks);
+        if (bounded && e % 4 == 0) {
+            BHT_HIP(hipEventRecord(j->ahead[slot], j->stream));
+            j->ahead_rec[slot] = true;
+        }
         j->cur = 1 - j->cur;
         const uint64_t epoch = j->epoch++;
         epoch_tail(j, epoch);
@@ -1397,7 +1495,7 @@ void batch_rebuild(bht_batch *b) {
         L.dim = g.dim;
         L.njobs = (int)g.jobs.size();
         L.tw = &c->tw;
-        L.fft_lds = (size_t)len_cap * sizeof(float2);
+        L.fft_cap = len_cap;
         L.keys = g.keys.p;
         L.keys_alt = g.keys_alt.p;
         L.vals = g.vals.as<uint32_t>();
@@ -1508,6 +1606,7 @@ void batch_admit(bht_batch *b, bht_job *j, int method, float theta) {
         prepare_run(j, method, theta, cached);
     }
     j->in_batch = true;
+    j->lazy_ready = false;
     b->members.push_back(j);
     b->dirty = true;
 }
@@ -1601,8 +1700,9 @@ bht_status bht_ctx_create(int device, uint64_t hbm_budget_bytes, bht_ctx **out) 
     });
 }
 
-void bht_ctx_destroy(bht_ctx *ctx) {
-    if (!ctx) return;
+namespace {
+
+void ctx_free(bht_ctx *ctx) {
     hipSetDevice(ctx->device);
     ctx->release(ctx->tw.table, ctx->tw.total * sizeof(float2));
     ctx->release(ctx->tw.len_of_boxes, (MAX_BOXES + 1) * sizeof(int));
@@ -1610,6 +1710,34 @@ void bht_ctx_destroy(bht_ctx *ctx) {
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     ctx->trim(0);
     delete ctx;
+}
+
+void ctx_acquire(bht_ctx *ctx) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->users++;
+}
+
+void ctx_drop(bht_ctx *ctx) {
+    bool last;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        last = --ctx->users == 0 && ctx->closed;
+    }
+    if (last) ctx_free(ctx);
+}
+
+}  // namespace
+
+void bht_ctx_destroy(bht_ctx *ctx) {
+    if (!ctx) return;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (ctx->users > 0) {
+            ctx->closed = true;
+            return;
+        }
+    }
+    ctx_free(ctx);
 }
 
 uint64_t bht_ctx_bytes_in_use(const bht_ctx *ctx) {
@@ -1626,13 +1754,16 @@ bht_status bht_job_create(bht_ctx *ctx, uint64_t n_samples, const bht_params *pa
         else bht_params_default(&p);
         if (n_samples >= (1ull << 31)) throw Error(ERR_INVALID, "n_samples must be below 2^31");
         *out = new bht_job(ctx, n_samples, p);
+        ctx_acquire(ctx);
     });
 }
 
 void bht_job_destroy(bht_job *job) {
     if (!job) return;
-    hipSetDevice(job->ctx->device);
-    delete job;
+    bht_ctx *ctx = job->ctx;
+    hipSetDevice(ctx->device);
+    delete job;  // returns its buffers to the context
+    ctx_drop(ctx);
 }
 
 bht_status bht_job_set_params(bht_job *job, const bht_params *params) {
@@ -1876,6 +2007,7 @@ bht_status bht_job_set_state(bht_job *job, const float *y, const float *uy, cons
         BHT_HIP(hipMemsetAsync(&sc->mean_pending, 0, sizeof(int), job->stream));
         l2_sync(job->stream);
         BHT_HIP(hipStreamSynchronize(job->stream));
+        job->lazy_ready = false;
     });
 }
 
@@ -2131,7 +2263,10 @@ bht_status bht_symmetrize(bht_ctx *ctx, const uint32_t *cols, const float *vals,
 
 bht_status bht_batch_create(bht_ctx *ctx, bht_batch **out) {
     if (!ctx || !out) return fail(BHT_ERR_INVALID, "null argument");
-    return guarded([&] { *out = new bht_batch(ctx); });
+    return guarded([&] {
+        *out = new bht_batch(ctx);
+        ctx_acquire(ctx);
+    });
 }
 
 void bht_batch_destroy(bht_batch *batch) {
@@ -2142,7 +2277,9 @@ void bht_batch_destroy(bht_batch *batch) {
         j->in_batch = false;
         j->prepared = false;
     }
+    bht_ctx *ctx = batch->ctx;
     delete batch;
+    ctx_drop(ctx);
 }
 
 bht_status bht_batch_add(bht_batch *batch, bht_job *job, uint32_t method, float theta) {
@@ -2231,6 +2368,8 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
         if (D == 2) b += 2 * nodes * len * 8;                     // packed spectra
         b += 2 * h * h * 4;                                       // kernel transforms
         b += 2 * ((n + 63) / 64) * 36 * 4;                        // spread partials
+        if (D == 2 && n >= DYN_FFT_MIN_POINTS)
+            b += (uint64_t)FFT_FB_BLOCKS * 2 * len * 8 + n * 8;   // FFT lines, sort output
         b += pad_tile(n) * 16 + group_sort_temp_bytes(pad_tile(n));  // batch arena
     }
     *bytes = b;

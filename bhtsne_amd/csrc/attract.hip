@@ -50,7 +50,8 @@ __device__ __forceinline__ RowState<D> load_row_state(const StepMem &M, int i) {
 template <int D>
 __device__ __forceinline__ void apply_update(const StepMem &M, const StepCtx &c, int i,
                                              const float *acc, const float *yi,
-                                             const RowState<D> &st, double &m0, double &m1) {
+                                             const RowState<D> &st, double &m0, double &m1,
+                                             float *ynew) {
 #pragma unroll
     for (int d = 0; d < D; d++) {
         const size_t k = (size_t)i * D + d;
@@ -64,6 +65,7 @@ __device__ __forceinline__ void apply_update(const StepMem &M, const StepCtx &c,
         M.gains[k] = gain;
         M.uy[k] = uy;
         M.y_next[k] = yn;
+        ynew[d] = yn;
         if (d == 0) m0 += (double)yn;
         else m1 += (double)yn;
     }
@@ -74,6 +76,7 @@ __device__ __forceinline__ void apply_update(const StepMem &M, const StepCtx &c,
 template <int D>
 __global__ void __launch_bounds__(UPD_THREADS) k_update(const JobDev *jobs, const uint32_t *map) {
     __shared__ double red[UPD_THREADS / 64];
+    __shared__ float fred[4][UPD_THREADS / 64];
     const BlockCtx bc = block_ctx(map, jobs, F_PT);
     const JobDev &J = jobs[bc.job];
     const int epoch = J.scal->epoch;
@@ -85,20 +88,40 @@ __global__ void __launch_bounds__(UPD_THREADS) k_update(const JobDev *jobs, cons
     M.frep = global_view((const float *)J.frep);
     BHT_GLOBAL const float *y = global_view((const float *)J.ybuf[epoch & 1]);
     BHT_GLOBAL const float *fattr = global_view((const float *)J.fattr);
+    const float sh[2] = {J.scal->ysub[0], J.scal->ysub[1]};
     double m0 = 0.0, m1 = 0.0;
+    float mn[2] = {INFINITY, INFINITY}, mx[2] = {-INFINITY, -INFINITY};
     const int stride = bc.nb * UPD_THREADS;
     for (int i = bc.lb * UPD_THREADS + threadIdx.x; i < J.n; i += stride) {
         float acc[2] = {0.0f, 0.0f}, yi[2] = {0.0f, 0.0f};
 #pragma unroll
         for (int d = 0; d < D; d++) {
             acc[d] = fattr[(size_t)i * D + d];
-            yi[d] = y[(size_t)i * D + d];
+            yi[d] = y[(size_t)i * D + d] - sh[d];
         }
-        apply_update<D>(M, sc, i, acc, yi, load_row_state<D>(M, i), m0, m1);
+        float yn[2];
+        apply_update<D>(M, sc, i, acc, yi, load_row_state<D>(M, i), m0, m1, yn);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            mn[d] = fminf(mn[d], yn[d]);
+            mx[d] = fmaxf(mx[d], yn[d]);
+        }
     }
     const double s0 = block_sum_f64(m0, red);
     const double s1 = D == 2 ? block_sum_f64(m1, red) : 0.0;
+    // per-block bounds of the new embedding (the next epoch's grid plan when it is centred lazily)
+    float b[4] = {wave_min_f32(mn[0]), wave_max_f32(mx[0]), wave_min_f32(mn[1]),
+                  wave_max_f32(mx[1])};
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < 4; q++) fred[q][w] = b[q];
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int q = 1; q < UPD_THREADS / 64; q++) {
+            b[0] = fminf(b[0], fred[0][q]); b[1] = fmaxf(b[1], fred[1][q]);
+            b[2] = fminf(b[2], fred[2][q]); b[3] = fmaxf(b[3], fred[3][q]);
+        }
+        reinterpret_cast<float4 *>(J.fpart)[bc.lb] = make_float4(b[0], b[1], b[2], b[3]);
         J.mpart[2 * bc.lb] = s0;
         J.mpart[2 * bc.lb + 1] = s1;
         if (bc.lb == 0) {

@@ -24,9 +24,13 @@ int point_blocks(int n) {
 // parallelism (4 rounds cost C4's plain fits 8%).
 __host__ __device__ inline int spread_rounds(int n) { return n >= 65536 ? 4 : 1; }
 
+// Two waves per spreading block: the block's LDS slab (20 KiB) then fits a CU beside two of the
+// attraction's blocks.
+constexpr int SPR_THREADS = 128;
+
 int spread_blocks(int n) {
     const uint64_t nchunks = ceil_div((uint64_t)(n > 0 ? n : 1), 64 * spread_rounds(n));
-    return (int)std::min<uint64_t>(ceil_div(nchunks, PT_THREADS / 64), 2048);
+    return (int)std::min<uint64_t>(ceil_div(nchunks, SPR_THREADS / 64), 4096);
 }
 
 // Grid sizes of every kernel family for one job.  A job alone fills the device by itself; in a
@@ -103,11 +107,11 @@ __device__ __forceinline__ int locate_box(float v, const GridPlan &p, float &loc
 // advances the job's epoch index (which selects its current embedding buffer).
 constexpr int FIN_THREADS = 1024;  // up to 16384 attraction blocks' partials per job
 
-__global__ void __launch_bounds__(FIN_THREADS) k_mean_finalize(const JobDev *jobs,
-                                                               const uint32_t *map) {
-    __shared__ double red[FIN_THREADS / 64];
-    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
-    const JobDev &J = jobs[bc.job];
+//
+// The shift is either applied at once by k_center (the embedding buffer then holds centred
+// coordinates and ysub is 0), or left pending (lazy: ysub = mean, and every reader of the epoch's
+// embedding takes fl(y - ysub), the value k_center would have stored).
+__device__ void mean_body(const JobDev &J, int lazy, double *red) {
     EpochScalars *scal = J.scal;
     const double *mpart = J.mpart;
     const int nparts = J.nblk[F_PT], n = J.n;  // partials of k_update
@@ -116,22 +120,34 @@ __global__ void __launch_bounds__(FIN_THREADS) k_mean_finalize(const JobDev *job
         if (threadIdx.x == 0) {
             scal->mean[0] = 0.0f;
             scal->mean[1] = 0.0f;
+            scal->ysub[0] = 0.0f;
+            scal->ysub[1] = 0.0f;
         }
         return;
     }
     double t0 = 0.0, t1 = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += FIN_THREADS) {
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         t0 += mpart[2 * b];
         t1 += mpart[2 * b + 1];
     }
     t0 = block_sum_f64(t0, red);
     t1 = block_sum_f64(t1, red);
     if (threadIdx.x == 0) {
-        scal->mean[0] = (float)(t0 / (double)n);
-        scal->mean[1] = (float)(t1 / (double)n);
+        const float m0 = (float)(t0 / (double)n), m1 = (float)(t1 / (double)n);
+        scal->mean[0] = m0;
+        scal->mean[1] = m1;
+        scal->ysub[0] = lazy ? m0 : 0.0f;
+        scal->ysub[1] = lazy ? m1 : 0.0f;
         scal->mean_pending = 0;
         scal->epoch = scal->epoch_next;
     }
+}
+
+__global__ void __launch_bounds__(FIN_THREADS) k_mean_finalize(const JobDev *jobs,
+                                                               const uint32_t *map) {
+    __shared__ double red[FIN_THREADS / 64];
+    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
+    mean_body(jobs[bc.job], 0, red);
 }
 
 // y -= mean; per-block, per-axis min / max.
@@ -174,33 +190,34 @@ __global__ void __launch_bounds__(PT_THREADS) k_center(const JobDev *jobs, const
     }
 }
 
-// Combines the bounds; with do_plan, decides the FIt-SNE grid of this epoch.
+// Combines the bounds (per-block partials of the embedding, less the pending shift); with
+// do_plan, decides the FIt-SNE grid of this epoch.  All threads of the block.
 template <int D>
-__global__ void __launch_bounds__(256) k_plan(const JobDev *jobs, const uint32_t *map,
-                                              const int *len_of_boxes, const int *off_of_boxes,
-                                              int do_plan) {
-    __shared__ float red[4][4];
-    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
-    const JobDev &J = jobs[bc.job];
+__device__ void plan_body(const JobDev &J, const int *len_of_boxes, const int *off_of_boxes,
+                          int do_plan, float (*red)[16]) {
     EpochScalars *scal = J.scal;
     const float *fpart = J.fpart;
     const int nparts = J.nblk[F_PT], min_boxes = J.min_boxes, max_boxes = J.max_boxes;
     float mn0 = INFINITY, mx0 = -INFINITY, mn1 = INFINITY, mx1 = -INFINITY;
-    for (int b = threadIdx.x; b < nparts; b += 256) {
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         float4 pv = reinterpret_cast<const float4 *>(fpart)[b];
         mn0 = fminf(mn0, pv.x); mx0 = fmaxf(mx0, pv.y);
         mn1 = fminf(mn1, pv.z); mx1 = fmaxf(mx1, pv.w);
     }
     mn0 = wave_min_f32(mn0); mx0 = wave_max_f32(mx0);
     mn1 = wave_min_f32(mn1); mx1 = wave_max_f32(mx1);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     if (lane == 0) { red[0][w] = mn0; red[1][w] = mx0; red[2][w] = mn1; red[3][w] = mx1; }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int q = 1; q < 4; q++) {
+    for (int q = 1; q < nw; q++) {
         mn0 = fminf(mn0, red[0][q]); mx0 = fmaxf(mx0, red[1][q]);
         mn1 = fminf(mn1, red[2][q]); mx1 = fmaxf(mx1, red[3][q]);
     }
+    // fl(. - s) is monotone, so the bounds of the shifted embedding are the shifted bounds
+    const float s0 = scal->ysub[0], s1 = scal->ysub[1];
+    mn0 = mn0 - s0; mx0 = mx0 - s0;
+    mn1 = mn1 - s1; mx1 = mx1 - s1;
     scal->amin[0] = mn0; scal->amax[0] = mx0;
     scal->amin[1] = mn1; scal->amax[1] = mx1;
     if (!do_plan || J.method != M_FITSNE) return;
@@ -232,6 +249,50 @@ __global__ void __launch_bounds__(256) k_plan(const JobDev *jobs, const uint32_t
     frexpf(fmaxf(fmaxf(fabsf(lo), fabsf(hi)), 1.0f), &e2);
     p.qscale = ldexpf(1.0f, e2);
     scal->plan = p;
+    // 7-bit passes of the box sort: keys are below boxes^D
+    const uint32_t nkeys = D == 2 ? (uint32_t)boxes * (uint32_t)boxes : (uint32_t)boxes;
+    int kbits = 0;
+    while (kbits < 32 && (1u << kbits) < nkeys) kbits++;
+    scal->sort_passes = kbits <= 7 ? 1 : (kbits + 6) / 7;
+    // the host sizes later FFT launches by the length (a system-scope store to pinned memory)
+    if (J.host_len)
+        __hip_atomic_store(J.host_len, p.len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_plan(const JobDev *jobs, const uint32_t *map,
+                                              const int *len_of_boxes, const int *off_of_boxes,
+                                              int do_plan) {
+    __shared__ float red[4][16];
+    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
+    plan_body<D>(jobs[bc.job], len_of_boxes, off_of_boxes, do_plan, red);
+}
+
+// The head of a lazily centred FIt-SNE epoch: the mean of the last update's sums left pending
+// (ysub), then the bounds and the grid plan from the update's per-block bounds.
+template <int D>
+__global__ void __launch_bounds__(FIN_THREADS) k_epoch_head(const JobDev *jobs,
+                                                            const uint32_t *map,
+                                                            const int *len_of_boxes,
+                                                            const int *off_of_boxes) {
+    __shared__ double dred[FIN_THREADS / 64];
+    __shared__ float red[4][16];
+    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
+    const JobDev &J = jobs[bc.job];
+    mean_body(J, 1, dred);
+    __syncthreads();
+    plan_body<D>(J, len_of_boxes, off_of_boxes, 1, red);
+}
+
+void epoch_head(const LaunchSet &L, hipStream_t s) {
+    const int *lo = L.tw->len_of_boxes, *of = L.tw->off_of_boxes;
+    if (L.dim == 2)
+        hipLaunchKernelGGL(k_epoch_head<2>, dim3(L.nb[F_ONE]), dim3(FIN_THREADS), 0, s, L.jobs,
+                           L.map[F_ONE], lo, of);
+    else
+        hipLaunchKernelGGL(k_epoch_head<1>, dim3(L.nb[F_ONE]), dim3(FIN_THREADS), 0, s, L.jobs,
+                           L.map[F_ONE], lo, of);
+    BHT_LAUNCH_CHECK();
 }
 
 void center_points(const LaunchSet &L, hipStream_t s) {
@@ -298,6 +359,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_box_keys(const JobDev *jobs,
     const JobDev &J = jobs[bc.job];
     const GridPlan p = J.scal->plan;
     const float *y = job_y(J);
+    const float s0 = J.scal->ysub[0], s1 = J.scal->ysub[1];
     uint32_t *keys = J.keys, *vals = J.kidx;
     const int n = J.n;
     const int stride = bc.nb * PT_THREADS;
@@ -306,11 +368,13 @@ __global__ void __launch_bounds__(PT_THREADS) k_box_keys(const JobDev *jobs,
         uint32_t key;
         if (D == 2) {
             float2 v = reinterpret_cast<const float2 *>(y)[i];
+            v.x = v.x - s0;
+            v.y = v.y - s1;
             uint32_t b0 = (uint32_t)locate_box(v.x, p, l);
             uint32_t b1 = (uint32_t)locate_box(v.y, p, l);
             key = b0 * (uint32_t)p.boxes + b1;
         } else {
-            key = (uint32_t)locate_box(y[i], p, l);
+            key = (uint32_t)locate_box(y[i] - s0, p, l);
         }
         keys[i] = key;
         vals[i] = (uint32_t)i;
@@ -342,10 +406,12 @@ constexpr int NV2 = 36;  // 9 nodes x 4 terms (D = 2)
 constexpr int NV1 = 9;   // 3 nodes x 3 terms (D = 1)
 
 template <int D>
-__device__ __forceinline__ void point_contrib(const float *y, uint32_t i, const GridPlan &p,
-                                              const Lagrange3 &lb, float *v) {
+__device__ __forceinline__ void point_contrib(const float *y, uint32_t i, float s0, float s1,
+                                              const GridPlan &p, const Lagrange3 &lb, float *v) {
     if (D == 2) {
         float2 pt = reinterpret_cast<const float2 *>(y)[i];
+        pt.x = pt.x - s0;
+        pt.y = pt.y - s1;
         float l0, l1, w0[3], w1[3];
         locate_box(pt.x, p, l0);
         locate_box(pt.y, p, l1);
@@ -365,7 +431,7 @@ __device__ __forceinline__ void point_contrib(const float *y, uint32_t i, const 
                 for (int t = 0; t < 4; t++) v[(k0 + 3 * k1) * 4 + t] = wt * c[t];
             }
     } else {
-        float pt = y[i];
+        float pt = y[i] - s0;
         float l0, w0[3];
         locate_box(pt, p, l0);
         lb.weights(l0, w0);
@@ -416,12 +482,13 @@ constexpr int SLAB_ROW = 40;             // floats per point row of the spreadin
 constexpr int SLAB_ROW4 = SLAB_ROW / 4;
 
 template <int D>
-__global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const uint32_t *map) {
+__global__ void __launch_bounds__(SPR_THREADS) k_spread(const JobDev *jobs, const uint32_t *map) {
     constexpr int NV = D == 2 ? NV2 : NV1;
     const BlockCtx bc = block_ctx(map, jobs, F_SPR);
     const JobDev &J = jobs[bc.job];
     const GridPlan p = J.scal->plan;
     const float *y = job_y(J);
+    const float ys0 = J.scal->ysub[0], ys1 = J.scal->ysub[1];
     const uint32_t *keys = J.skeys, *vals = J.sidx;
     float4 *grid = J.grid;
     float *part = J.part;
@@ -430,12 +497,12 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const
     const int nchunks = (n + chunk - 1) / chunk;
     const Lagrange3 lb;
     const int lane = threadIdx.x & 63;
-    const int wave0 = (bc.lb * PT_THREADS + threadIdx.x) >> 6;
-    const int nwaves = (bc.nb * PT_THREADS) >> 6;
+    const int wave0 = (bc.lb * SPR_THREADS + threadIdx.x) >> 6;
+    const int nwaves = (bc.nb * SPR_THREADS) >> 6;
     float *part_first = part;
     float *part_last = part + (size_t)nchunks * NV;
     // a round's contributions, one row per point; lane t then sums term t down the rows
-    __shared__ float4 slab_mem[PT_THREADS / 64][64][SLAB_ROW4];
+    __shared__ float4 slab_mem[SPR_THREADS / 64][64][SLAB_ROW4];
     float *slab = reinterpret_cast<float *>(slab_mem[threadIdx.x >> 6]);
     for (int c = wave0; c < nchunks; c += nwaves) {
         const int c0 = c * chunk;
@@ -451,7 +518,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const
             {
                 float v[NV];
                 if (valid) {
-                    point_contrib<D>(y, vals[s], p, lb, v);
+                    point_contrib<D>(y, vals[s], ys0, ys1, p, lb, v);
                 } else {
 #pragma unroll
                     for (int t = 0; t < NV; t++) v[t] = 0.0f;
@@ -523,7 +590,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread(const JobDev *jobs, const
 // by a 64-way search of the sorted keys and sums the chunk partials with the lanes (strided, then a
 // fixed butterfly), so a run of any length costs O(length / 64 / 64) steps.
 template <int D>
-__global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const JobDev *jobs,
+__global__ void __launch_bounds__(SPR_THREADS) k_spread_fixup(const JobDev *jobs,
                                                              const uint32_t *map) {
     constexpr int NV = D == 2 ? NV2 : NV1;
     const BlockCtx bc = block_ctx(map, jobs, F_SPR);
@@ -537,8 +604,8 @@ __global__ void __launch_bounds__(PT_THREADS) k_spread_fixup(const JobDev *jobs,
     const float *part_first = J.part;
     const float *part_last = J.part + (size_t)nchunks * NV;
     const int lane = threadIdx.x & 63;
-    const int wave0 = (bc.lb * PT_THREADS + threadIdx.x) >> 6;
-    const int nwaves = (bc.nb * PT_THREADS) >> 6;
+    const int wave0 = (bc.lb * SPR_THREADS + threadIdx.x) >> 6;
+    const int nwaves = (bc.nb * SPR_THREADS) >> 6;
     for (int c = wave0; c < nchunks; c += nwaves) {
         const int last_s = min(c * chunk + chunk - 1, n - 1);
         const uint32_t key = keys[last_s];
@@ -584,6 +651,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_gather(const JobDev *jobs, const
     const JobDev &J = jobs[bc.job];
     const GridPlan p = J.scal->plan;
     const float *y = job_y(J);
+    const float s0 = J.scal->ysub[0], s1 = J.scal->ysub[1];
     const float4 *grid = J.grid;
     float *frep = J.frep;
     const int n = J.n;
@@ -593,6 +661,8 @@ __global__ void __launch_bounds__(PT_THREADS) k_gather(const JobDev *jobs, const
     for (int i = bc.lb * PT_THREADS + threadIdx.x; i < n; i += stride) {
         if (D == 2) {
             float2 pt = reinterpret_cast<const float2 *>(y)[i];
+            pt.x = pt.x - s0;
+            pt.y = pt.y - s1;
             float l0, l1, w0[3], w1[3];
             int b0 = locate_box(pt.x, p, l0);
             int b1 = locate_box(pt.y, p, l1);
@@ -622,7 +692,7 @@ __global__ void __launch_bounds__(PT_THREADS) k_gather(const JobDev *jobs, const
             float zt = (1.0f + norm_sq) * phi0 - 2.0f * cross + phi3;
             zacc += (double)zt;
         } else {
-            float pt = y[i];
+            float pt = y[i] - s0;
             float l0, w0[3];
             int b0 = locate_box(pt, p, l0);
             lb.weights(l0, w0);
@@ -657,11 +727,11 @@ const uint32_t *fitsne_repulsion(const LaunchSet &L, hipStream_t s) {
     const bool alt = sort_group<uint32_t>(L, box_key_bits(L.dim), s);
     hipLaunchKernelGGL(k_zero_grid, zero, dim3(256), 0, s, L.jobs, L.map[F_ZERO]);
     if (L.dim == 2) {
-        hipLaunchKernelGGL(k_spread<2>, spr, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_SPR]);
-        hipLaunchKernelGGL(k_spread_fixup<2>, spr, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_SPR]);
+        hipLaunchKernelGGL(k_spread<2>, spr, dim3(SPR_THREADS), 0, s, L.jobs, L.map[F_SPR]);
+        hipLaunchKernelGGL(k_spread_fixup<2>, spr, dim3(SPR_THREADS), 0, s, L.jobs, L.map[F_SPR]);
     } else {
-        hipLaunchKernelGGL(k_spread<1>, spr, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_SPR]);
-        hipLaunchKernelGGL(k_spread_fixup<1>, spr, dim3(PT_THREADS), 0, s, L.jobs, L.map[F_SPR]);
+        hipLaunchKernelGGL(k_spread<1>, spr, dim3(SPR_THREADS), 0, s, L.jobs, L.map[F_SPR]);
+        hipLaunchKernelGGL(k_spread_fixup<1>, spr, dim3(SPR_THREADS), 0, s, L.jobs, L.map[F_SPR]);
     }
     BHT_LAUNCH_CHECK();
     fft_kernel_spectrum(L, s);
@@ -673,6 +743,7 @@ const uint32_t *fitsne_repulsion(const LaunchSet &L, hipStream_t s) {
     BHT_LAUNCH_CHECK();
     finalize_z(L, F_PT, s);
     // point indices of job 0 in box order: a spatial order of the current embedding
+    if (L.sort_npass) return L.vals_fin;
     return alt ? L.vals_alt : L.vals;
 }
 

@@ -29,8 +29,10 @@ struct alignas(128) EpochScalars {
     GridPlan plan;
     double zsum;           // sum of the per-point Z terms (FP64)
     float z, inv_z;        // Z and 1/Z of this epoch
+    int sort_passes;       // 7-bit passes the box sort needs for this epoch's grid
     // -- written by the centring, attraction and update kernels and at set-up
     alignas(128) float mean[2]; // pending recentring shift (applied by the next centring kernel)
+    float ysub[2];         // shift every reader subtracts from the epoch's embedding buffer
     int epoch;             // index of the epoch being executed
     int pad0;
     double psum;           // FP64 sum of P (normalisation)
@@ -120,6 +122,12 @@ struct JobDev {
     float *rker;        // kernel row transform, (len_cap/2+1)^2
     float *kspec;       // kernel spectrum quarter, (len_cap/2+1)^2
     float *part;        // spread partials, 2 x nchunks x 36
+    // FFT line buffers for launches whose LDS is smaller than the grid has grown (two lines of
+    // len_cap points per fallback block; null when the launches always hold len_cap), and the
+    // host-visible
+    // word the grid plan reports its FFT length to (null: none)
+    float2 *fft_fb;
+    int *host_len;
     // Barnes-Hut: Morton sort input and output, tree
     uint64_t *codes;
     uint32_t *cidx;
@@ -141,6 +149,8 @@ __device__ __forceinline__ BlockCtx block_ctx(const uint32_t *map, const JobDev 
 
 __device__ __forceinline__ int job_epoch(const JobDev &J) { return J.scal->epoch; }
 __device__ __forceinline__ float *job_y(const JobDev &J) { return J.ybuf[J.scal->epoch & 1]; }
+
+constexpr int FFT_FB_BLOCKS = 64;  // blocks of an FFT launch that transform lines too long for LDS
 
 // Block counts of one job (the same formulas size the single-job grids and the batch maps).
 int point_blocks(int n);
@@ -184,7 +194,7 @@ struct LaunchSet {
     int nb[NFAM] = {};                  // grid size of each family
     int dim = 2;
     int njobs = 1;
-    size_t fft_lds = 0;                 // dynamic LDS of the FFT kernels (max len_cap)
+    int fft_cap = 0;                    // FFT line points per block the launches' LDS holds
     const TwiddleSet *tw = nullptr;
     // sort arena of the group (u32 box keys for FIt-SNE, u64 Morton codes for Barnes-Hut)
     void *keys = nullptr, *keys_alt = nullptr;
@@ -194,6 +204,11 @@ struct LaunchSet {
     const SortSeg *segs = nullptr;      // null: one unpadded segment of sort_n keys
     const uint32_t *tile_seg = nullptr; // segment of each tile
     uint32_t sort_tiles = 0;            // tiles of the segmented sort
+    // a single job's box sort with its pass count decided on the device (the grid plan's
+    // sort_passes); its result is then in (keys_fin, vals_fin)
+    const int *sort_npass = nullptr;
+    void *keys_fin = nullptr;
+    uint32_t *vals_fin = nullptr;
 };
 
 // ---- cross-stream cache maintenance (sync.hip) -----------------------------------------------
@@ -242,6 +257,9 @@ void center_plan(const LaunchSet &L, bool do_plan, hipStream_t s);
 // with the grid plan (which only the repulsion reads).
 void center_points(const LaunchSet &L, hipStream_t s);
 void plan_grid(const LaunchSet &L, bool do_plan, hipStream_t s);
+// Head of a lazily centred FIt-SNE epoch (the recentring shift left to the readers) and its grid
+// plan, from the last update's sums and bounds.
+void epoch_head(const LaunchSet &L, hipStream_t s);
 // Z and 1/Z from the per-block partial sums (minus n for FIt-SNE).
 void finalize_z(const LaunchSet &L, int part_family, hipStream_t s);
 // after center_plan: frep, Z; returns the sorted point indices of job 0 (a spatial order)

@@ -163,14 +163,22 @@ __device__ __forceinline__ TileCtx tile_ctx(uint64_t n, unsigned ntiles, const S
     return c;
 }
 
+// A sort whose pass count is decided on the device: passes from *npass on do nothing (every key's
+// digit there is 0), and pass *npass - 1 writes the final buffers.
+__device__ __forceinline__ bool pass_skipped(const int *npass, int pass) {
+    return npass != nullptr && pass >= *npass;
+}
+
 // Digit counts of one tile (BITS-bit digits at `shift`).
 template <typename K, int BITS>
 __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K *keys, uint64_t n, int shift,
                                                         uint32_t *hist, unsigned ntiles,
                                                         const SortSeg *segs,
-                                                        const uint32_t *tile_seg) {
+                                                        const uint32_t *tile_seg,
+                                                        const int *npass, int pass) {
     constexpr int BINS = 1 << BITS;
     __shared__ uint32_t h[BINS];
+    if (pass_skipped(npass, pass)) return;
     const TileCtx tc = tile_ctx<BINS>(n, ntiles, segs, tile_seg);
     for (int b = threadIdx.x; b < BINS; b += RS_THREADS) h[b] = 0;
     __syncthreads();
@@ -188,8 +196,10 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K *keys, uint64_t 
 template <int BITS>
 __global__ void __launch_bounds__(256) k_rs_digit_scan(uint32_t *hist, uint32_t *totals,
                                                        unsigned ntiles, const SortSeg *segs,
-                                                       unsigned nsegs) {
+                                                       unsigned nsegs, const int *npass,
+                                                       int pass) {
     constexpr unsigned BINS = 1u << BITS;
+    if (pass_skipped(npass, pass)) return;
     const unsigned lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
     if (wave >= (uint64_t)nsegs * BINS) return;
@@ -267,9 +277,16 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K *keys, const 
                                                            const uint32_t *offs,
                                                            const uint32_t *totals,
                                                            unsigned ntiles, const SortSeg *segs,
-                                                           const uint32_t *tile_seg) {
+                                                           const uint32_t *tile_seg,
+                                                           const int *npass, int pass, K *keys_fin,
+                                                           uint32_t *vals_fin) {
     constexpr int BINS = 1 << BITS;
     __shared__ uint32_t base[BINS];
+    if (pass_skipped(npass, pass)) return;
+    if (npass != nullptr && pass == *npass - 1) {
+        keys_out = keys_fin;
+        vals_out = vals_fin;
+    }
     __shared__ uint32_t wcnt[RS_THREADS / 64][BINS];
     __shared__ uint32_t wsum[RS_THREADS / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -342,13 +359,15 @@ size_t group_sort_temp_bytes(uint64_t n) { return temp_bytes_bins(n, 1 << GROUP_
 
 // Sorts (keys, vals) by bits [begin_bit, end_bit) of the key, stable, BITS bits per pass.
 // Ping-pongs between the two buffer pairs; returns true when the result is in (keys_alt,
-// vals_alt).
+// vals_alt).  With npass (device), the passes from *npass on are skipped and the result is in
+// (keys_fin, vals_fin).
 // Three launches per pass: tile counts, their per-digit scan, the scatter.
 template <typename K, int BITS>
 static bool radix_passes(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, uint64_t n,
                          unsigned ntiles, const SortSeg *segs, unsigned nsegs,
                          const uint32_t *tile_seg, int begin_bit, int end_bit, void *temp,
-                         hipStream_t s) {
+                         hipStream_t s, const int *npass = nullptr, K *keys_fin = nullptr,
+                         uint32_t *vals_fin = nullptr) {
     const uint64_t nh = (uint64_t)ntiles << BITS;
     uint32_t *hist = (uint32_t *)temp;
     uint32_t *totals = hist + nh;
@@ -356,13 +375,15 @@ static bool radix_passes(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_al
     K *ki = keys, *ko = keys_alt;
     uint32_t *vi = vals, *vo = vals_alt;
     bool alt = false;
-    for (int shift = begin_bit; shift < end_bit; shift += BITS) {
+    int pass = 0;
+    for (int shift = begin_bit; shift < end_bit; shift += BITS, pass++) {
         hipLaunchKernelGGL((k_rs_hist<K, BITS>), dim3(ntiles), dim3(RS_THREADS), 0, s, ki, n,
-                           shift, hist, ntiles, segs, tile_seg);
+                           shift, hist, ntiles, segs, tile_seg, npass, pass);
         hipLaunchKernelGGL((k_rs_digit_scan<BITS>), dim3(scan_blocks), dim3(256), 0, s, hist,
-                           totals, ntiles, segs, nsegs);
+                           totals, ntiles, segs, nsegs, npass, pass);
         hipLaunchKernelGGL((k_rs_scatter<K, BITS>), dim3(ntiles), dim3(RS_THREADS), 0, s, ki, vi,
-                           ko, vo, n, shift, hist, totals, ntiles, segs, tile_seg);
+                           ko, vo, n, shift, hist, totals, ntiles, segs, tile_seg, npass, pass,
+                           keys_fin, vals_fin);
         BHT_LAUNCH_CHECK();
         K *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;
@@ -393,7 +414,7 @@ static bool sort_group_bits(const LaunchSet &L, int end_bit, hipStream_t s) {
     if (L.sort_n == 0) return false;
     return radix_passes<K, BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt, L.sort_n,
                                  ceil_div(L.sort_n, RS_TILE), nullptr, 1, nullptr, 0, end_bit,
-                                 L.sort_temp, s);
+                                 L.sort_temp, s, L.sort_npass, (K *)L.keys_fin, L.vals_fin);
 }
 
 template <typename K>

@@ -200,22 +200,28 @@ __device__ __forceinline__ float2 lds_pos(const float *a, uint32_t i) {
     return make_float2(lds_view(a)[i], 0.0f);
 }
 
-// Copies points [first, first + count) of y (D floats each) into LDS; 16-byte loads when the
-// start is 16-byte aligned.
+// Copies points [first, first + count) of y (D floats each) into LDS, less the epoch's pending
+// recentring shift (s0, s1); 16-byte loads when the start is 16-byte aligned.
 template <int D>
 __device__ void stage_points(BHT_GLOBAL const float *y, uint32_t first, uint32_t count,
-                             float *dst) {
+                             float *dst, float s0, float s1) {
     const uint32_t nf = count * D;
     BHT_GLOBAL const float *src = y + (size_t)first * D;
     BHT_LDS float *out = lds_view(dst);
+    // coordinate t of the copy is axis t % D (the start is a point boundary)
+    const f4v sub = D == 2 ? f4v{s0, s1, s0, s1} : f4v{s0, s0, s0, s0};
     if (((first * D) & 3u) == 0) {
         for (uint32_t k = threadIdx.x; 4 * k < nf; k += T_THREADS) {
-            if (4 * k + 3 < nf) ((BHT_LDS f4v *)out)[k] = ((BHT_GLOBAL const f4v *)src)[k];
-            else
-                for (uint32_t t = 4 * k; t < nf; t++) out[t] = src[t];
+            if (4 * k + 3 < nf) {
+                ((BHT_LDS f4v *)out)[k] = ((BHT_GLOBAL const f4v *)src)[k] - sub;
+            } else {
+                for (uint32_t t = 4 * k; t < nf; t++)
+                    out[t] = src[t] - (D == 2 && (t & 1u) ? s1 : s0);
+            }
         }
     } else {
-        for (uint32_t k = threadIdx.x; k < nf; k += T_THREADS) out[k] = src[k];
+        for (uint32_t k = threadIdx.x; k < nf; k += T_THREADS)
+            out[k] = src[k] - (D == 2 && (k & 1u) ? s1 : s0);
     }
 }
 
@@ -338,6 +344,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
     // read the rescaled values rescales them in place as it reads
     const bool stop_now = EPOCH && J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
     const float inv_exag = J.inv_exag;
+    const float ys0 = J.scal->ysub[0], ys1 = J.scal->ysub[1];
 
     // queued: the blocks take row blocks from the job's work queue (reset by the update kernel);
     // otherwise each block has its own row block
@@ -351,7 +358,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
     while (lb < nrb) {
         const uint32_t row0 = lb * R;
         const uint32_t nrows = row0 < n ? (n - row0 < R ? n - row0 : R) : 0u;
-        stage_points<D>(y, row0, nrows, yrow);
+        stage_points<D>(y, row0, nrows, yrow, ys0, ys1);
         for (uint32_t k = threadIdx.x; k < nrows * 2; k += T_THREADS) lds_view(acc)[k] = 0.0f;
         const uint32_t s_begin = blk_seg[lb], s_end = blk_seg[lb + 1];
         for (uint32_t s = s_begin; s < s_end; s++) {
@@ -378,7 +385,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
             };
             if (gc < g_end) open_groups();
             __syncthreads();  // the previous tile's reads are done
-            stage_points<D>(y, first, count, win);
+            stage_points<D>(y, first, count, win, ys0, ys1);
             __syncthreads();
             while (gc < g_end) {
                 while (true) {

@@ -92,6 +92,11 @@ typedef struct bht_params {
  * the job's state, which is then restored; every candidate gives the same result bits).  This
  * flag uses the built-in form instead. */
 #define BHT_FLAG_NO_AUTOTUNE 4u
+/* A 2-D FIt-SNE job of 32768 points or more sizes the LDS of its FFT launches for the lengths its
+ * grid has reached (so that they fit a CU beside the attraction), and transforms a line that has
+ * outgrown them in global memory with the same arithmetic.  This flag sizes them for 64 points, so
+ * that every line takes the global-memory path (a check of that path; results are the same bits). */
+#define BHT_FLAG_FFT_GLOBAL_LINES 8u
 
 void bht_params_default(bht_params *p);
 

@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 
 SERIAL = 2  # BHT_FLAG_SERIAL_EPOCH
 NO_AUTOTUNE = 4  # BHT_FLAG_NO_AUTOTUNE
+GLOBAL_LINES = 8  # BHT_FLAG_FFT_GLOBAL_LINES
 
 
 @pytest.fixture(scope="module")
@@ -44,15 +45,20 @@ def test_forked_fit_repeats_and_equals_serial(bt, request_id, epochs):
     b, kb = _fit(bt, x, y0, aff, 0, epochs, d_out)
     c, kc = _fit(bt, x, y0, aff, SERIAL, epochs, d_out)
     d, kd = _fit(bt, x, y0, aff, NO_AUTOTUNE, epochs, d_out)
+    # every FFT line transformed in global memory instead of LDS: the same arithmetic
+    e, ke = _fit(bt, x, y0, aff, GLOBAL_LINES, epochs, d_out)
     diff_ab = int((a != b).sum())
     diff_ac = int((a != c).sum())
     diff_ad = int((a != d).sum())
+    diff_ae = int((a != e).sum())
     print(f"forked vs forked: {diff_ab} differing values; forked vs serial: {diff_ac}; "
-          f"autotuned vs built-in form: {diff_ad}; KL {ka} {kb} {kc} {kd}", flush=True)
+          f"autotuned vs built-in form: {diff_ad}; LDS vs global FFT lines: {diff_ae}; "
+          f"KL {ka} {kb} {kc} {kd} {ke}", flush=True)
     assert np.all(np.isfinite(a))
     assert diff_ab == 0 and ka == kb
     assert diff_ac == 0 and ka == kc
     assert diff_ad == 0 and ka == kd
+    assert diff_ae == 0 and ka == ke
 
 
 def test_autotuner_choice_is_reported_and_cached(bt):
