@@ -45,6 +45,45 @@ def test_binding_signatures_cover_the_header():
     assert set(_declared()) == set(_lib.SIGNATURES)
 
 
+def _header_prototypes():
+    """name -> number of parameters of every function the header declares."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(bht_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def _header_constants():
+    text = open(HEADER).read()
+    return {m.group(1): int(m.group(2))
+            for m in re.finditer(r"#define\s+(BHT_[A-Z0-9_]+)\s+(\d+)u?\b", text)}
+
+
+def test_rust_sys_binding_covers_the_header():
+    """The Rust -sys crate (not compiled here: no cargo) declares every function of the header
+    with the same number of parameters, and every constant with the header's value."""
+    src = open(os.path.join(ROOT, "rust", "bhtsne-hip-sys", "src", "lib.rs")).read()
+    src = re.sub(r"//[^\n]*", "", src)
+    fns = {}
+    for m in re.finditer(r"pub fn\s+(bht_[a-z0-9_]+)\s*\(([^)]*)\)", src):
+        args = m.group(2).strip().rstrip(",")
+        fns[m.group(1)] = 0 if not args else args.count(",") + 1
+    assert fns == _header_prototypes()
+    consts = {m.group(1): int(m.group(2))
+              for m in re.finditer(r"pub const\s+(BHT_[A-Z0-9_]+)\s*:\s*\w+\s*=\s*(\d+)", src)}
+    assert consts == _header_constants()
+    # bht_params: the same fields, types and order
+    ctypes_of = {"uint32_t": "u32", "uint64_t": "u64", "float": "f32"}
+    hdr = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct bht_params \{(.*?)\} bht_params;", hdr, re.S).group(1)
+    c_fields = [(n, ctypes_of[t]) for t, n in re.findall(r"(\w+)\s+(\w+);", body)]
+    rs_body = re.search(r"pub struct bht_params \{(.*?)\}", src, re.S).group(1)
+    rs_fields = re.findall(r"pub (\w+):\s*(\w+),", rs_body)
+    assert rs_fields == c_fields
+
+
 def test_params_default_is_the_reference_builder_default(lib):
     from bhtsne_amd import _lib
     L = _lib.load()
