@@ -224,6 +224,16 @@ class TSNE:
         if metric not in (None, "euclidean"):
             raise NotImplementedError("the MI355X engine implements the Euclidean metric only")
 
+    def exact(self, metric="euclidean"):
+        """The exact O(n^2) fit (src/lib.rs:489-646) with the Euclidean distance or, as the
+        reference's exact tests use, the squared Euclidean distance ("sqeuclidean")."""
+        m = {"euclidean": _lib.METRIC_EUCLIDEAN, "sqeuclidean": _lib.METRIC_SQEUCLIDEAN}.get(metric)
+        if m is None:
+            raise NotImplementedError("the exact path takes metric 'euclidean' or 'sqeuclidean'")
+        check(self._L.bht_job_exact(self._h, m, self._cb(), None))
+        self._fitted = True
+        return self
+
     def barnes_hut(self, theta: float, metric="euclidean"):
         self._euclid(metric)
         check(self._L.bht_job_barnes_hut(self._h, float(theta), self._cb(), None))

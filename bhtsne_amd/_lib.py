@@ -22,6 +22,8 @@ METHOD_BARNES_HUT = 0
 METHOD_FITSNE = 1
 KNN_AUTO = 0
 KNN_EXACT = 1
+METRIC_EUCLIDEAN = 0
+METRIC_SQEUCLIDEAN = 1
 BANDWIDTH_FP64 = 0
 BANDWIDTH_F32_SEQ = 1
 
@@ -65,6 +67,7 @@ SIGNATURES = {
                                                       C.c_void_p, C.c_void_p, C.c_uint64,
                                                       EPOCH_CB, C.c_void_p]),
     "bht_job_fit_sne": (C.c_int32, [C.c_void_p, EPOCH_CB, C.c_void_p]),
+    "bht_job_exact": (C.c_int32, [C.c_void_p, C.c_uint32, EPOCH_CB, C.c_void_p]),
     "bht_job_fit_sne_with_neighbors": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p,
                                                    C.c_void_p, C.c_uint64, EPOCH_CB,
                                                    C.c_void_p]),

@@ -132,6 +132,108 @@ void search_beta(const float *dist, float *p, int64_t n, int k, float perplexity
     BHT_LAUNCH_CHECK();
 }
 
+// The same search over dense rows (the exact path, src/lib.rs:530-542): row i holds the distances
+// to all n points, its own (0) included, as the reference's dense rows do.  One wave per row; the
+// row is read from global memory in every iteration, and in mode F32_SEQ the row's p values are
+// staged in p_out for lane 0's left-to-right sums.
+template <int MODE>
+__global__ void __launch_bounds__(AF_THREADS) k_search_beta_dense(const float *dist, float *p_out,
+                                                                  int64_t n, float perplexity) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t row = (int64_t)blockIdx.x * AF_ROWS + w;
+    if (row >= n) return;
+    const float *d = dist + row * n;
+    float *p = p_out + row * n;
+    bool found = false;
+    float beta = 1.0f, min_beta = -FLT_MAX, max_beta = FLT_MAX;
+    const float tolerance = 1e-5f;
+    const float log_perp = log_f32_rn(perplexity);
+    float sum_f = 0.0f;
+    double sum_d = 0.0;
+    int iteration = 0;
+    float beta_used = beta;  // the beta of the last p values computed
+    while (!found && iteration < 200) {
+        float diff;
+        beta_used = beta;
+        if (MODE == 0) {
+            double s = 0.0, e = 0.0;
+            for (int64_t j = lane; j < n; j += 64) {
+                const float dj = d[j];
+                const float pj = exp_f32_rn(-beta * (dj * dj));
+                s += (double)pj;
+                e += (double)(beta * pj) * (double)(dj * dj);
+            }
+            s = wave_sum_f64(s) + (double)FLT_MIN;
+            e = wave_sum_f64(e);
+            sum_d = s;
+            diff = (float)(e / s + log(s) - (double)log_perp);
+        } else {
+            for (int64_t j = lane; j < n; j += 64) {
+                const float dj = d[j];
+                p[j] = exp_f32_rn(-beta * (dj * dj));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            float df = 0.0f, sf = 0.0f;
+            if (lane == 0) {
+                float s = 0.0f;
+                for (int64_t j = 0; j < n; j++) s = s + p[j];
+                s = s + FLT_MIN;
+                float entropy = 0.0f;
+                for (int64_t j = 0; j < n; j++) entropy = entropy + beta * p[j] * (d[j] * d[j]);
+                entropy = entropy / s + log_f32_rn(s);
+                df = entropy - log_perp;
+                sf = s;
+            }
+            diff = __shfl(df, 0, 64);
+            sum_f = __shfl(sf, 0, 64);
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (diff < tolerance && -diff < tolerance) {
+            found = true;
+        } else {
+            if (diff > 0.0f) {
+                min_beta = beta;
+                if (max_beta == FLT_MAX || max_beta == -FLT_MAX) beta *= 2.0f;
+                else beta = (beta + max_beta) / 2.0f;
+            } else {
+                max_beta = beta;
+                if (min_beta == -FLT_MAX || min_beta == FLT_MAX) beta /= 2.0f;
+                else beta = (beta + min_beta) / 2.0f;
+            }
+            if (isinf(beta) && beta > 0.0f) beta = FLT_MAX;
+            if (isinf(beta) && beta < 0.0f) beta = -FLT_MAX;
+        }
+        iteration += 1;
+    }
+    // the last iteration's p (recomputed for the FP64 mode, which does not keep them)
+    for (int64_t j = lane; j < n; j += 64) {
+        const float dj = d[j];
+        float v;
+        if (MODE == 0) {
+            const float pj = exp_f32_rn(-beta_used * (dj * dj));
+            v = (float)((double)pj / (sum_d + (double)FLT_EPSILON));
+        } else {
+            v = p[j] / (sum_f + FLT_EPSILON);
+        }
+        p[j] = v;
+    }
+}
+
+void search_beta_dense(const float *dist, float *p, int64_t n, float perplexity, int mode,
+                       hipStream_t s) {
+    unsigned nb = ceil_div((uint64_t)n, AF_ROWS);
+    if (nb == 0) return;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_search_beta_dense<0>, dim3(nb), dim3(AF_THREADS), 0, s, dist, p, n,
+                           perplexity);
+    else
+        hipLaunchKernelGGL(k_search_beta_dense<1>, dim3(nb), dim3(AF_THREADS), 0, s, dist, p, n,
+                           perplexity);
+    BHT_LAUNCH_CHECK();
+}
+
 // ---- symmetrisation --------------------------------------------------------------------------
 
 // 1. sort every row's (column, p) pairs by column: bitonic sort of 256 slots by one wave in LDS.

@@ -27,6 +27,9 @@ struct SymWork {
 
 void search_beta(const float *dist, float *p, int64_t n, int k, float perplexity, int mode,
                  hipStream_t s);
+// The same search over n dense rows of n distances (each row's own 0 included).
+void search_beta_dense(const float *dist, float *p, int64_t n, float perplexity, int mode,
+                       hipStream_t s);
 void sym_sort_rows(uint32_t *cols, float *vals, int64_t n, int k, hipStream_t s);
 uint64_t symmetrize_device(uint32_t *cols, float *vals, int64_t n, int k, SymWork &wk,
                            hipStream_t s);

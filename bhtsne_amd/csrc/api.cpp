@@ -234,7 +234,7 @@ const char *MSG_THETA =
 
 // ---- job -------------------------------------------------------------------------------------
 
-enum FitKind { FIT_NONE = -1, FIT_BH = 0, FIT_FITSNE = 1 };
+enum FitKind { FIT_NONE = -1, FIT_BH = 0, FIT_FITSNE = 1, FIT_EXACT = 2 };
 
 struct bht_job {
     bht_ctx *ctx;
@@ -281,6 +281,8 @@ struct bht_job {
     // the device holds per-block bounds of the current embedding buffer (from the last update or
     // centring), so that the next FIt-SNE epoch can leave the recentring to its readers
     bool lazy_ready = false;
+    // the exact fit: dense P (n x n) and per-row sums (Z, KL)
+    DevBuf pdense, zrow;
     hipEvent_t ahead[4] = {};
     bool ahead_rec[4] = {};
     // Barnes-Hut workspace
@@ -476,7 +478,7 @@ void fill_jobdev(bht_job *j) {
     JobDev &d = j->hd;
     d.n = (int)j->n;
     d.dim = (int)j->dim();
-    d.method = j->method == FIT_BH ? M_BH : M_FITSNE;
+    d.method = j->method == FIT_BH ? M_BH : j->method == FIT_EXACT ? M_EXACT : M_FITSNE;
     d.theta_sq = j->theta * j->theta;
     d.ybuf[0] = j->y[0].as<float>();
     d.ybuf[1] = j->y[1].as<float>();
@@ -885,11 +887,37 @@ void build_affinities_from_data(bht_job *j) {
 }
 
 // finalize_p_and_seed / prepare_cached (lib.rs:663-685, 1227-1236) + buffer prep.
+// Random or given seed, zero velocity, unit gains, cleared scalars (src/lib.rs:671-684,
+// tsne/mod.rs:96-105).
+void seed_state(bht_job *j) {
+    const uint64_t len = j->n * j->dim();
+    hipStream_t s = j->stream;
+    std::vector<float> host(len);
+    if (j->have_init) {
+        host = j->init;
+        j->init.clear();
+        j->have_init = false;
+    } else {
+        // random_init (mod.rs:142-147) draws from an unseeded N(0, 1e-4); here the stream is
+        // seeded per job so that runs repeat.
+        std::mt19937_64 rng(0x6268747366ull + 7919ull * (j->seed_counter++));
+        std::normal_distribution<double> nd(0.0, 1e-4);
+        for (auto &v : host) v = (float)nd(rng);
+    }
+    j->cur = 0;
+    upload_points(j, host.data(), j->ycur());
+    BHT_HIP(hipMemsetAsync(j->uy.p, 0, len * 4, s));
+    std::vector<float> ones(len, 1.0f);
+    BHT_HIP(hipMemcpyAsync(j->gains.p, ones.data(), len * 4, hipMemcpyHostToDevice, s));
+    BHT_HIP(hipMemsetAsync(j->scal.p, 0, sizeof(EpochScalars), s));
+    l2_sync(s);  // after the seed's host copies
+    BHT_HIP(hipStreamSynchronize(s));
+}
+
 void prepare_run(bht_job *j, int method, float theta, bool cached) {
     if (cached) j->stop_lying_fired = false;
     if (method == FIT_FITSNE) ensure_fitsne_workspace(j);
     else ensure_bh_workspace(j);
-    const uint64_t len = j->n * j->dim();
     hipStream_t s = j->stream;
     // P and the data may have been written by host copies or on another stream
     l2_sync(s);
@@ -908,27 +936,7 @@ void prepare_run(bht_job *j, int method, float theta, bool cached) {
         scale_values(j->vals.as<float>(), j->nnz, 1.0f / exag, s);
         scale_values(j->qv.as<float>(), std::min<uint64_t>(j->n, j->nnz), 1.0f / exag, s);
     }
-    // seed
-    std::vector<float> host(len);
-    if (j->have_init) {
-        host = j->init;
-        j->init.clear();
-        j->have_init = false;
-    } else {
-        // random_init (mod.rs:142-147) draws from an unseeded N(0, 1e-4); here the stream is
-        // seeded per job so that runs repeat.
-        std::mt19937_64 rng(0x6268747366ull + 7919ull * (j->seed_counter++));
-        std::normal_distribution<double> nd(0.0, 1e-4);
-        for (auto &v : host) v = (float)nd(rng);
-    }
-    j->cur = 0;
-    upload_points(j, host.data(), j->ycur());
-    BHT_HIP(hipMemsetAsync(j->uy.p, 0, len * 4, s));
-    std::vector<float> ones(len, 1.0f);
-    BHT_HIP(hipMemcpyAsync(j->gains.p, ones.data(), len * 4, hipMemcpyHostToDevice, s));
-    BHT_HIP(hipMemsetAsync(sc, 0, sizeof(EpochScalars), s));
-    l2_sync(s);  // after the seed's host copies
-    BHT_HIP(hipStreamSynchronize(s));
+    seed_state(j);
     j->method = method;
     j->theta = theta;
     j->epoch = 0;
@@ -1223,7 +1231,13 @@ void finish_run(bht_job *j) {
     // (exports, KL) are rescaled here once the schedule has stopped lying.  If the run ended
     // right after stop_lying_epoch, no epoch has rescaled the tiled copy yet either.
     const uint64_t sl = j->params.stop_lying_epoch;
-    if (sl != 0 && j->epoch >= sl + 1) {
+    if (j->method == FIT_EXACT) {
+        // the exact epochs read P divided by the exaggeration once stop-lying fired; the stored P
+        // (KL) is divided here
+        if (sl != 0 && j->epoch >= sl + 1)
+            scale_values(j->pdense.as<float>(), j->n * j->n,
+                         1.0f / j->params.early_exaggeration, j->stream);
+    } else if (sl != 0 && j->epoch >= sl + 1) {
         const float inv = 1.0f / j->params.early_exaggeration;
         scale_values(j->vals.as<float>(), j->nnz, inv, j->stream);
         scale_values(j->qv.as<float>(), std::min<uint64_t>(j->n, j->nnz), inv, j->stream);
@@ -1257,6 +1271,81 @@ void validate_init(const bht_job *j) {
                                      std::to_string(j->init.size()) +
                                      " values, expected n_samples * D = " +
                                      std::to_string(j->n * j->dim()));
+}
+
+// ---- the exact fit (src/lib.rs:489-646) -----------------------------------------------------
+
+constexpr uint64_t EXACT_MAX_POINTS = 65536;
+
+// Dense P from the samples: distances, a bandwidth search per dense row, symmetrisation, and the
+// normalisation with the exaggeration folded in; then the seed.
+void prepare_exact(bht_job *j, int metric) {
+    const uint64_t n = j->n;
+    if (n > EXACT_MAX_POINTS)
+        throw Error(ERR_INVALID, "the exact path supports at most " +
+                                     std::to_string(EXACT_MAX_POINTS) + " samples");
+    if (!j->have_data) throw Error(ERR_STATE, "no sample vectors: call bht_job_set_data_f32");
+    ensure_state_buffers(j);
+    bht_ctx *c = j->ctx;
+    hipStream_t s = j->stream;
+    const uint64_t nn = std::max<uint64_t>(n * n, 1);
+    j->pdense.ensure(c, nn * sizeof(float));
+    j->zrow.ensure(c, 2 * std::max<uint64_t>(n, 1) * sizeof(double));
+    {
+        DevBuf dist(c);
+        dist.ensure(c, nn * sizeof(float));
+        l2_sync(s);  // the samples may come from host copies
+        exact_distances(j->x.as<float>(), (int64_t)n, (int)j->d_in, metric, dist.as<float>(), s);
+        search_beta_dense(dist.as<float>(), j->pdense.as<float>(), (int64_t)n,
+                          j->params.perplexity, (int)j->params.bandwidth_mode, s);
+        BHT_HIP(hipStreamSynchronize(s));  // before the distances are released
+    }
+    exact_symmetrize(j->pdense.as<float>(), (int64_t)n, s);
+    // normalise with the exaggeration folded in (tsne/mod.rs:284-293)
+    EpochScalars *sc = j->scal.as<EpochScalars>();
+    BHT_HIP(hipMemsetAsync(sc, 0, sizeof(EpochScalars), s));
+    sum_values_f64(j->pdense.as<float>(), n * n, sc, j->dpart.as<double>(), s);
+    double psum = 0.0;
+    BHT_HIP(hipMemcpyAsync(&psum, &sc->psum, sizeof(double), hipMemcpyDeviceToHost, s));
+    BHT_HIP(hipStreamSynchronize(s));
+    const float exag = j->params.early_exaggeration;
+    scale_values(j->pdense.as<float>(), n * n, exag / ((float)psum + FLT_EPSILON), s);
+    if (j->params.stop_lying_epoch == 0)
+        scale_values(j->pdense.as<float>(), n * n, 1.0f / exag, s);
+    // the update takes the attraction sums as the whole gradient: the repulsion stays zero
+    BHT_HIP(hipMemsetAsync(j->frep.p, 0, n * j->dim() * sizeof(float), s));
+    seed_state(j);
+    j->method = FIT_EXACT;
+    j->theta = 0.0f;
+    j->epoch = 0;
+    j->lazy_ready = false;
+    j->run_momentum = j->params.momentum;
+    j->prepared = true;
+    // an exact fit keeps no sparse affinities (the reference's affinities() is None after it)
+    j->have_p = false;
+    sync_jobdev(j, false);
+}
+
+void run_exact_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
+    const uint64_t len = j->n * j->dim();
+    std::vector<float> snap(cb ? len : 0);
+    const LaunchSet L = single_set(j);
+    for (uint64_t e = 0; e < count; e++) {
+        // zero_mean of the previous epoch's update (src/lib.rs:631), then the exact gradient
+        center_points(L, j->stream);
+        exact_gradient(L.jobs, j->pdense.as<float>(), (int64_t)j->n, (int)j->dim(),
+                       j->zrow.as<double>(), j->stream);
+        update_points(L, j->stream);
+        j->cur = 1 - j->cur;
+        const uint64_t epoch = j->epoch++;
+        epoch_tail(j, epoch);
+        if (cb) {
+            recenter(j);
+            download_points(j, j->ycur(), snap.data());
+            cb(epoch, snap.data(), len, user);
+        }
+    }
+    j->last_choice = -1;
 }
 
 void fit(bht_job *j, int method, float theta, bht_epoch_cb cb, void *user, bool cached) {
@@ -1799,6 +1888,21 @@ bht_status bht_job_set_initial_embedding(bht_job *job, const float *y, uint64_t 
     return BHT_OK;
 }
 
+bht_status bht_job_exact(bht_job *job, uint32_t metric, bht_epoch_cb cb, void *user) {
+    if (!job) return fail(BHT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (metric != BHT_METRIC_EUCLIDEAN && metric != BHT_METRIC_SQEUCLIDEAN)
+            throw Error(ERR_INVALID, "unknown metric");
+        if (job->dim() != 1 && job->dim() != 2)
+            throw Error(ERR_INVALID, "the exact path supports d_out in {1, 2}");
+        check_perplexity(job);
+        validate_init(job);
+        prepare_exact(job, (int)metric);
+        run_exact_epochs(job, job->params.epochs, cb, user);
+        finish_run(job);
+    });
+}
+
 bht_status bht_job_fit_sne(bht_job *job, bht_epoch_cb cb, void *user) {
     if (!job) return fail(BHT_ERR_INVALID, "null argument");
     return guarded([&] {
@@ -2075,6 +2179,21 @@ bht_status bht_job_kl_divergence(bht_job *job, double *reference_kl, double *tru
     if (!job) return fail(BHT_ERR_INVALID, "null argument");
     return guarded([&] {
         if (job->fit_kind == FIT_NONE) throw Error(ERR_STATE, "no fit yet");  // -> None
+        if (job->fit_kind == FIT_EXACT) {  // evaluate_error (tsne/mod.rs:579-618)
+            const int saved = job->method;
+            job->method = FIT_EXACT;
+            const LaunchSet L = single_set(job);
+            exact_kl(L.jobs, job->pdense.as<float>(), (int64_t)job->n, (int)job->dim(),
+                     job->zrow.as<double>(), job->stream);
+            EpochScalars h;
+            BHT_HIP(hipMemcpyAsync(&h, job->scal.p, sizeof(h), hipMemcpyDeviceToHost,
+                                   job->stream));
+            BHT_HIP(hipStreamSynchronize(job->stream));
+            job->method = saved;
+            if (reference_kl) *reference_kl = h.kl_ref;
+            if (true_kl) *true_kl = h.kl_true;
+            return;
+        }
         // the reference indexes p_values by the sample index (mod.rs:768-770)
         if (job->nnz < job->n)
             throw Error(ERR_INVALID, "kl_divergence needs at least one affinity per sample");

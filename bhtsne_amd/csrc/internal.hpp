@@ -86,7 +86,7 @@ enum Family {
     NFAM
 };
 
-enum Method { M_BH = 0, M_FITSNE = 1 };
+enum Method { M_BH = 0, M_FITSNE = 1, M_EXACT = 2 };
 
 struct JobDev {
     int n, dim, method;
@@ -257,6 +257,16 @@ void center_plan(const LaunchSet &L, bool do_plan, hipStream_t s);
 // with the grid plan (which only the repulsion reads).
 void center_points(const LaunchSet &L, hipStream_t s);
 void plan_grid(const LaunchSet &L, bool do_plan, hipStream_t s);
+// The exact fit (exactfit.hip): dense input distances (metric 0: Euclidean, 1: squared
+// Euclidean), dense symmetrisation, and per epoch Z (zrow: n doubles) and the gradient of job 0
+// of `jobs` into its attraction sums; the KL of the current embedding (zrow: 2n doubles).
+void exact_distances(const float *x, int64_t n, int d_in, int metric, float *dist, hipStream_t s);
+void exact_symmetrize(float *p, int64_t n, hipStream_t s);
+void exact_gradient(const JobDev *jobs, const float *P, int64_t n, int dim, double *zrow,
+                    hipStream_t s);
+void exact_kl(const JobDev *jobs, const float *P, int64_t n, int dim, double *zrow,
+              hipStream_t s);
+
 // Head of a lazily centred FIt-SNE epoch (the recentring shift left to the readers) and its grid
 // plan, from the last update's sums and bounds.
 void epoch_head(const LaunchSet &L, hipStream_t s);

@@ -126,6 +126,13 @@ bht_status bht_job_barnes_hut_with_neighbors(bht_job *job, float theta, const ui
                                              const float *nbr_distance, const uint64_t *row_lens,
                                              uint64_t n_rows, bht_epoch_cb cb, void *user);
 bht_status bht_job_fit_sne(bht_job *job, bht_epoch_cb cb, void *user);
+/* The exact O(n^2) fit (tSNE::exact, src/lib.rs:489-646) with the distance `metric` between the
+ * sample vectors: dense P over all pairs (each row's own entry included, as the reference's dense
+ * rows do), the exact gradient with Z summed over all n^2 ordered pairs, n <= 65536.  The
+ * reference's exact tests pass the squared Euclidean distance.  affinities() is None after it. */
+#define BHT_METRIC_EUCLIDEAN 0u
+#define BHT_METRIC_SQEUCLIDEAN 1u
+bht_status bht_job_exact(bht_job *job, uint32_t metric, bht_epoch_cb cb, void *user);
 bht_status bht_job_fit_sne_with_neighbors(bht_job *job, const uint64_t *nbr_index,
                                           const float *nbr_distance, const uint64_t *row_lens,
                                           uint64_t n_rows, bht_epoch_cb cb, void *user);

@@ -87,6 +87,13 @@ def lib():
                                                          C.c_int64, f64p]),
         "oracle_exact_attraction_f64": (None, [f32p, C.c_int64, C.c_int, u64p, u32p, f32p,
                                                f64p]),
+        "oracle_exact_distances": (None, [f32p, C.c_int64, C.c_int, C.c_int, f32p]),
+        "oracle_exact_affinities": (None, [f32p, C.c_int64, C.c_int, C.c_int, C.c_float,
+                                           C.c_float, C.c_int, f32p]),
+        "oracle_exact_gradient": (C.c_float, [f32p, C.c_int64, C.c_int, f32p, f32p]),
+        "oracle_exact_epochs": (None, [f32p, f32p, f32p, C.c_int64, C.c_int, f32p, f32p, i64p,
+                                       C.c_int64, C.c_int64, C.POINTER(C.c_float)]),
+        "oracle_exact_kl": (C.c_float, [f32p, C.c_int64, C.c_int, f32p, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -386,3 +393,55 @@ def build_affinities(x, perplexity):
     idx, dist = knn(x, k)
     p = search_beta(dist, perplexity)
     return symmetrize(idx, p)
+
+
+# --- the exact fit (src/lib.rs:489-646), with the reference's own semantics ----------------------
+
+def exact_distances(x, metric="euclidean"):
+    x = _f32(x)
+    n, d = x.shape
+    out = np.zeros((n, n), np.float32)
+    lib().oracle_exact_distances(x, n, d, 0 if metric == "euclidean" else 1, out)
+    return out
+
+
+def exact_affinities(x, perplexity, metric="euclidean", exag=12.0, stop_lying_epoch=250):
+    """Dense P as the exact fit builds it: normalised and exaggerated (divided back at once when
+    stop_lying_epoch is 0)."""
+    x = _f32(x)
+    n, d = x.shape
+    P = np.zeros((n, n), np.float32)
+    lib().oracle_exact_affinities(x, n, d, 0 if metric == "euclidean" else 1, float(perplexity),
+                                  float(exag), 1 if stop_lying_epoch == 0 else 0, P)
+    return P
+
+
+def exact_gradient(y, P):
+    """(dy, 1/Z) of the exact path for one state."""
+    y = _f32(y)
+    n, D = y.shape
+    dy = np.zeros_like(y)
+    inv = lib().oracle_exact_gradient(y, n, D, _f32(P), dy)
+    return dy, inv
+
+
+def exact_epochs(y, uy, gains, P, *, learning_rate, momentum=0.5, final_momentum=0.8,
+                 early_exaggeration=12.0, momentum_switch_epoch=250, stop_lying_epoch=250, e0=0,
+                 e1=1):
+    """Epochs [e0, e1) of the exact loop, in place (P is rescaled at stop-lying)."""
+    n, D = y.shape
+    sched = np.array([learning_rate, momentum, final_momentum, early_exaggeration], np.float32)
+    isched = np.array([momentum_switch_epoch, stop_lying_epoch], np.int64)
+    mom = C.c_float(momentum)
+    lib().oracle_exact_epochs(y, uy, gains, n, D, P, sched, isched, int(e0), int(e1),
+                              C.byref(mom))
+    return mom.value
+
+
+def exact_kl(y, P):
+    """(reference formula in f32, the same sum in double) of evaluate_error (mod.rs:579-618)."""
+    y = _f32(y)
+    n, D = y.shape
+    t = C.c_double(0.0)
+    r = lib().oracle_exact_kl(y, n, D, _f32(P), C.byref(t))
+    return float(r), t.value

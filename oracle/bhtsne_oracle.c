@@ -1551,3 +1551,129 @@ EXPORT void oracle_exact_attraction_f64(const float *y, int64_t n, int D, const 
         for (int a = 0; a < D; a++) pos[i * D + a] = f[a];
     }
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* The exact fit (src/lib.rs:489-646), restated with the reference's own semantics: dense rows  */
+/* include each point's own entry (d_ii = 0), the metric value is squared again inside          */
+/* search_beta, and Z sums q over all n^2 ordered pairs, diagonal included.  (The FP64          */
+/* yardstick above, oracle_exact_repulsion_f64, is the quirk-free O(n^2) repulsion.)            */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Dense distances (compute_pairwise_distance_matrix, mod.rs:161-189): the upper triangle from
+ * f(x_i, x_j), mirrored; the diagonal 0.  metric 0: sqrt of the left-to-right sum of squared
+ * differences; 1: the sum itself. */
+EXPORT void oracle_exact_distances(const float *x, int64_t n, int d, int metric, float *dist) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; i++) {
+        dist[(size_t)i * n + i] = 0.0f;
+        for (int64_t j = i + 1; j < n; j++) {
+            float s = 0.0f;
+            for (int t = 0; t < d; t++) {
+                float diff = x[(size_t)i * d + t] - x[(size_t)j * d + t];
+                s = s + diff * diff;
+            }
+            dist[(size_t)i * n + j] = metric == 0 ? sqrtf(s) : s;
+        }
+    }
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = i + 1; j < n; j++) dist[(size_t)j * n + i] = dist[(size_t)i * n + j];
+}
+
+/* Dense P (lib.rs:529-561): search_beta on every full row, P_ij = P_ji = p_ij + p_ji for i < j,
+ * then normalize_p_values with the exaggeration (and stop_lying at once when stop_lying_epoch is
+ * 0, finalize_p_and_seed lib.rs:663-669). */
+EXPORT void oracle_exact_affinities(const float *x, int64_t n, int d, int metric,
+                                    float perplexity, float exag, int stop_lying_now, float *P) {
+    float *dist = (float *)malloc(sizeof(float) * (size_t)(n * n));
+    oracle_exact_distances(x, n, d, metric, dist);
+    oracle_search_beta(P, dist, n, (int)n, perplexity);
+    free(dist);
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = i + 1; j < n; j++) {
+            float s = P[(size_t)i * n + j] + P[(size_t)j * n + i];
+            P[(size_t)i * n + j] = s;
+            P[(size_t)j * n + i] = s;
+        }
+    oracle_normalize_p(P, n * n, exag);
+    if (stop_lying_now) oracle_stop_lying(P, n * n, exag);
+}
+
+/* Exact gradient of one epoch (lib.rs:571-623): d_ij = sum of powi(2) differences, q = 1/(1+d),
+ * Z = sum of all n^2 q (pairwise f32 sum, as rayon's), inverse 1/Z in f32, and per row
+ * dy_i += (y_i - y_j) * ((p - q inv) * q) left to right over j.  Returns 1/Z. */
+EXPORT float oracle_exact_gradient(const float *y, int64_t n, int D, const float *P, float *dy) {
+    float *q = (float *)malloc(sizeof(float) * (size_t)(n * n));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) {
+            float s = 0.0f;
+            for (int a = 0; a < D; a++) {
+                float diff = y[(size_t)i * D + a] - y[(size_t)j * D + a];
+                s = s + diff * diff;
+            }
+            q[(size_t)i * n + j] = 1.0f / (1.0f + s);
+        }
+    const float inv = 1.0f / pairwise_sum_f32(q, n * n, 1);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        float acc[8] = {0};
+        for (int64_t j = 0; j < n; j++) {
+            const float qq = q[(size_t)i * n + j];
+            const float m = (P[(size_t)i * n + j] - qq * inv) * qq;
+            for (int a = 0; a < D; a++)
+                acc[a] += (y[(size_t)i * D + a] - y[(size_t)j * D + a]) * m;
+        }
+        for (int a = 0; a < D; a++) dy[(size_t)i * D + a] = acc[a];
+    }
+    free(q);
+    return inv;
+}
+
+/* Epochs [e0, e1) of the exact loop: gradient, update_solution (the same step as the approximate
+ * paths' with a zero repulsion), zero_mean, the schedule (lib.rs:1162-1179).  sched / isched /
+ * momentum_io as in oracle_run_epochs. */
+EXPORT void oracle_exact_epochs(float *y, float *uy, float *gains, int64_t n, int D, float *P,
+                                const float *sched, const int64_t *isched, int64_t e0, int64_t e1,
+                                float *momentum_io) {
+    int64_t len = n * D;
+    float *dy = (float *)malloc(sizeof(float) * (size_t)len);
+    float *zero = (float *)calloc((size_t)len, sizeof(float));
+    float lr = sched[0], final_m = sched[2], exag = sched[3];
+    float momentum = *momentum_io;
+    for (int64_t e = e0; e < e1; e++) {
+        oracle_exact_gradient(y, n, D, P, dy);
+        oracle_gradient_step(y, dy, zero, uy, gains, len, lr, momentum, 0.0f);
+        oracle_zero_mean(y, n, D, NULL);
+        if (e == isched[1] && e != 0) oracle_stop_lying(P, n * n, exag);
+        if (e == isched[0]) momentum = final_m;
+    }
+    *momentum_io = momentum;
+    free(dy);
+    free(zero);
+}
+
+/* evaluate_error (mod.rs:579-618): sum over all n^2 pairs of p ln((p + min) / (q inv + min)), in
+ * f32 (returned) and in double (true_out). */
+EXPORT float oracle_exact_kl(const float *y, int64_t n, int D, const float *P, double *true_out) {
+    float *q = (float *)malloc(sizeof(float) * (size_t)(n * n));
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) {
+            float s = 0.0f;
+            for (int a = 0; a < D; a++) {
+                float diff = y[(size_t)i * D + a] - y[(size_t)j * D + a];
+                s = s + diff * diff;
+            }
+            q[(size_t)i * n + j] = 1.0f / (1.0f + s);
+        }
+    const float inv = 1.0f / pairwise_sum_f32(q, n * n, 1);
+    float kl = 0.0f;
+    double t = 0.0;
+    for (int64_t e = 0; e < n * n; e++) {
+        const float p = P[e], qn = q[e] * inv;
+        kl = kl + p * logf((p + FLT_MIN) / (qn + FLT_MIN));
+        t += (double)p * log(((double)p + (double)FLT_MIN) / ((double)qn + (double)FLT_MIN));
+    }
+    free(q);
+    if (true_out) *true_out = t;
+    return kl;
+}

@@ -27,6 +27,8 @@ pub const BHT_METHOD_BARNES_HUT: u32 = 0;
 pub const BHT_METHOD_FITSNE: u32 = 1;
 pub const BHT_KNN_AUTO: u32 = 0;
 pub const BHT_KNN_EXACT: u32 = 1;
+pub const BHT_METRIC_EUCLIDEAN: u32 = 0;
+pub const BHT_METRIC_SQEUCLIDEAN: u32 = 1;
 
 #[repr(C)]
 pub struct bht_ctx {
@@ -94,6 +96,8 @@ extern "C" {
                                              row_lens: *const u64, n_rows: u64, cb: bht_epoch_cb,
                                              user: *mut c_void) -> bht_status;
     pub fn bht_job_fit_sne(job: *mut bht_job, cb: bht_epoch_cb, user: *mut c_void) -> bht_status;
+    pub fn bht_job_exact(job: *mut bht_job, metric: u32, cb: bht_epoch_cb, user: *mut c_void)
+        -> bht_status;
     pub fn bht_job_fit_sne_with_neighbors(job: *mut bht_job, nbr_index: *const u64,
                                           nbr_distance: *const f32, row_lens: *const u64,
                                           n_rows: u64, cb: bht_epoch_cb, user: *mut c_void)

@@ -223,6 +223,24 @@ impl<'data, const D: usize> TSne<'data, D> {
         self
     }
 
+    /// `exact(euclidean)`: the O(n^2) fit with the Euclidean distance.
+    pub fn exact_euclidean(&mut self) -> &mut Self {
+        self.exact_with(sys::BHT_METRIC_EUCLIDEAN)
+    }
+
+    /// `exact(squared euclidean)`, the distance the reference's exact tests pass.
+    pub fn exact_squared_euclidean(&mut self) -> &mut Self {
+        self.exact_with(sys::BHT_METRIC_SQEUCLIDEAN)
+    }
+
+    fn exact_with(&mut self, metric: u32) -> &mut Self {
+        self.push_params();
+        let (cb, user) = self.cb_parts();
+        check(unsafe { sys::bht_job_exact(self.raw, metric, cb, user) });
+        self.fitted = true;
+        self
+    }
+
     /// `fit_sne(euclidean)`.
     pub fn fit_sne_euclidean(&mut self) -> &mut Self {
         self.push_params();
