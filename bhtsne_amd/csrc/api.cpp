@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -263,7 +264,8 @@ struct bht_job {
     int cur = 0;
     bool have_init = false;
     std::vector<float> init;
-    uint64_t seed_counter = 0;
+    uint64_t seed_counter = 0;  // random initialisations drawn by this job
+    uint64_t serial = 0;        // creation index of the job in the process
 
     // epoch workspace
     DevBuf keys, svals, keys_alt, svals_alt, grid, spec, rker, kspec, part, dpart, mpart, fpart,
@@ -898,9 +900,11 @@ void seed_state(bht_job *j) {
         j->init.clear();
         j->have_init = false;
     } else {
-        // random_init (mod.rs:142-147) draws from an unseeded N(0, 1e-4); here the stream is
-        // seeded per job so that runs repeat.
-        std::mt19937_64 rng(0x6268747366ull + 7919ull * (j->seed_counter++));
+        // random_init (mod.rs:142-147) draws from an unseeded N(0, 1e-4), so separate runs start
+        // apart; here each draw is seeded by the job's creation index in the process and its
+        // draw count, so that separate jobs start apart too and a program repeats its runs.
+        std::mt19937_64 rng(0x6268747366ull + 0x9e3779b97f4a7c15ull * j->serial +
+                            7919ull * (j->seed_counter++));
         std::normal_distribution<double> nd(0.0, 1e-4);
         for (auto &v : host) v = (float)nd(rng);
     }
@@ -1842,7 +1846,9 @@ bht_status bht_job_create(bht_ctx *ctx, uint64_t n_samples, const bht_params *pa
         if (params) p = *params;
         else bht_params_default(&p);
         if (n_samples >= (1ull << 31)) throw Error(ERR_INVALID, "n_samples must be below 2^31");
+        static std::atomic<uint64_t> next_serial{0};
         *out = new bht_job(ctx, n_samples, p);
+        (*out)->serial = next_serial++;
         ctx_acquire(ctx);
     });
 }
