@@ -310,7 +310,8 @@ def main():
     form = t.epoch_form()
     grid = t.grid_info() if method == "fitsne" else None
     if args.short:
-        log(f"{steps} epochs in {elapsed:.4f}s -> {eps:.1f} epochs/s; grid {grid}")
+        log(f"{steps} epochs in {elapsed:.4f}s -> {eps:.1f} epochs/s; grid {grid}; "
+            f"epoch form {form}")
         print(json.dumps({"metric": METRIC, "value": round(eps, 3), "unit": "epochs/s",
                           "steps": steps, "short": True}), flush=True)
         return

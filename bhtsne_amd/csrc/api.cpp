@@ -1066,7 +1066,10 @@ uint64_t tune_key(const bht_job *j) {
     return ((uint64_t)j->method << 40) | ((uint64_t)j->dim() << 32) | (2ull * lg + half);
 }
 
-constexpr int TUNE_WARM = 1, TUNE_TIMED = 2;
+// Per candidate and round: TUNE_WARM untimed epochs (then a synchronisation), one epoch that lets
+// the host enqueue ahead of the device, and TUNE_TIMED timed epochs; TUNE_ROUNDS rounds over the
+// candidates, each candidate's best round counts.
+constexpr int TUNE_WARM = 1, TUNE_LEAD = 1, TUNE_TIMED = 3, TUNE_ROUNDS = 2;
 
 // Epoch-carried device state: both embedding buffers, uy, gains, the scalars (epoch counter,
 // pending recentring) and the update's partial sums (the next recentring reads them).
@@ -1118,10 +1121,10 @@ void snapshot_restore(bht_job *j, StateSnapshot &sn) {
 TuneChoice autotune(bht_job *j, const LaunchSet &L, bool fit) {
     TuneChoice best;
     best.queue_blocks = attract_default_queue_blocks();
-    // every candidate runs epochs [epoch, epoch + TUNE_WARM + TUNE_TIMED) from the snapshot;
-    // none may be the epoch after stop-lying (it rescales the tiled P in place)
+    // every candidate runs epochs [epoch, epoch + TUNE_WARM + TUNE_LEAD + TUNE_TIMED) from the
+    // snapshot; none may be the epoch after stop-lying (it rescales the tiled P in place)
     const uint64_t sl = j->params.stop_lying_epoch;
-    const uint64_t trials_end = j->epoch + TUNE_WARM + TUNE_TIMED;
+    const uint64_t trials_end = j->epoch + TUNE_WARM + TUNE_LEAD + TUNE_TIMED;
     if (sl != 0 && j->epoch <= sl + 1 && sl + 1 < trials_end) return best;
     std::vector<TuneChoice> cands;
     const int nblk = L.nb[F_ATT];
@@ -1135,32 +1138,34 @@ TuneChoice autotune(bht_job *j, const LaunchSet &L, bool fit) {
     hipEvent_t t0, t1;
     BHT_HIP(hipEventCreate(&t0));
     BHT_HIP(hipEventCreate(&t1));
-    float best_ms = 0.0f;
+    std::vector<float> cand_ms(cands.size(), 0.0f);
     const bool was_timing = j->timing;
     j->timing = false;
     try {
-        for (size_t c = 0; c < cands.size(); c++) {
-            for (int e = 0; e < TUNE_WARM; e++) {
-                enqueue_epoch(j, L, fit, cands[c].fork != 0, cands[c].queue_blocks);
-                j->cur = 1 - j->cur;
+        for (int r = 0; r < TUNE_ROUNDS; r++) {
+            for (size_t c = 0; c < cands.size(); c++) {
+                auto epoch_of = [&]() {
+                    enqueue_epoch(j, L, fit, cands[c].fork != 0, cands[c].queue_blocks);
+                    j->cur = 1 - j->cur;
+                };
+                for (int e = 0; e < TUNE_WARM; e++) epoch_of();
+                // the later epochs size their FFT launches by the grid the warm epoch reported
+                BHT_HIP(hipStreamSynchronize(j->stream));
+                for (int e = 0; e < TUNE_LEAD; e++) epoch_of();
+                BHT_HIP(hipEventRecord(t0, j->stream));
+                for (int e = 0; e < TUNE_TIMED; e++) epoch_of();
+                BHT_HIP(hipEventRecord(t1, j->stream));
+                BHT_HIP(hipEventSynchronize(t1));
+                float ms = 0.0f;
+                BHT_HIP(hipEventElapsedTime(&ms, t0, t1));
+                if (r == 0 || ms < cand_ms[c]) cand_ms[c] = ms;
+                snapshot_restore(j, sn);
             }
-            // the timed epochs size their FFT launches by the grid the warm epoch reported
-            BHT_HIP(hipStreamSynchronize(j->stream));
-            BHT_HIP(hipEventRecord(t0, j->stream));
-            for (int e = 0; e < TUNE_TIMED; e++) {
-                enqueue_epoch(j, L, fit, cands[c].fork != 0, cands[c].queue_blocks);
-                j->cur = 1 - j->cur;
-            }
-            BHT_HIP(hipEventRecord(t1, j->stream));
-            BHT_HIP(hipEventSynchronize(t1));
-            float ms = 0.0f;
-            BHT_HIP(hipEventElapsedTime(&ms, t0, t1));
-            if (c == 0 || ms < best_ms) {
-                best_ms = ms;
-                best = cands[c];
-            }
-            snapshot_restore(j, sn);
         }
+        size_t pick = 0;
+        for (size_t c = 1; c < cands.size(); c++)
+            if (cand_ms[c] < cand_ms[pick]) pick = c;
+        best = cands[pick];
     } catch (...) {
         j->timing = was_timing;
         hipEventDestroy(t0);
