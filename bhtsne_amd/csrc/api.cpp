@@ -235,6 +235,12 @@ const char *MSG_THETA =
 
 // ---- job -------------------------------------------------------------------------------------
 
+// Events that order work between streams (or let the host wait for it) and carry no data: the
+// data a stream hands to another goes through the cross-XCD cache maintenance of l2_sync, and what
+// the host reads is written with system-scope stores.  They skip the system-scope fence (an L2
+// write-back and invalidate) that an event otherwise performs when it completes.
+constexpr unsigned HANDOVER_EVENT = hipEventDisableTiming | hipEventDisableSystemFence;
+
 enum FitKind { FIT_NONE = -1, FIT_BH = 0, FIT_FITSNE = 1, FIT_EXACT = 2 };
 
 struct bht_job {
@@ -328,9 +334,9 @@ struct bht_job {
         BHT_HIP(hipSetDevice(ctx->device));
         BHT_HIP(hipStreamCreate(&stream));
         BHT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-        BHT_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-        BHT_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-        for (auto &e : ahead) BHT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        BHT_HIP(hipEventCreateWithFlags(&ev_fork, HANDOVER_EVENT));
+        BHT_HIP(hipEventCreateWithFlags(&ev_join, HANDOVER_EVENT));
+        for (auto &e : ahead) BHT_HIP(hipEventCreateWithFlags(&e, HANDOVER_EVENT));
     }
     ~bht_job() {
         for (auto e : events) hipEventDestroy(e);
@@ -1467,8 +1473,8 @@ struct bht_batch {
         BHT_HIP(hipSetDevice(ctx->device));
         BHT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         BHT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-        BHT_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-        BHT_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        BHT_HIP(hipEventCreateWithFlags(&ev_fork, HANDOVER_EVENT));
+        BHT_HIP(hipEventCreateWithFlags(&ev_join, HANDOVER_EVENT));
     }
     ~bht_batch() {
         drop_graph();
