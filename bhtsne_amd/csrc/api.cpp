@@ -259,6 +259,7 @@ struct bht_job {
     DevBuf rows, cols, vals;
     DevBuf tkey, tval, tblk, tseg_tile, tseg_e0, tseg_g0, tgrp;
     uint64_t nnz = 0;
+    uint64_t tile_nnz = 0;  // edges of the tiled copy, each group padded to a multiple of 8
     bool have_p = false;
     bool have_cached_perplexity = false;
     float cached_perplexity = 0.0f;
@@ -499,7 +500,7 @@ void fill_jobdev(bht_job *j) {
     d.vals = j->vals.as<float>();
     d.tile_rows = tiled_rows_per_block((int)j->n);
     d.tile_points = tiled_tile_points((int)j->dim());
-    d.tile_nnz = (uint32_t)j->nnz;
+    d.tile_nnz = (uint32_t)j->tile_nnz;
     d.tkey = j->tkey.as<uint32_t>();
     d.tval = j->tval.as<float>();
     d.blk_seg = j->tblk.as<uint32_t>();
@@ -547,9 +548,8 @@ void refresh_tiled(bht_job *j) {
     bht_ctx *c = j->ctx;
     const int n = (int)j->n, D = (int)j->dim();
     const TiledSizes z = tiled_sizes(n, D, j->nnz);
-    const uint64_t nnz1 = std::max<uint64_t>(j->nnz, 1);
-    j->tkey.ensure(c, nnz1 * 4);
-    j->tval.ensure(c, nnz1 * 4);
+    j->tkey.ensure(c, z.padded_cap * 4);
+    j->tval.ensure(c, z.padded_cap * 4);
     j->tblk.ensure(c, ((size_t)tiled_blocks(n) + 1) * 4);
     j->tseg_tile.ensure(c, z.max_segs * 4);
     j->tseg_e0.ensure(c, z.max_segs * 4);
@@ -560,9 +560,8 @@ void refresh_tiled(bht_job *j) {
     TiledOut o{j->tkey.as<uint32_t>(), j->tval.as<float>(), j->tblk.as<uint32_t>(),
                j->tseg_tile.as<uint32_t>(), j->tseg_e0.as<uint32_t>(), j->tseg_g0.as<uint32_t>(),
                j->tgrp.as<uint32_t>()};
-    tiled_build(j->rows.as<uint32_t>(), j->cols.as<uint32_t>(), j->vals.as<float>(), n, D,
-                j->nnz, o, temp.p, j->stream);
-    BHT_HIP(hipStreamSynchronize(j->stream));  // before the workspace is released
+    j->tile_nnz = tiled_build(j->rows.as<uint32_t>(), j->cols.as<uint32_t>(),
+                              j->vals.as<float>(), n, D, j->nnz, o, temp.p, j->stream);
 }
 
 // Refreshes the job's own device table entry (single-job launches); with `layout`, also the
@@ -979,15 +978,19 @@ void epoch_tail(bht_job *j, uint64_t epoch) {
     if (epoch == j->params.momentum_switch_epoch) j->params.momentum = j->params.final_momentum;
 }
 
-// FFT line points the epoch's FFT launches hold in LDS: a quarter more than the latest length the
-// job's grid plan reported (rounded to 64), at most the grid capacity; the whole capacity until the
-// plan has reported, and for jobs without the fallback buffer.
+// FFT line points the epoch's FFT launches hold in LDS: four times the latest length the job's grid
+// plan reported (rounded to 64), at most the grid capacity; the whole capacity until the plan has
+// reported, and for jobs without the fallback buffer.  The report is up to 16 epochs old (the host
+// enqueues ahead), and the length can grow that much meanwhile: early in a run a point thrown out
+// of the cloud took it from 300 to 1080 within four epochs.  Lines longer than the launch's LDS
+// take the global-memory fallback, which is several times slower (with a quarter more than the
+// report, a 300-epoch run at 1M points took 7% longer).
 int fft_cap_for(const bht_job *j) {
     if (!j->host_len || !j->hd.fft_fb) return j->len_cap;
     if (j->params.flags & BHT_FLAG_FFT_GLOBAL_LINES) return 64;
     const int seen = __atomic_load_n(j->host_len, __ATOMIC_ACQUIRE);
     if (seen <= 0) return j->len_cap;
-    const int want = (seen + seen / 4 + 63) / 64 * 64;
+    const int want = (4 * seen + 63) / 64 * 64;
     return std::min(want, j->len_cap);
 }
 
@@ -1020,11 +1023,13 @@ void enqueue_epoch(bht_job *j, const LaunchSet &L0, bool fit, bool fork, int que
         j->mark(j->stream, 2);
         if (fit) fitsne_repulsion(L, rs);
         else if (j->n > 0) bh_repulsion(L, rs);
+        // The update reads F- and Z written on `side`: the side writes its lines back and
+        // drops stale copies on every XCD before the join, while the attraction still runs
+        // (which neither reads nor writes them).
+        l2_sync(rs);
         j->mark(rs, 1);
         BHT_HIP(hipEventRecord(j->ev_join, j->side));
         BHT_HIP(hipStreamWaitEvent(j->stream, j->ev_join, 0));
-        // the update reads F- and Z written on `side`
-        l2_sync(j->stream);
     } else {
         if (fit) fitsne_repulsion(L, rs);
         else if (j->n > 0) bh_repulsion(L, rs);
@@ -1134,7 +1139,7 @@ TuneChoice autotune(bht_job *j, const LaunchSet &L, bool fit) {
     if (sl != 0 && j->epoch <= sl + 1 && sl + 1 < trials_end) return best;
     std::vector<TuneChoice> cands;
     const int nblk = L.nb[F_ATT];
-    for (int q : {256, 320, 352, 384, 416, 448, 512})
+    for (int q : {192, 256, 320, 384, 448, 512})
         if (q < nblk) cands.push_back(TuneChoice{1, q});
     cands.push_back(TuneChoice{1, 0});  // forked, one block per row block
     cands.push_back(TuneChoice{0, 0});  // in order on one stream
@@ -1256,7 +1261,7 @@ void finish_run(bht_job *j) {
         const float inv = 1.0f / j->params.early_exaggeration;
         scale_values(j->vals.as<float>(), j->nnz, inv, j->stream);
         scale_values(j->qv.as<float>(), std::min<uint64_t>(j->n, j->nnz), inv, j->stream);
-        if (j->epoch == sl + 1) scale_values(j->tval.as<float>(), j->nnz, inv, j->stream);
+        if (j->epoch == sl + 1) scale_values(j->tval.as<float>(), j->tile_nnz, inv, j->stream);
     }
     BHT_HIP(hipStreamSynchronize(j->stream));
     j->fit_kind = j->method;

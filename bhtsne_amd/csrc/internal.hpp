@@ -158,6 +158,7 @@ constexpr int MAX_UPDATE_PARTS = 1024;  // point_blocks() cap: partial sums of t
 // Column-tiled layout of P (tiled.hip).
 struct TiledSizes {
     uint64_t max_segs, max_groups;  // bounds of the segment and group tables
+    uint64_t padded_cap;            // bound of the padded edge count (tkey, tval entries)
     uint64_t persistent;            // bytes of the layout itself
     uint64_t temp;                  // bytes of workspace of tiled_build
 };
@@ -170,8 +171,9 @@ uint32_t tiled_rows_per_block(int n);
 uint32_t tiled_tile_points(int dim);
 int tiled_blocks(int n);
 TiledSizes tiled_sizes(int n, int dim, uint64_t nnz);
-void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, int n, int dim,
-                 uint64_t nnz, const TiledOut &o, void *temp, hipStream_t s);
+// Builds the layout; returns the padded edge count (tkey, tval entries).  Synchronises s.
+uint64_t tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, int n,
+                     int dim, uint64_t nnz, const TiledOut &o, void *temp, hipStream_t s);
 int spread_blocks(int n);
 int traverse_blocks(int n);
 void job_block_counts(JobDev &J, bool batched);

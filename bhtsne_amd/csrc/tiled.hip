@@ -8,16 +8,21 @@
 // block copies that tile's positions into LDS once and then reads every gathered position from LDS.
 //
 // Layout (built once per CSR layout by tiled_build):
-//   tkey[p], tval[p]   the edges, sorted by (row block, tile, row), stable; tkey packs the row
-//                      within its block (high 16 bits) and the column within its tile (low 16).
+//   tkey[p], tval[p]   the edges, sorted by (row block, tile, row), stable, every group padded
+//                      to a multiple of 8 edges with zero-valued edges of its last row; tkey packs
+//                      byte offsets: of the row's accumulator pair within its block (8 bytes per
+//                      row, high 16 bits) and of the column's position within its tile (4 D
+//                      bytes per point, low 16 bits), so they address LDS directly.
 //   blk_seg[b]         first segment of row block b (b = 0 .. nblocks; sentinel = nsegs)
 //   seg_tile[s]        tile of segment s (a segment = one block's edges in one tile)
 //   seg_e0[s]          first edge of segment s (sentinel = nnz)
 //   seg_g0[s]          first group of segment s (sentinel = ngroups)
 //   grp_e0[g]          first edge of group g (sentinel = nnz)
 // A group is a run of whole rows of a segment, as many as fit one wave pass (GROUP_SPAN edges from
-// the group's first edge rounded down to a multiple of 8), or a single row that is longer.  A
-// row's edges in a segment belong to exactly one group, which one wave processes alone.
+// the group's first edge rounded down to a multiple of 8 in the unpadded order), or a single row
+// that is longer.  A row's edges in a segment belong to exactly one group, which one wave
+// processes alone.  Padded, each group starts at a multiple of 8, so every lane's 8 edges belong
+// to one group or none.
 //
 // Within a group a wave takes GROUP_SPAN edges per pass, 8 consecutive edges per lane.  Each lane
 // sums its edges by row (runs of equal keys), rows complete inside the lane are added to the
@@ -44,13 +49,14 @@ constexpr int GROUP_SPAN = 512;  // edges per wave pass: 64 lanes x 8
 // with the two sides' speeds, so the driver's autotuner picks it per job size; this is the value
 // it starts from.  Alone on the device the attraction keeps a block per row block.
 constexpr int ATT_QUEUE_BLOCKS = 384;
-// LDS per block: window + the block's row positions + the accumulators, under 80,064 bytes so
-// that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran at half speed)
+// LDS per block: window + the block's row positions + the accumulators (64,096 bytes at D = 2),
+// under 80,064 bytes so that two blocks fit a CU (a block asking for the whole 160 KiB of a CU ran
+// at half speed)
 constexpr int WIN_FLOATS = 11800;
 constexpr uint32_t NO_ROW = 0xffffu;
 // the accumulators of a block's rows as (x, y) pairs (y unused for D = 1), then a dummy pair
-// per lane (adds that change no row)
-constexpr uint32_t ACC_DUMMY = TILE_ROWS_MAX * 2;
+// per lane (adds that change no row); byte offset of the first dummy pair
+constexpr uint32_t ACC_DUMMY_B = TILE_ROWS_MAX * 8;
 
 typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -76,7 +82,7 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t 
 
 struct TileGeom {
     uint32_t rows_per_block, tile_points;
-    int row_bits, tile_bits;
+    int row_bits, tile_bits, dim;
 };
 
 // key = block << (tile_bits + row_bits) | tile << row_bits | row within block; one wave per row
@@ -105,7 +111,7 @@ __global__ void k_tile_pack(const uint64_t *keys, const uint32_t *idx, const uin
         const uint32_t e = idx[p];
         const uint32_t r = (uint32_t)(k & ((1ull << g.row_bits) - 1));
         const uint32_t t = (uint32_t)((k >> g.row_bits) & ((1ull << g.tile_bits) - 1));
-        tkey[p] = (r << 16) | (cols[e] - t * g.tile_points);
+        tkey[p] = ((r * 8u) << 16) | ((cols[e] - t * g.tile_points) * 4u * (uint32_t)g.dim);
         tval[p] = vals[e];
         const uint64_t prev = p > 0 ? keys[p - 1] : ~0ull;
         seg_flag[p] = (p == 0 || (prev >> g.row_bits) != (k >> g.row_bits)) ? 1u : 0u;
@@ -189,15 +195,45 @@ __global__ void k_tile_index(const uint32_t *seg_blk, const uint32_t *seg_e0,
     }
 }
 
+// Padded length of every group (its edges rounded up to a multiple of 8); the sentinel's is 0.
+__global__ void k_tile_pad_len(const uint32_t *grp_e0, const uint32_t *seg_g0,
+                               const uint32_t *counts, uint32_t max_groups, uint32_t *plen) {
+    const uint32_t ngroups = seg_g0[counts[0]];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < max_groups; g += stride)
+        plen[g] = g < ngroups ? (grp_e0[g + 1] - grp_e0[g] + 7u) & ~7u : 0u;
+}
+
+// Copies every group to its padded start; the padding edges take the group's last row, the
+// column offset 0 and the value 0 (they add +-0 to that row's sum, which changes no bit).
+__global__ void k_tile_pad_copy(const uint32_t *ukey, const float *uval, const uint32_t *grp_e0,
+                                const uint32_t *pg0, const uint32_t *seg_g0,
+                                const uint32_t *counts, uint32_t *tkey, float *tval) {
+    const uint32_t ngroups = seg_g0[counts[0]];
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t g = wave; g < ngroups; g += nwaves) {
+        const uint32_t s = grp_e0[g], len = grp_e0[g + 1] - s, d = pg0[g], plen = pg0[g + 1] - d;
+        const uint32_t last_row = ukey[s + len - 1] & 0xffff0000u;
+        for (uint32_t i = lane; i < plen; i += 64) {
+            tkey[d + i] = i < len ? ukey[s + i] : last_row;
+            tval[d + i] = i < len ? uval[s + i] : 0.0f;
+        }
+    }
+}
+
 // ---- attraction -------------------------------------------------------------------------------
 
+// The position at byte offset `off` of an LDS array of points (D floats each).
 template <int D>
-__device__ __forceinline__ float2 lds_pos(const float *a, uint32_t i) {
+__device__ __forceinline__ float2 lds_pos_at(const float *a, uint32_t off) {
+    BHT_LDS const char *b = (BHT_LDS const char *)lds_view(a) + off;
     if (D == 2) {
-        const f2v v = ((BHT_LDS const f2v *)lds_view(a))[i];
+        const f2v v = *(BHT_LDS const f2v *)b;
         return make_float2(v.x, v.y);
     }
-    return make_float2(lds_view(a)[i], 0.0f);
+    return make_float2(*(BHT_LDS const float *)b, 0.0f);
 }
 
 // Copies points [first, first + count) of y (D floats each) into LDS, less the epoch's pending
@@ -224,6 +260,9 @@ __device__ void stage_points(BHT_GLOBAL const float *y, uint32_t first, uint32_t
             out[k] = src[k] - (D == 2 && (k & 1u) ? s1 : s0);
     }
 }
+
+// s_waitcnt vmcnt(0): waits for every vector memory access of the wave in flight.
+__device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0f70); }
 
 // A lane's 8 consecutive edges: packed keys and values.
 struct EdgeLoad {
@@ -256,22 +295,22 @@ __device__ __forceinline__ EdgeLoad load_edges(BHT_GLOBAL const uint32_t *tkey,
     return r;
 }
 
-// acc[at[u]] += (v0[u], v1[u]) for the lane's eight adds: all reads, then all writes (the eight
-// addresses are distinct rows or the lane's dummy slot; accumulators are stored as pairs).
+// The pair at byte offset at[u] of acc += (v0[u], v1[u]) for the lane's eight adds: all reads,
+// then all writes (the eight offsets are distinct rows or the lane's dummy pair).
 template <int D>
 __device__ __forceinline__ void acc_add8(float *acc, const uint32_t *at, const float *v0,
                                          const float *v1) {
-    BHT_LDS f2v *a = (BHT_LDS f2v *)lds_view(acc);
+    BHT_LDS char *a = (BHT_LDS char *)lds_view(acc);
     f2v cur[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) cur[u] = a[at[u] >> 1];
+    for (int u = 0; u < 8; u++) cur[u] = *(BHT_LDS f2v *)(a + at[u]);
 #pragma unroll
     for (int u = 0; u < 8; u++) {
         cur[u].x = cur[u].x + v0[u];
         if (D == 2) cur[u].y = cur[u].y + v1[u];
     }
 #pragma unroll
-    for (int u = 0; u < 8; u++) a[at[u] >> 1] = cur[u];
+    for (int u = 0; u < 8; u++) *(BHT_LDS f2v *)(a + at[u]) = cur[u];
 }
 
 // DPP controls (wave64): shifts within rows of 16 lanes, broadcasts of a row's last lane to the
@@ -318,13 +357,21 @@ __device__ __forceinline__ void seg_scan(float &s0, float &s1, int &flag) {
     seg_step<DPP_ROW_BCAST31, 0xc>(s0, s1, flag);
 }
 
+// LDS of a block: the column window, the block's row positions, the accumulators ((x, y) pairs).
+template <int D>
+constexpr size_t att_lds_bytes() {
+    return (size_t)(WIN_FLOATS + TILE_ROWS_MAX * D + TILE_ROWS_MAX * 2 + 64 * 2) * sizeof(float);
+}
+
+// Register budget: two blocks fill a CU's LDS share for this kernel (four waves per SIMD), and
+// the repulsion's kernels that run beside it on the same CU get the registers left over.  The
+// LDS is dynamic so that the compiler budgets registers for five waves per SIMD (at most 96 per
+// lane: 128 left for the other kernels' waves), not for the four the LDS would allow.
 template <int D, bool EPOCH>
-__global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
-                                                             const uint32_t *map,
-                                                             float *pos_out, int queued) {
-    __shared__ __align__(16) float win[WIN_FLOATS];
-    __shared__ __align__(16) float yrow[TILE_ROWS_MAX * D];
-    __shared__ __align__(16) float acc[TILE_ROWS_MAX * 2 + 64 * 2];  // (x, y) pairs
+__global__ void __launch_bounds__(T_THREADS) __attribute__((amdgpu_waves_per_eu(5)))
+k_attract_tiled(const JobDev *jobs, const uint32_t *map, float *pos_out, int queued) {
+    extern __shared__ __align__(16) float att_lds[];
+    float *win = att_lds, *yrow = att_lds + WIN_FLOATS, *acc = yrow + TILE_ROWS_MAX * D;
     __shared__ uint32_t next_lb;
     const BlockCtx bc = block_ctx(map, jobs, F_ATT);
     const JobDev &J = jobs[bc.job];
@@ -343,7 +390,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
     // stop_lying fires after epoch == stop_lying_epoch (lib.rs:1168-1171): the first epoch to
     // read the rescaled values rescales them in place as it reads
     const bool stop_now = EPOCH && J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
-    const float inv_exag = J.inv_exag;
+    const float p_scale = stop_now ? J.inv_exag : 1.0f;
     const float ys0 = J.scal->ysub[0], ys1 = J.scal->ysub[1];
 
     // queued: the blocks take row blocks from the job's work queue (reset by the update kernel);
@@ -369,7 +416,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
             // wave's k-th group.  Each pass's edges are loaded while the pass before is summed,
             // and the first pass's while the tile is copied into LDS.
             uint32_t gc = seg_g0[s] + wid;
-            uint32_t hb = 0, he = 0, ng = 0, k = 0, e_begin = 0, e_end = 0, base = 0;
+            uint32_t hb = 0, he = 0, ng = 0, k = 0, e_end = 0, base = 0;
             EdgeLoad cur = {};
             auto open_groups = [&]() {
                 const uint32_t gk = gc + (uint32_t)lane * T_WAVES;
@@ -378,54 +425,54 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                 const uint32_t left = (g_end - gc + T_WAVES - 1) / T_WAVES;
                 ng = left < 64u ? left : 64u;
                 k = 0;
-                e_begin = (uint32_t)__builtin_amdgcn_readlane((int)hb, 0);
+                base = (uint32_t)__builtin_amdgcn_readlane((int)hb, 0);
                 e_end = (uint32_t)__builtin_amdgcn_readlane((int)he, 0);
-                base = e_begin & ~7u;
                 cur = load_edges(tkey, tval, base + 8u * lane, nnz);
             };
             if (gc < g_end) open_groups();
             __syncthreads();  // the previous tile's reads are done
             stage_points<D>(y, first, count, win, ys0, ys1);
             __syncthreads();
+            // The first pass's edges have landed (the copy waited for its loads).  Said
+            // explicitly, so that the compiler's wait at the top of the pass loop does not also
+            // cover the next pass's loads, issued just before it.
+            vm_wait_all();
             while (gc < g_end) {
                 while (true) {
                     // the pass after this one: the group's next 512 edges or the next group
-                    uint32_t n_begin = e_begin, n_end = e_end, n_base = base + GROUP_SPAN, nk = k;
+                    uint32_t n_end = e_end, n_base = base + GROUP_SPAN, nk = k;
                     bool more = n_base < e_end;
                     if (!more && k + 1 < ng) {
                         nk = k + 1;
-                        n_begin = (uint32_t)__builtin_amdgcn_readlane((int)hb, (int)nk);
+                        n_base = (uint32_t)__builtin_amdgcn_readlane((int)hb, (int)nk);
                         n_end = (uint32_t)__builtin_amdgcn_readlane((int)he, (int)nk);
-                        n_base = n_begin & ~7u;
                         more = true;
                     }
                     EdgeLoad nxt = {};
                     if (more) nxt = load_edges(tkey, tval, n_base + 8u * lane, nnz);
+                    // A group starts at a multiple of 8 and holds a multiple of 8 edges, so a
+                    // lane's edges are all in the group or all past its end.  The lanes past the
+                    // end (the last lanes of the wave) hold real edges of other groups: their
+                    // offsets stay inside the LDS arrays, and what they compute reaches no row.
                     const uint32_t e0 = base + 8u * lane;
-                    // edges of this group only; the others get no row and no weight
-                    const uint32_t span = e_end - e_begin;
-                    uint32_t row[8], col[8];
-                    float pv[8];
+                    const bool lane_ok = e0 < e_end;
+                    uint32_t row[8];
+                    float *pv = cur.pv;
 #pragma unroll
                     for (int u = 0; u < 8; u++) {
-                        const bool ok = e0 + (uint32_t)u - e_begin < span;
-                        row[u] = ok ? cur.key[u] >> 16 : NO_ROW;
-                        col[u] = ok ? cur.key[u] & 0xffffu : 0u;
-                        pv[u] = ok ? cur.pv[u] : 0.0f;
+                        row[u] = cur.key[u] >> 16;
+                        pv[u] = pv[u] * p_scale;  // exact when p_scale is 1
                     }
-                    if (stop_now) {
-#pragma unroll
-                        for (int u = 0; u < 8; u++) {
-                            pv[u] = pv[u] * inv_exag;
-                            if (row[u] != NO_ROW) tval[e0 + u] = pv[u];
-                        }
+                    if (stop_now && lane_ok) {
+                        *(BHT_GLOBAL f4v *)(tval + e0) = f4v{pv[0], pv[1], pv[2], pv[3]};
+                        *(BHT_GLOBAL f4v *)(tval + e0 + 4) = f4v{pv[4], pv[5], pv[6], pv[7]};
                     }
                     // p q (y_i - y_j), q = 1 / (1 + |y_i - y_j|^2) (hardware reciprocal)
                     float f0[8], f1[8];
 #pragma unroll
                     for (int u = 0; u < 8; u++) {
-                        const float2 yi = lds_pos<D>(yrow, row[u] != NO_ROW ? row[u] : 0u);
-                        const float2 yj = lds_pos<D>(win, col[u]);
+                        const float2 yi = lds_pos_at<D>(yrow, D == 2 ? row[u] : row[u] >> 1);
+                        const float2 yj = lds_pos_at<D>(win, cur.key[u] & 0xffffu);
                         const float d0 = yi.x - yj.x;
                         const float d1 = D == 2 ? yi.y - yj.y : 0.0f;
                         float dist = d0 * d0;
@@ -441,48 +488,52 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                         f0[u] = same ? f0[u - 1] + f0[u] : f0[u];
                         f1[u] = same ? f1[u - 1] + f1[u] : f1[u];
                     }
-                    // the lanes' tail runs joined by a segmented scan across the wave
+                    // the lanes' tail runs joined by a segmented scan across the wave (a lane in
+                    // the group follows lanes in the group only)
                     const uint32_t tail_row = row[7];
                     const uint32_t prev_tail = (uint32_t)__builtin_amdgcn_update_dpp(
                         (int)NO_ROW, (int)tail_row, DPP_WAVE_SHR1, 0xf, 0xf, false);
-                    int flag = (lane == 0 || prev_tail != tail_row || tail_row == NO_ROW) ? 1 : 0;
+                    int flag = (lane == 0 || prev_tail != tail_row || !lane_ok) ? 1 : 0;
                     float s0 = f0[7], s1 = f1[7];
                     seg_scan(s0, s1, flag);
                     // what the lanes before hold of this lane's head row
                     const float p0 = dpp_f<DPP_WAVE_SHR1, 0xf>(-0.0f, s0);
                     const float p1 = dpp_f<DPP_WAVE_SHR1, 0xf>(-0.0f, s1);
-                    const bool joins = lane > 0 && prev_tail == row[0] && row[0] != NO_ROW;
+                    const bool joins = lane > 0 && prev_tail == row[0];
                     const uint32_t next_head = (uint32_t)__builtin_amdgcn_update_dpp(
-                        (int)NO_ROW, (int)row[0], DPP_WAVE_SHL1, 0xf, 0xf, false);
+                        (int)NO_ROW, (int)(lane_ok ? row[0] : NO_ROW), DPP_WAVE_SHL1, 0xf, 0xf,
+                        false);
                     // every run that ends in this lane is added to its row once; the lane's other
-                    // seven adds go to its own dummy slot.  No two lanes of the wave touch the
+                    // seven adds go to its own dummy pair.  No two lanes of the wave touch the
                     // same row here, and no other wave touches these rows in this segment, so
                     // the read-add-write needs no atomics (what several waves leave in a dummy
-                    // slot is never read).
-                    const uint32_t own = ACC_DUMMY + (uint32_t)lane * 2;
+                    // pair is never read).
+                    const uint32_t own = ACC_DUMMY_B + (uint32_t)lane * 8u;
                     uint32_t at[8];
                     float v0[8], v1[8];
 #pragma unroll
                     for (int u = 0; u < 7; u++) {
-                        const bool ends = row[u] != row[u + 1] && row[u] != NO_ROW;
+                        const bool ends = row[u] != row[u + 1];
                         const bool head = joins && row[u] == row[0];
-                        at[u] = ends ? row[u] * 2 : own;
+                        at[u] = ends ? row[u] : own;
                         v0[u] = head ? p0 + f0[u] : f0[u];
                         v1[u] = head ? p1 + f1[u] : f1[u];
                     }
-                    at[7] = tail_row != NO_ROW && next_head != tail_row ? tail_row * 2 : own;
+                    at[7] = next_head != tail_row ? tail_row : own;
                     v0[7] = s0;
                     v1[7] = s1;
-                    acc_add8<D>(acc, at, v0, v1);
+                    if (lane_ok) acc_add8<D>(acc, at, v0, v1);
                     if (!more) break;
                     cur = nxt;
                     base = n_base;
-                    e_begin = n_begin;
                     e_end = n_end;
                     k = nk;
                 }
                 gc += 64u * T_WAVES;
-                if (gc < g_end) open_groups();
+                if (gc < g_end) {
+                    open_groups();
+                    vm_wait_all();
+                }
             }
         }
         __syncthreads();
@@ -522,15 +573,18 @@ TiledSizes tiled_sizes(int n, int dim, uint64_t nnz) {
     z.max_segs = std::min<uint64_t>(nnz, nb * nt) + 1;
     // two consecutive groups of a segment hold more than GROUP_SPAN - 8 edges together
     z.max_groups = 2 * z.max_segs + 2 * (nnz / (GROUP_SPAN - 8)) + 2;
-    z.persistent = nnz * 8 + (nb + 1) * 4 + z.max_segs * 12 + (z.max_groups + 1) * 4;
+    // a group pads at most 7 edges, and no more groups than edges hold any
+    z.padded_cap = std::max<uint64_t>(nnz + 7 * std::min(z.max_groups, nnz), 8);
+    z.persistent = z.padded_cap * 8 + (nb + 1) * 4 + z.max_segs * 12 + (z.max_groups + 1) * 4;
     const uint64_t nn = std::max<uint64_t>(nnz, 1);
     z.temp = nn * 8 * 2 + nn * 4 * 2 + radix_sort_temp_bytes(nn) + nn * 4 * 4 +
-             scan_temp_bytes(nn) + z.max_segs * 8 + 64 + 12 * 256;
+             scan_temp_bytes(std::max(nn, z.max_groups)) + z.max_segs * 8 +
+             (2 * z.max_groups + 1) * 4 + 64 + 14 * 256;
     return z;
 }
 
-void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, int n, int dim,
-                 uint64_t nnz, const TiledOut &o, void *temp, hipStream_t s) {
+uint64_t tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, int n,
+                     int dim, uint64_t nnz, const TiledOut &o, void *temp, hipStream_t s) {
     const TiledSizes z = tiled_sizes(n, dim, nnz);
     const uint64_t nn = std::max<uint64_t>(nnz, 1);
     TileGeom g;
@@ -540,8 +594,11 @@ void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, 
     const uint64_t ntiles = ceil_div((uint64_t)(n > 0 ? n : 1), g.tile_points);
     g.row_bits = bits_for(g.rows_per_block);
     g.tile_bits = std::max(1, bits_for(ntiles));
+    g.dim = dim;
     const int key_bits = g.row_bits + g.tile_bits + bits_for(nblocks);
     if (key_bits > 64) throw Error(ERR_INVALID, "tiled layout: too many points");
+    // edge positions are 32-bit
+    if (z.padded_cap > 0xffffffffull) throw Error(ERR_INVALID, "tiled layout: too many affinities");
     // temp carve-up (256-byte aligned pieces)
     char *p = (char *)temp;
     auto take = [&](size_t bytes) {
@@ -554,15 +611,17 @@ void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, 
     void *sort_temp = take(radix_sort_temp_bytes(nn));
     uint32_t *seg_flag = (uint32_t *)take(nn * 4), *seg_pos = (uint32_t *)take(nn * 4);
     uint32_t *run_flag = (uint32_t *)take(nn * 4), *run_pos = (uint32_t *)take(nn * 4);
-    void *scan_temp = take(scan_temp_bytes(nn));
+    void *scan_temp = take(scan_temp_bytes(std::max(nn, z.max_groups)));
     uint32_t *seg_blk = (uint32_t *)take(z.max_segs * 4);
     uint32_t *seg_ngrp = (uint32_t *)take(z.max_segs * 4);
+    uint32_t *ugrp = (uint32_t *)take((z.max_groups + 1) * 4);  // group starts, unpadded
+    uint32_t *plen = (uint32_t *)take(z.max_groups * 4);        // padded group lengths
     uint32_t *counts = (uint32_t *)take(64);  // nsegs, nruns
     if (nnz == 0) {
         BHT_HIP(hipMemsetAsync(o.blk_seg, 0, (nblocks + 1) * 4, s));
         BHT_HIP(hipMemsetAsync(o.seg_g0, 0, 4, s));
         BHT_HIP(hipMemsetAsync(o.grp_e0, 0, 4, s));
-        return;
+        return 0;
     }
     const int nb_rows = (int)std::min<uint64_t>(ceil_div((uint64_t)n, 4), 8192);
     hipLaunchKernelGGL(k_tile_keys, dim3(nb_rows), dim3(256), 0, s, rows, cols, n, g, keys, idx);
@@ -571,9 +630,12 @@ void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, 
         radix_sort_pairs<uint64_t>(keys, idx, keys_alt, idx_alt, nnz, 0, key_bits, sort_temp, s);
     const uint64_t *sk = alt ? keys_alt : keys;
     const uint32_t *si = alt ? idx_alt : idx;
+    // the edges in sorted order, unpadded, in the sort's other key buffer
+    uint32_t *ukey = (uint32_t *)(alt ? keys : keys_alt);
+    float *uval = (float *)(ukey + nn);
     const int nb_e = (int)std::min<uint64_t>(ceil_div(nnz, 256), 16384);
     hipLaunchKernelGGL(k_tile_pack, dim3(nb_e), dim3(256), 0, s, sk, si, cols, vals, nnz, g,
-                       o.tkey, o.tval, seg_flag, run_flag);
+                       ukey, uval, seg_flag, run_flag);
     BHT_LAUNCH_CHECK();
     scan_exclusive<uint32_t>(seg_flag, seg_pos, nnz, scan_temp, counts, s);
     scan_exclusive<uint32_t>(run_flag, run_pos, nnz, scan_temp, counts + 1, s);
@@ -589,8 +651,24 @@ void tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, 
     scan_exclusive<uint32_t>(seg_ngrp, o.seg_g0, z.max_segs, scan_temp, counts + 2, s);
     const int nb_g = (int)std::min<uint64_t>(ceil_div(z.max_segs + nblocks + 1, 256), 8192);
     hipLaunchKernelGGL(k_tile_index, dim3(nb_g), dim3(256), 0, s, seg_blk, o.seg_e0, o.seg_g0,
-                       runs, counts, nblocks, nnz, o.blk_seg, o.grp_e0);
+                       runs, counts, nblocks, nnz, o.blk_seg, ugrp);
     BHT_LAUNCH_CHECK();
+    // the padded group starts: an exclusive scan of the padded lengths; its total (the padded
+    // edge count) is the sentinel after every group
+    const int nb_p = (int)std::min<uint64_t>(ceil_div(z.max_groups, 256), 8192);
+    hipLaunchKernelGGL(k_tile_pad_len, dim3(nb_p), dim3(256), 0, s, ugrp, o.seg_g0, counts,
+                       (uint32_t)z.max_groups, plen);
+    BHT_LAUNCH_CHECK();
+    scan_exclusive<uint32_t>(plen, o.grp_e0, z.max_groups, scan_temp, o.grp_e0 + z.max_groups,
+                             s);
+    hipLaunchKernelGGL(k_tile_pad_copy, dim3(nb_p), dim3(256), 0, s, ukey, uval, ugrp, o.grp_e0,
+                       o.seg_g0, counts, o.tkey, o.tval);
+    BHT_LAUNCH_CHECK();
+    uint32_t padded = 0;
+    BHT_HIP(hipMemcpyAsync(&padded, o.grp_e0 + z.max_groups, 4, hipMemcpyDeviceToHost, s));
+    BHT_HIP(hipStreamSynchronize(s));
+    if (padded > z.padded_cap) throw Error(ERR_STATE, "tiled layout: padded edges beyond their bound");
+    return padded;
 }
 
 int attract_default_queue_blocks() { return ATT_QUEUE_BLOCKS; }
@@ -599,18 +677,20 @@ void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s, in
     const int queued = epoch && queue_blocks > 0 && L.map[F_ATT] == nullptr ? 1 : 0;
     const dim3 g(queued ? std::min(L.nb[F_ATT], queue_blocks) : L.nb[F_ATT]);
     if (L.dim == 2) {
+        constexpr size_t lds = att_lds_bytes<2>();
         if (epoch)
-            hipLaunchKernelGGL((k_attract_tiled<2, true>), g, dim3(T_THREADS), 0, s, L.jobs,
+            hipLaunchKernelGGL((k_attract_tiled<2, true>), g, dim3(T_THREADS), lds, s, L.jobs,
                                L.map[F_ATT], nullptr, queued);
         else
-            hipLaunchKernelGGL((k_attract_tiled<2, false>), g, dim3(T_THREADS), 0, s, L.jobs,
+            hipLaunchKernelGGL((k_attract_tiled<2, false>), g, dim3(T_THREADS), lds, s, L.jobs,
                                L.map[F_ATT], pos, 0);
     } else {
+        constexpr size_t lds = att_lds_bytes<1>();
         if (epoch)
-            hipLaunchKernelGGL((k_attract_tiled<1, true>), g, dim3(T_THREADS), 0, s, L.jobs,
+            hipLaunchKernelGGL((k_attract_tiled<1, true>), g, dim3(T_THREADS), lds, s, L.jobs,
                                L.map[F_ATT], nullptr, queued);
         else
-            hipLaunchKernelGGL((k_attract_tiled<1, false>), g, dim3(T_THREADS), 0, s, L.jobs,
+            hipLaunchKernelGGL((k_attract_tiled<1, false>), g, dim3(T_THREADS), lds, s, L.jobs,
                                L.map[F_ATT], pos, 0);
     }
     BHT_LAUNCH_CHECK();
