@@ -257,7 +257,7 @@ struct bht_job {
     // affinities: CSR with u32 offsets on the device, and its column-tiled copy (the attraction's
     // layout: edges, values, block / segment / group tables)
     DevBuf rows, cols, vals;
-    DevBuf tkey, tval, tblk, tseg_tile, tseg_e0, tseg_g0, tgrp;
+    DevBuf tkey, tval, tblk, tseg_tile, tseg_e0, tseg_g0, tgrp, tgrp_row;
     uint64_t nnz = 0;
     uint64_t tile_nnz = 0;  // edges of the tiled copy, each group padded to a multiple of 8
     bool have_p = false;
@@ -501,13 +501,14 @@ void fill_jobdev(bht_job *j) {
     d.tile_rows = tiled_rows_per_block((int)j->n);
     d.tile_points = tiled_tile_points((int)j->dim());
     d.tile_nnz = (uint32_t)j->tile_nnz;
-    d.tkey = j->tkey.as<uint32_t>();
+    d.tkey = j->tkey.as<uint16_t>();
     d.tval = j->tval.as<float>();
     d.blk_seg = j->tblk.as<uint32_t>();
     d.seg_tile = j->tseg_tile.as<uint32_t>();
     d.seg_e0 = j->tseg_e0.as<uint32_t>();
     d.seg_g0 = j->tseg_g0.as<uint32_t>();
     d.grp_e0 = j->tgrp.as<uint32_t>();
+    d.grp_row = j->tgrp_row.as<uint32_t>();
     d.qv = j->qv.as<float>();
     d.qv_n = (int)std::min<uint64_t>(j->n, j->nnz);
     d.scal = j->scal.as<EpochScalars>();
@@ -548,18 +549,19 @@ void refresh_tiled(bht_job *j) {
     bht_ctx *c = j->ctx;
     const int n = (int)j->n, D = (int)j->dim();
     const TiledSizes z = tiled_sizes(n, D, j->nnz);
-    j->tkey.ensure(c, z.padded_cap * 4);
+    j->tkey.ensure(c, z.padded_cap * 2);
     j->tval.ensure(c, z.padded_cap * 4);
     j->tblk.ensure(c, ((size_t)tiled_blocks(n) + 1) * 4);
     j->tseg_tile.ensure(c, z.max_segs * 4);
     j->tseg_e0.ensure(c, z.max_segs * 4);
     j->tseg_g0.ensure(c, z.max_segs * 4);
     j->tgrp.ensure(c, (z.max_groups + 1) * 4);
+    j->tgrp_row.ensure(c, (z.max_groups + 1) * 4);
     DevBuf temp(c);
     temp.ensure(c, z.temp);
-    TiledOut o{j->tkey.as<uint32_t>(), j->tval.as<float>(), j->tblk.as<uint32_t>(),
+    TiledOut o{j->tkey.as<uint16_t>(), j->tval.as<float>(), j->tblk.as<uint32_t>(),
                j->tseg_tile.as<uint32_t>(), j->tseg_e0.as<uint32_t>(), j->tseg_g0.as<uint32_t>(),
-               j->tgrp.as<uint32_t>()};
+               j->tgrp.as<uint32_t>(),     j->tgrp_row.as<uint32_t>()};
     j->tile_nnz = tiled_build(j->rows.as<uint32_t>(), j->cols.as<uint32_t>(),
                               j->vals.as<float>(), n, D, j->nnz, o, temp.p, j->stream);
 }

@@ -102,9 +102,9 @@ struct JobDev {
     // column-tiled copy of P for the attraction (tiled.hip): rows in blocks of tile_rows, columns
     // in tiles of tile_points; tval is the copy the epochs read (and rescale at stop-lying)
     uint32_t tile_rows, tile_points, tile_nnz;
-    const uint32_t *tkey;
+    const uint16_t *tkey;
     float *tval;
-    const uint32_t *blk_seg, *seg_tile, *seg_e0, *seg_g0, *grp_e0;
+    const uint32_t *blk_seg, *seg_tile, *seg_e0, *seg_g0, *grp_e0, *grp_row;
     float *qv;          // values at the reference's CSR positions [0, qv_n) (KL weights)
     int qv_n;
     EpochScalars *scal;
@@ -163,9 +163,9 @@ struct TiledSizes {
     uint64_t temp;                  // bytes of workspace of tiled_build
 };
 struct TiledOut {
-    uint32_t *tkey;
+    uint16_t *tkey;
     float *tval;
-    uint32_t *blk_seg, *seg_tile, *seg_e0, *seg_g0, *grp_e0;
+    uint32_t *blk_seg, *seg_tile, *seg_e0, *seg_g0, *grp_e0, *grp_row;
 };
 uint32_t tiled_rows_per_block(int n);
 uint32_t tiled_tile_points(int dim);
