@@ -70,8 +70,16 @@ def lib():
                                                C.POINTER(C.c_double)]),
         "oracle_morton_encode2": (C.c_uint64, [C.c_uint32, C.c_uint32]),
         "oracle_quantize": (C.c_uint32, [C.c_float, C.c_float, C.c_float, C.c_uint32]),
+        "oracle_morton_encode": (None, [u32p, C.c_int, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64)]),
+        "oracle_morton_bits": (C.c_int, [C.c_int]),
         "oracle_bh_build": (C.c_int64, [f32p, C.c_int64, f32p, u32p, u32p, u8p, u8p, f32p]),
+        "oracle_bh_build_d": (C.c_int64, [f32p, C.c_int64, C.c_int, f32p, u32p, u32p, u8p, u8p,
+                                          f32p]),
+        "oracle_bh_build_ranges": (C.c_int64, [f32p, C.c_int64, C.c_int, f32p, u32p, u32p, u8p,
+                                               u8p, f32p, u32p, u32p]),
         "oracle_bh_repulsion": (C.c_float, [f32p, C.c_int64, C.c_float, f32p, f32p]),
+        "oracle_bh_repulsion_d": (C.c_float, [f32p, C.c_int64, C.c_int, C.c_float, f32p, f32p]),
         "oracle_forces": (C.c_float, [f32p, C.c_int64, C.c_int, u64p, u32p, f32p, C.c_int,
                                       C.c_float, C.c_int, f32p, f32p]),
         "oracle_run_epochs": (None, [f32p, f32p, f32p, C.c_int64, C.c_int, u64p, u32p, f32p,
@@ -286,29 +294,49 @@ def quantize(p, mn, inv_scale, max_bucket=0xFFFFFFFF):
     return lib().oracle_quantize(float(p), float(mn), float(inv_scale), int(max_bucket))
 
 
+def morton_encode(coords):
+    """Z-order code of D quantised coordinates (D = 2..7, morton/d2.rs .. d7.rs) as an int."""
+    c = np.ascontiguousarray(coords, np.uint32)
+    lo, hi = C.c_uint64(0), C.c_uint64(0)
+    lib().oracle_morton_encode(c, int(c.size), C.byref(lo), C.byref(hi))
+    return (hi.value << 64) | lo.value
+
+
+def morton_bits(D):
+    """Bits per axis of the D-dimensional Morton code (0 for an unsupported D)."""
+    return lib().oracle_morton_bits(int(D))
+
+
 def bh_build(y):
-    """Morton arena for D = 2 (src/tsne/arena.rs:137-328). Returns a dict of node arrays."""
+    """Morton arena for D = y.shape[1] in 2..7 (src/tsne/arena.rs:137-328). Returns a dict of
+    node arrays."""
     y = _f32(y)
-    n = y.shape[0]
+    n, D = y.shape
     cap = max(2 * n - 1, 1)
-    com = np.zeros((cap, 2), np.float32)
+    com = np.zeros((cap, D), np.float32)
     cnt = np.zeros(cap, np.uint32)
     fc = np.zeros(cap, np.uint32)
     cc = np.zeros(cap, np.uint8)
     lv = np.zeros(cap, np.uint8)
-    hw2 = np.zeros(33, np.float32)
-    nn = lib().oracle_bh_build(y, n, com, cnt, fc, cc, lv, hw2)
+    hw2 = np.zeros(morton_bits(D) + 1, np.float32)
+    ranges = np.zeros((cap, 2), np.uint32)
+    order = np.zeros(max(n, 1), np.uint32)
+    nn = lib().oracle_bh_build_ranges(y, n, D, com, cnt, fc, cc, lv, hw2, ranges, order)
+    if nn < 0:
+        raise ValueError(f"Barnes-Hut arena: D = {D} is not in 2..7")
     return dict(com=com[:nn], count=cnt[:nn], first_child=fc[:nn], child_count=cc[:nn],
-                level=lv[:nn], hw2=hw2)
+                level=lv[:nn], hw2=hw2, ranges=ranges[:nn], order=order[:n])
 
 
 def bh_repulsion(y, theta):
-    """Barnes-Hut non-edge forces and q sums (arena.rs:338-392, repulsion.rs:95-158)."""
+    """Barnes-Hut non-edge forces and q sums (arena.rs:338-392, repulsion.rs:95-158), D = 2..7."""
     y = _f32(y)
-    n = y.shape[0]
+    n, D = y.shape
+    if morton_bits(D) == 0:
+        raise ValueError(f"Barnes-Hut repulsion: D = {D} is not in 2..7")
     neg = np.zeros_like(y)
     q = np.zeros(n, np.float32)
-    s = lib().oracle_bh_repulsion(y, n, float(theta), neg, q)
+    s = lib().oracle_bh_repulsion_d(y, n, D, float(theta), neg, q)
     return neg, q, s
 
 

@@ -1143,22 +1143,50 @@ EXPORT void oracle_fitsne_repulsion_f64(const float *yf, int64_t n, int D, int m
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* Barnes-Hut Morton arena (src/tsne/arena.rs:137-392, morton.rs:103-120, morton/d2.rs)        */
+/* Barnes-Hut Morton arena for D = 2..7 (src/tsne/arena.rs:137-392, morton.rs:1-120,            */
+/* morton/d2.rs .. d7.rs)                                                                      */
 /* ------------------------------------------------------------------------------------------ */
 
-/* part_1by1 (d2.rs:8-16) */
-static uint64_t part_1by1(uint64_t x) {
-    x &= 0x00000000ffffffffull;
-    x = (x | (x << 16)) & 0x0000ffff0000ffffull;
-    x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
-    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
-    x = (x | (x << 2)) & 0x3333333333333333ull;
-    return (x | (x << 1)) & 0x5555555555555555ull;
+typedef unsigned __int128 u128;
+
+/* Bits per axis of the Morton code (morton/d2.rs:32, d3.rs:31, d4.rs:29, d5.rs:90, d6.rs:98,
+ * d7.rs:93): 32 / 21 / 16 / 25 / 21 / 18, so D * bits is at most 64 for D <= 4 (u64 words) and at
+ * most 126 for D >= 5 (u128 words). */
+static int morton_bits(int D) {
+    static const int b[8] = {0, 0, 32, 21, 16, 25, 21, 18};
+    return (D >= 2 && D <= 7) ? b[D] : 0;
+}
+
+/* Traversal stack length of the reference (Morton::Stack: 128 / 192 / 256 / 800 / 1344 / 2304). */
+static int morton_stack(int D) {
+    static const int s[8] = {0, 0, 128, 192, 256, 800, 1344, 2304};
+    return (D >= 2 && D <= 7) ? s[D] : 0;
+}
+
+/* Z-order interleave: bit b of axis a lands at position b * D + a (the definition the reference's
+ * per-D spreaders part_1by1 .. part_1by6 implement with masks and tables; pinned by their
+ * known-answer tests, d2.rs:56-75 .. d7.rs:149-158). */
+static u128 morton_encode(const uint32_t *c, int D) {
+    const int bits = morton_bits(D);
+    u128 code = 0;
+    for (int b = 0; b < bits; b++)
+        for (int a = 0; a < D; a++) code |= (u128)((c[a] >> b) & 1u) << (b * D + a);
+    return code;
 }
 
 EXPORT uint64_t oracle_morton_encode2(uint32_t c0, uint32_t c1) {
-    return part_1by1(c0) | (part_1by1(c1) << 1);
+    const uint32_t c[2] = {c0, c1};
+    return (uint64_t)morton_encode(c, 2);
 }
+
+/* Generic encode: the code's low and high 64-bit words. */
+EXPORT void oracle_morton_encode(const uint32_t *c, int D, uint64_t *lo, uint64_t *hi) {
+    const u128 code = morton_encode(c, D);
+    *lo = (uint64_t)code;
+    *hi = (uint64_t)(code >> 64);
+}
+
+EXPORT int oracle_morton_bits(int D) { return morton_bits(D); }
 
 /* quantize (morton.rs:103-120) for one axis */
 EXPORT uint32_t oracle_quantize(float p, float mn, float inv_scale, uint32_t max_bucket) {
@@ -1172,7 +1200,7 @@ EXPORT uint32_t oracle_quantize(float p, float mn, float inv_scale, uint32_t max
 }
 
 typedef struct {
-    float com[2];
+    float com[7];
     uint32_t count;
     uint32_t first_child;
     uint8_t child_count;
@@ -1182,7 +1210,7 @@ typedef struct {
 #define SENTINEL 0xffffffffu
 
 typedef struct {
-    uint64_t code;
+    u128 code;
     uint32_t idx;
 } codeidx;
 
@@ -1192,40 +1220,61 @@ static int cmp_codeidx(const void *a, const void *b) {
     return (x->idx > y->idx) - (x->idx < y->idx);
 }
 
+static int msb128(u128 x) {
+    const uint64_t hi = (uint64_t)(x >> 64), lo = (uint64_t)x;
+    return hi ? 127 - __builtin_clzll(hi) : 63 - __builtin_clzll(lo);
+}
+
 typedef struct {
     bhnode *nodes;
     int64_t n_nodes;
+    int D, bits;
     float hw2[33];
-} arena2;
+    uint32_t (*ranges)[2]; /* kept only when keep_ranges is set */
+    uint32_t *order;       /* sorted point indices, kept with the ranges */
+    int keep_ranges;
+} arena_t;
 
-/* Arena::rebuild for D = 2 (arena.rs:137-328) */
-static void arena_build(arena2 *ar, const float *y, int64_t n) {
-    const int bits = 32;
+/* Arena::rebuild (arena.rs:137-328) for D = 2..7 */
+static void arena_build(arena_t *ar, const float *y, int64_t n, int D) {
+    const int bits = morton_bits(D);
     ar->nodes = NULL;
     ar->n_nodes = 0;
+    ar->D = D;
+    ar->bits = bits;
+    ar->ranges = NULL;
+    ar->order = NULL;
     if (n == 0) return;
-    float mn[2] = {FLT_MAX, FLT_MAX}, mx[2] = {-FLT_MAX, -FLT_MAX};
+    /* bounding_box (arena.rs:65-94) */
+    float mn[7], mx[7];
+    for (int a = 0; a < D; a++) {
+        mn[a] = FLT_MAX;
+        mx[a] = -FLT_MAX;
+    }
     for (int64_t i = 0; i < n; i++)
-        for (int a = 0; a < 2; a++) {
-            mn[a] = fminf(mn[a], y[i * 2 + a]);
-            mx[a] = fmaxf(mx[a], y[i * 2 + a]);
+        for (int a = 0; a < D; a++) {
+            mn[a] = fminf(mn[a], y[i * D + a]);
+            mx[a] = fmaxf(mx[a], y[i * D + a]);
         }
-    float extent[2] = {mx[0] - mn[0], mx[1] - mn[1]};
-    float scale = (float)(1ull << bits);
-    uint32_t max_bucket = 0xffffffffu;
-    float inv_scale[2];
-    for (int a = 0; a < 2; a++) inv_scale[a] = extent[a] > 0.0f ? scale / extent[a] : 0.0f;
+    float extent[7], inv_scale[7];
+    const float scale = (float)(1ull << bits);
+    const uint32_t max_bucket = (uint32_t)((1ull << bits) - 1);
     float max_extent = 0.0f;
-    for (int a = 0; a < 2; a++) max_extent = fmaxf(max_extent, extent[a]);
+    for (int a = 0; a < D; a++) {
+        extent[a] = mx[a] - mn[a];
+        inv_scale[a] = extent[a] > 0.0f ? scale / extent[a] : 0.0f;
+        max_extent = fmaxf(max_extent, extent[a]);
+    }
     for (int level = 0; level <= bits; level++) {
         float hw = max_extent / (float)(1ull << (level + 1));
         ar->hw2[level] = hw * hw;
     }
     codeidx *sorted = (codeidx *)malloc(sizeof(codeidx) * (size_t)n);
     for (int64_t i = 0; i < n; i++) {
-        uint32_t q0 = oracle_quantize(y[i * 2], mn[0], inv_scale[0], max_bucket);
-        uint32_t q1 = oracle_quantize(y[i * 2 + 1], mn[1], inv_scale[1], max_bucket);
-        sorted[i].code = oracle_morton_encode2(q0, q1);
+        uint32_t q[7];
+        for (int a = 0; a < D; a++)
+            q[a] = oracle_quantize(y[i * D + a], mn[a], inv_scale[a], max_bucket);
+        sorted[i].code = morton_encode(q, D);
         sorted[i].idx = (uint32_t)i;
     }
     qsort(sorted, (size_t)n, sizeof(codeidx), cmp_codeidx);
@@ -1240,20 +1289,19 @@ static void arena_build(arena2 *ar, const float *y, int64_t n) {
     ranges[0][0] = 0;
     ranges[0][1] = (uint32_t)n;
     nn = 1;
-    const uint64_t mask = 3;
+    const u128 mask = ((u128)1 << D) - 1;
     for (int64_t node = 0; node < nn; node++) {
         uint32_t start = ranges[node][0], end = ranges[node][1];
         if (end - start <= 1 || sorted[start].code == sorted[end - 1].code) continue;
-        uint64_t x = sorted[start].code ^ sorted[end - 1].code;
-        int highest = 63 - __builtin_clzll(x);
-        int level = (bits - 1) - highest / 2;
-        int shift = 2 * (bits - 1 - level);
+        int highest = msb128(sorted[start].code ^ sorted[end - 1].code);
+        int level = (bits - 1) - highest / D;
+        int shift = D * (bits - 1 - level);
         nodes[node].level = (uint8_t)level;
         uint32_t first_child = (uint32_t)nn;
-        uint8_t child_count = 0;
+        int child_count = 0;
         uint32_t cs = start;
         while (cs < end) {
-            uint64_t g = (sorted[cs].code >> shift) & mask;
+            u128 g = (sorted[cs].code >> shift) & mask;
             uint32_t ce = cs + 1;
             while (ce < end && ((sorted[ce].code >> shift) & mask) == g) ce++;
             nodes[nn].count = ce - cs;
@@ -1267,56 +1315,58 @@ static void arena_build(arena2 *ar, const float *y, int64_t n) {
             cs = ce;
         }
         nodes[node].first_child = first_child;
-        nodes[node].child_count = child_count;
+        nodes[node].child_count = (uint8_t)child_count; /* at most 2^D <= 128 */
     }
     /* leaf COM (arena.rs:264-284) */
     for (int64_t node = 0; node < nn; node++) {
         if (nodes[node].first_child != SENTINEL) continue;
-        float c[2] = {0.0f, 0.0f};
+        float c[7] = {0};
         for (uint32_t s = ranges[node][0]; s < ranges[node][1]; s++) {
-            const float *p = y + (size_t)sorted[s].idx * 2;
-            c[0] += p[0];
-            c[1] += p[1];
+            const float *p = y + (size_t)sorted[s].idx * D;
+            for (int a = 0; a < D; a++) c[a] += p[a];
         }
         float inv = 1.0f / (float)nodes[node].count;
-        nodes[node].com[0] = c[0] * inv;
-        nodes[node].com[1] = c[1] * inv;
+        for (int a = 0; a < D; a++) nodes[node].com[a] = c[a] * inv;
     }
     /* internal COM bottom-up (arena.rs:289-309) */
     for (int64_t node = nn - 1; node >= 0; node--) {
         if (nodes[node].first_child == SENTINEL) continue;
         float total = (float)nodes[node].count;
-        float c[2] = {0.0f, 0.0f};
+        float c[7] = {0};
         for (uint32_t ch = nodes[node].first_child;
              ch < nodes[node].first_child + nodes[node].child_count; ch++) {
             float wgt = (float)nodes[ch].count;
-            c[0] += nodes[ch].com[0] * wgt;
-            c[1] += nodes[ch].com[1] * wgt;
+            for (int a = 0; a < D; a++) c[a] += nodes[ch].com[a] * wgt;
         }
         float inv = 1.0f / total;
-        nodes[node].com[0] = c[0] * inv;
-        nodes[node].com[1] = c[1] * inv;
+        for (int a = 0; a < D; a++) nodes[node].com[a] = c[a] * inv;
     }
-    free(ranges);
+    if (ar->keep_ranges) {
+        ar->ranges = ranges;
+        ar->order = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+        for (int64_t i = 0; i < n; i++) ar->order[i] = sorted[i].idx;
+    } else {
+        free(ranges);
+    }
     free(sorted);
     ar->nodes = nodes;
     ar->n_nodes = nn;
 }
 
-/* compute_non_edge_forces (arena.rs:338-392) */
-static void arena_non_edge(const arena2 *ar, int64_t index, float theta_sq, const float *y,
-                           float *neg, float *q_sum) {
+/* compute_non_edge_forces (arena.rs:338-392) with the reference's per-D stack length */
+static void arena_non_edge(const arena_t *ar, int64_t index, float theta_sq, const float *y,
+                           float *neg, float *q_sum, uint32_t *stack) {
     if (ar->n_nodes == 0) return;
-    const float *query = y + (size_t)index * 2;
-    uint32_t stack[128];
+    const int D = ar->D, cap = morton_stack(D);
+    const float *query = y + (size_t)index * D;
     int top = 0;
     stack[top++] = 0;
     while (top > 0) {
         top--;
         const bhnode *node = &ar->nodes[stack[top]];
-        float disp[2];
+        float disp[7];
         float distance = 0.0f;
-        for (int a = 0; a < 2; a++) {
+        for (int a = 0; a < D; a++) {
             float delta = query[a] - node->com[a];
             disp[a] = delta;
             distance += delta * delta;
@@ -1324,6 +1374,7 @@ static void arena_non_edge(const arena2 *ar, int64_t index, float theta_sq, cons
         if (node->first_child == SENTINEL) {
             if (distance == 0.0f) continue;
         } else if (ar->hw2[node->level] >= theta_sq * distance) {
+            if (top + node->child_count > cap) abort(); /* the reference's fixed stack overflows */
             for (uint32_t c = 0; c < node->child_count; c++) stack[top++] = node->first_child + c;
             continue;
         }
@@ -1331,47 +1382,79 @@ static void arena_non_edge(const arena2 *ar, int64_t index, float theta_sq, cons
         float magnitude = (float)node->count * inverse;
         *q_sum += magnitude;
         magnitude = magnitude * inverse;
-        for (int a = 0; a < 2; a++) neg[a] += magnitude * disp[a];
+        for (int a = 0; a < D; a++) neg[a] += magnitude * disp[a];
     }
 }
 
-/* Build the arena and export its nodes (for invariant tests).  Arrays sized 2n-1; returns the
- * node count.  hw2_out has 33 entries. */
-EXPORT int64_t oracle_bh_build(const float *y, int64_t n, float *com, uint32_t *count,
-                               uint32_t *first_child, uint8_t *child_count, uint8_t *level,
-                               float *hw2_out) {
-    arena2 ar;
-    arena_build(&ar, y, n);
+/* Build the arena and export its nodes (for invariant tests).  Arrays sized 2n-1 (com: D per
+ * node); returns the node count (-1 for an unsupported D).  hw2_out has bits + 1 entries.
+ * ranges_out (2 per node: start, end in the sorted order) and order_out (n sorted point
+ * indices) may be NULL. */
+EXPORT int64_t oracle_bh_build_ranges(const float *y, int64_t n, int D, float *com,
+                                      uint32_t *count, uint32_t *first_child,
+                                      uint8_t *child_count, uint8_t *level, float *hw2_out,
+                                      uint32_t *ranges_out, uint32_t *order_out) {
+    if (!morton_bits(D)) return -1;
+    arena_t ar;
+    ar.keep_ranges = ranges_out != NULL || order_out != NULL;
+    arena_build(&ar, y, n, D);
+    if (ar.ranges) {
+        for (int64_t i = 0; i < ar.n_nodes && ranges_out; i++) {
+            ranges_out[2 * i] = ar.ranges[i][0];
+            ranges_out[2 * i + 1] = ar.ranges[i][1];
+        }
+        for (int64_t i = 0; i < n && order_out; i++) order_out[i] = ar.order[i];
+        free(ar.ranges);
+        free(ar.order);
+    }
     for (int64_t i = 0; i < ar.n_nodes; i++) {
-        com[2 * i] = ar.nodes[i].com[0];
-        com[2 * i + 1] = ar.nodes[i].com[1];
+        for (int a = 0; a < D; a++) com[D * i + a] = ar.nodes[i].com[a];
         count[i] = ar.nodes[i].count;
         first_child[i] = ar.nodes[i].first_child;
         child_count[i] = ar.nodes[i].child_count;
         level[i] = ar.nodes[i].level;
     }
     if (hw2_out)
-        for (int l = 0; l < 33; l++) hw2_out[l] = ar.hw2[l];
+        for (int l = 0; l <= ar.bits; l++) hw2_out[l] = ar.hw2[l];
     int64_t nn = ar.n_nodes;
     free(ar.nodes);
     return nn;
 }
 
-/* Barnes-Hut repulsion for every point: neg (n*2) and per-point q sums; returns sum of q (f32,
+EXPORT int64_t oracle_bh_build_d(const float *y, int64_t n, int D, float *com, uint32_t *count,
+                                 uint32_t *first_child, uint8_t *child_count, uint8_t *level,
+                                 float *hw2_out) {
+    return oracle_bh_build_ranges(y, n, D, com, count, first_child, child_count, level, hw2_out,
+                                  NULL, NULL);
+}
+
+EXPORT int64_t oracle_bh_build(const float *y, int64_t n, float *com, uint32_t *count,
+                               uint32_t *first_child, uint8_t *child_count, uint8_t *level,
+                               float *hw2_out) {
+    return oracle_bh_build_d(y, n, 2, com, count, first_child, child_count, level, hw2_out);
+}
+
+/* Barnes-Hut repulsion for every point: neg (n*D) and per-point q sums; returns sum of q (f32,
  * sequential as repulsion.rs:155). */
-EXPORT float oracle_bh_repulsion(const float *y, int64_t n, float theta, float *neg,
-                                 float *q_sums) {
-    arena2 ar;
-    arena_build(&ar, y, n);
+EXPORT float oracle_bh_repulsion_d(const float *y, int64_t n, int D, float theta, float *neg,
+                                   float *q_sums) {
+    if (!morton_bits(D)) abort();
+    arena_t ar;
+    ar.keep_ranges = 0;
+    arena_build(&ar, y, n, D);
     float theta_sq = theta * theta;
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t i = 0; i < n; i++) {
-        float f[2] = {0.0f, 0.0f};
-        float q = 0.0f;
-        arena_non_edge(&ar, i, theta_sq, y, f, &q);
-        neg[i * 2] = f[0];
-        neg[i * 2 + 1] = f[1];
-        q_sums[i] = q;
+#pragma omp parallel
+    {
+        uint32_t *stack = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)morton_stack(D));
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < n; i++) {
+            float f[7] = {0};
+            float q = 0.0f;
+            arena_non_edge(&ar, i, theta_sq, y, f, &q, stack);
+            for (int a = 0; a < D; a++) neg[i * D + a] = f[a];
+            q_sums[i] = q;
+        }
+        free(stack);
     }
     float s = 0.0f;
     for (int64_t i = 0; i < n; i++) s = s + q_sums[i];
@@ -1379,12 +1462,17 @@ EXPORT float oracle_bh_repulsion(const float *y, int64_t n, float theta, float *
     return s;
 }
 
+EXPORT float oracle_bh_repulsion(const float *y, int64_t n, float theta, float *neg,
+                                 float *q_sums) {
+    return oracle_bh_repulsion_d(y, n, 2, theta, neg, q_sums);
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* One approximate epoch and the run loop (src/lib.rs:961-1014, 1162-1179; repulsion.rs)       */
 /* ------------------------------------------------------------------------------------------ */
 
 /* Forces for the current state: pos, neg (n*D) and returns inverse_norm (1/Z).
- * method 0 = FIt-SNE (min_boxes), 1 = Barnes-Hut (theta, D = 2 only). */
+ * method 0 = FIt-SNE (min_boxes, D = 1, 2), 1 = Barnes-Hut (theta, D = 2..7). */
 EXPORT float oracle_forces(const float *y, int64_t n, int D, const uint64_t *rows,
                            const uint32_t *cols, const float *vals, int method, float theta,
                            int min_boxes, float *pos, float *neg) {
@@ -1395,7 +1483,7 @@ EXPORT float oracle_forces(const float *y, int64_t n, int D, const uint64_t *row
         return 1.0f / z;
     }
     float *q = (float *)malloc(sizeof(float) * (size_t)n);
-    float s = oracle_bh_repulsion(y, n, theta, neg, q);
+    float s = oracle_bh_repulsion_d(y, n, D, theta, neg, q);
     free(q);
     return 1.0f / s;
 }
@@ -1472,7 +1560,7 @@ EXPORT void oracle_kl_divergence(const float *y, int64_t n, int D, const uint64_
         oracle_fitsne_repulsion(y, n, D, min_boxes, neg, &z, NULL);
     } else {
         float *q = (float *)malloc(sizeof(float) * (size_t)n);
-        z = oracle_bh_repulsion(y, n, theta, neg, q);
+        z = oracle_bh_repulsion_d(y, n, D, theta, neg, q);
         free(q);
     }
     free(neg);
