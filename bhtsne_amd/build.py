@@ -14,7 +14,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BHT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["sort.hip", "fitsne.hip", "fft.hip", "attract.hip", "tiled.hip", "affinity.hip",
-           "knn.hip", "reorder.hip", "bh.hip", "kl.hip", "exact.hip", "exactfit.hip", "sync.hip", "api.cpp"]
+           "knn.hip", "reorder.hip", "bh.hip", "bhnd.hip", "kl.hip", "exact.hip", "exactfit.hip", "sync.hip", "api.cpp"]
 
 # -ffp-contract=off: no FMA contraction, so per-element arithmetic keeps the reference's rounding
 # (explicit fmaf() calls in the FFT are unaffected).  Packed FP32 instructions are disabled.

@@ -296,6 +296,7 @@ struct bht_job {
     bool ahead_rec[4] = {};
     // Barnes-Hut workspace
     DevBuf codes, codes_alt, bidx, bidx_alt, bsort_temp, bnode, bmeta, brange, bhw2, bcount;
+    DevBuf bclo, bchi, bslo;  // two-word Morton codes (D >= 5): low / high words, sorted lows
     // the job's entry of the device job table (host copy + device copy)
     JobDev hd{};
     DevBuf jd;
@@ -389,8 +390,11 @@ void ensure_state_buffers(bht_job *j) {
     j->pos.ensure(j->ctx, b);
     j->scal.ensure(j->ctx, sizeof(EpochScalars));
     j->dpart.ensure(j->ctx, 4096 * sizeof(double));
-    j->mpart.ensure(j->ctx, 2 * (size_t)MAX_UPDATE_PARTS * sizeof(double));
-    j->fpart.ensure(j->ctx, 4 * 1024 * sizeof(float));
+    // update partials (the recentring sums, one per axis and block) and per-block bounds (min and
+    // max per axis, at least 4 floats per block)
+    const size_t axes = std::max<size_t>(2, j->dim());
+    j->mpart.ensure(j->ctx, axes * (size_t)MAX_UPDATE_PARTS * sizeof(double));
+    j->fpart.ensure(j->ctx, 2 * axes * (size_t)MAX_UPDATE_PARTS * sizeof(float));
 }
 
 // Jobs from this size on run their epochs forked and size their FFT launches dynamically.
@@ -455,8 +459,14 @@ void ensure_bh_workspace(bht_job *j) {
     j->bidx.ensure(c, n * 4);
     j->bidx_alt.ensure(c, n * 4);
     j->bsort_temp.ensure(c, group_sort_temp_bytes(n));
-    j->bnode.ensure(c, 2 * n * sizeof(float4));
+    const size_t node_bytes = j->dim() >= 3 ? (size_t)nd_node_floats((int)j->dim()) * 4 : 16;
+    j->bnode.ensure(c, 2 * n * node_bytes);
     j->bmeta.ensure(c, 2 * n * 4);
+    if (j->dim() >= 5) {
+        j->bclo.ensure(c, n * 8);
+        j->bchi.ensure(c, n * 8);
+        j->bslo.ensure(c, n * 8);
+    }
     j->brange.ensure(c, 4 * n * 4);
     j->bhw2.ensure(c, 64 * 4);
     j->bcount.ensure(c, 16);
@@ -539,6 +549,9 @@ void fill_jobdev(bht_job *j) {
         d.cidx = j->bidx.as<uint32_t>();
         d.scodes = alt ? j->codes_alt.as<uint64_t>() : j->codes.as<uint64_t>();
         d.scidx = alt ? j->bidx_alt.as<uint32_t>() : j->bidx.as<uint32_t>();
+        d.clo = j->bclo.as<uint64_t>();
+        d.chi = j->bchi.as<uint64_t>();
+        d.slo = j->bslo.as<uint64_t>();
     }
     job_block_counts(d, false);
 }
@@ -548,6 +561,16 @@ void fill_jobdev(bht_job *j) {
 void refresh_tiled(bht_job *j) {
     bht_ctx *c = j->ctx;
     const int n = (int)j->n, D = (int)j->dim();
+    if (D >= 3) {
+        // the attraction of D = 3..7 reads the CSR rows; tval is its copy of the values (rescaled
+        // in place at stop-lying)
+        j->tval.ensure(c, std::max<uint64_t>(j->nnz, 1) * 4);
+        if (j->nnz)
+            BHT_HIP(hipMemcpyAsync(j->tval.p, j->vals.p, j->nnz * 4, hipMemcpyDeviceToDevice,
+                                   j->stream));
+        j->tile_nnz = j->nnz;
+        return;
+    }
     const TiledSizes z = tiled_sizes(n, D, j->nnz);
     j->tkey.ensure(c, z.padded_cap * 2);
     j->tval.ensure(c, z.padded_cap * 4);
@@ -795,7 +818,9 @@ constexpr uint64_t REORDER_MIN_POINTS = 32768;
 constexpr uint64_t FORK_MIN_POINTS = 32768;
 
 bool reorder_enabled(const bht_job *j) {
-    return !(j->params.flags & BHT_FLAG_NO_REORDER) && j->n >= REORDER_MIN_POINTS;
+    // the renumbering serves the column-tiled attraction of D <= 2
+    return !(j->params.flags & BHT_FLAG_NO_REORDER) && j->n >= REORDER_MIN_POINTS &&
+           j->dim() <= 2;
 }
 
 // Renumbers the points in the diffusion order of the affinity graph: the attraction gathers
@@ -1193,7 +1218,10 @@ TuneChoice autotune(bht_job *j, const LaunchSet &L, bool fit) {
 
 TuneChoice epoch_choice(bht_job *j, const LaunchSet &L, bool fit) {
     TuneChoice ch;
-    ch.fork = j->n >= FORK_MIN_POINTS && !(j->params.flags & BHT_FLAG_SERIAL_EPOCH) ? 1 : 0;
+    ch.fork = j->n >= FORK_MIN_POINTS && j->dim() <= 2 &&
+                      !(j->params.flags & BHT_FLAG_SERIAL_EPOCH)
+                  ? 1
+                  : 0;
     ch.queue_blocks = ch.fork ? attract_default_queue_blocks() : 0;
     if (!ch.fork || (j->params.flags & BHT_FLAG_NO_AUTOTUNE)) return ch;
     TuneCache &tc = *j->ctx->tune;
@@ -1378,8 +1406,8 @@ void fit(bht_job *j, int method, float theta, bht_epoch_cb cb, void *user, bool 
 }
 
 void check_dims(const bht_job *j, int method) {
-    if (method == FIT_BH && j->dim() != 2)
-        throw Error(ERR_INVALID, "the Barnes-Hut path supports d_out = 2");
+    if (method == FIT_BH && (j->dim() < 2 || j->dim() > 7))
+        throw Error(ERR_INVALID, "the Barnes-Hut path supports d_out in 2..7");
     if (method == FIT_FITSNE && j->dim() != 1 && j->dim() != 2)
         throw Error(ERR_INVALID, "the FIt-SNE path supports d_out in {1, 2}");
 }
@@ -1917,8 +1945,8 @@ bht_status bht_job_exact(bht_job *job, uint32_t metric, bht_epoch_cb cb, void *u
     return guarded([&] {
         if (metric != BHT_METRIC_EUCLIDEAN && metric != BHT_METRIC_SQEUCLIDEAN)
             throw Error(ERR_INVALID, "unknown metric");
-        if (job->dim() != 1 && job->dim() != 2)
-            throw Error(ERR_INVALID, "the exact path supports d_out in {1, 2}");
+        if (job->dim() < 1 || job->dim() > 7)
+            throw Error(ERR_INVALID, "the exact path supports d_out in 1..7");
         check_perplexity(job);
         validate_init(job);
         prepare_exact(job, (int)metric);
@@ -2499,7 +2527,9 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
         b += 5 * n * 4 + n * D * 4 + scan_temp_bytes(n);  // renumbering; its transients fit above
     if (method == BHT_METHOD_BARNES_HUT) {
         b += n * (2 * 8 + 2 * 4) + group_sort_temp_bytes(n);  // Morton sort
-        b += 2 * n * 16 + 2 * n * 4 + 4 * n * 4;               // tree
+        if (D >= 5) b += 3 * n * 8;                            // two-word codes
+        const uint64_t node_bytes = D >= 3 && D <= 7 ? (uint64_t)nd_node_floats((int)D) * 4 : 16;
+        b += 2 * n * node_bytes + 2 * n * 4 + 4 * n * 4;       // tree
         b += pad_tile(n) * 24 + group_sort_temp_bytes(pad_tile(n));  // batch arena
     } else {
         uint32_t maxb = std::min<uint32_t>(std::max<uint32_t>(p.max_boxes, 1), MAX_BOXES);

@@ -133,6 +133,10 @@ __global__ void __launch_bounds__(UPD_THREADS) k_update(const JobDev *jobs, cons
 }
 
 void update_points(const LaunchSet &L, hipStream_t s) {
+    if (L.dim >= 3) {
+        nd_update(L, s);
+        return;
+    }
     const dim3 g(L.nb[F_PT]);
     if (L.dim == 2)
         hipLaunchKernelGGL(k_update<2>, g, dim3(UPD_THREADS), 0, s, L.jobs, L.map[F_PT]);

@@ -278,6 +278,7 @@ int traverse_blocks(int n) {
 }
 
 const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s) {
+    if (L.dim >= 3) return nd_bh_repulsion(L, s);
     hipLaunchKernelGGL(k_bh_codes, dim3(L.nb[F_PT]), dim3(256), 0, s, L.jobs, L.map[F_PT]);
     BHT_LAUNCH_CHECK();
     const bool alt = sort_group<uint64_t>(L, MORTON_KEY_BITS, s);

@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(XF_THREADS) k_exact_zrows(const JobDev *jobs, 
     const JobDev &J = jobs[0];
     const float *y = job_y(J);
     const int64_t n = J.n, i = blockIdx.x;
-    float yi[2] = {0.0f, 0.0f};
+    float yi[D];
 #pragma unroll
     for (int a = 0; a < D; a++) yi[a] = y[i * D + a];
     double z = 0.0;
@@ -105,10 +105,12 @@ __global__ void __launch_bounds__(XF_THREADS) k_exact_grad(const JobDev *jobs, c
     const int epoch = J.scal->epoch;
     const bool lied_off = J.stop_lying_epoch != 0 && epoch > J.stop_lying_epoch;
     const float inv_exag = J.inv_exag, inv_z = J.scal->inv_z;
-    float yi[2] = {0.0f, 0.0f};
+    float yi[D];
 #pragma unroll
     for (int a = 0; a < D; a++) yi[a] = y[i * D + a];
-    double g[2] = {0.0, 0.0};
+    double g[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) g[a] = 0.0;
     const float *prow = P + i * n;
     for (int64_t j = threadIdx.x; j < n; j += XF_THREADS) {
         const float q = 1.0f / (1.0f + sq_dist<D>(yi, y, j));
@@ -135,7 +137,7 @@ __global__ void __launch_bounds__(XF_THREADS) k_exact_kl(const JobDev *jobs, con
     const float *y = job_y(J);
     const int64_t n = J.n, i = blockIdx.x;
     const float inv_z = J.scal->inv_z;
-    float yi[2] = {0.0f, 0.0f};
+    float yi[D];
 #pragma unroll
     for (int a = 0; a < D; a++) yi[a] = y[i * D + a];
     double kl = 0.0;
@@ -175,14 +177,27 @@ void exact_symmetrize(float *p, int64_t n, hipStream_t s) {
     BHT_LAUNCH_CHECK();
 }
 
+// Instantiates f for the output dimension (1..7).
+template <typename F>
+void with_dim(int dim, F &&f) {
+    switch (dim) {
+    case 1: f(std::integral_constant<int, 1>()); break;
+    case 2: f(std::integral_constant<int, 2>()); break;
+    case 3: f(std::integral_constant<int, 3>()); break;
+    case 4: f(std::integral_constant<int, 4>()); break;
+    case 5: f(std::integral_constant<int, 5>()); break;
+    case 6: f(std::integral_constant<int, 6>()); break;
+    case 7: f(std::integral_constant<int, 7>()); break;
+    default: throw Error(ERR_INVALID, "the exact path supports d_out in 1..7");
+    }
+}
+
 void exact_z(const JobDev *jobs, int64_t n, int dim, double *zrow, hipStream_t s) {
     if (n == 0) return;
-    if (dim == 2)
-        hipLaunchKernelGGL(k_exact_zrows<2>, dim3((unsigned)n), dim3(XF_THREADS), 0, s, jobs,
-                           zrow);
-    else
-        hipLaunchKernelGGL(k_exact_zrows<1>, dim3((unsigned)n), dim3(XF_THREADS), 0, s, jobs,
-                           zrow);
+    with_dim(dim, [&](auto d) {
+        hipLaunchKernelGGL(k_exact_zrows<decltype(d)::value>, dim3((unsigned)n),
+                           dim3(XF_THREADS), 0, s, jobs, zrow);
+    });
     hipLaunchKernelGGL(k_exact_zfinal, dim3(1), dim3(1024), 0, s, jobs, (const double *)zrow);
     BHT_LAUNCH_CHECK();
 }
@@ -191,10 +206,10 @@ void exact_gradient(const JobDev *jobs, const float *P, int64_t n, int dim, doub
                     hipStream_t s) {
     if (n == 0) return;
     exact_z(jobs, n, dim, zrow, s);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_exact_grad<2>, dim3((unsigned)n), dim3(XF_THREADS), 0, s, jobs, P);
-    else
-        hipLaunchKernelGGL(k_exact_grad<1>, dim3((unsigned)n), dim3(XF_THREADS), 0, s, jobs, P);
+    with_dim(dim, [&](auto d) {
+        hipLaunchKernelGGL(k_exact_grad<decltype(d)::value>, dim3((unsigned)n), dim3(XF_THREADS),
+                           0, s, jobs, P);
+    });
     BHT_LAUNCH_CHECK();
 }
 
@@ -203,12 +218,10 @@ void exact_kl(const JobDev *jobs, const float *P, int64_t n, int dim, double *zr
     if (n == 0) return;
     exact_z(jobs, n, dim, zrow, s);
     double *klrow = zrow + n;
-    if (dim == 2)
-        hipLaunchKernelGGL(k_exact_kl<2>, dim3((unsigned)n), dim3(XF_THREADS), 0, s, jobs, P,
-                           klrow);
-    else
-        hipLaunchKernelGGL(k_exact_kl<1>, dim3((unsigned)n), dim3(XF_THREADS), 0, s, jobs, P,
-                           klrow);
+    with_dim(dim, [&](auto d) {
+        hipLaunchKernelGGL(k_exact_kl<decltype(d)::value>, dim3((unsigned)n), dim3(XF_THREADS),
+                           0, s, jobs, P, klrow);
+    });
     hipLaunchKernelGGL(k_exact_klfinal, dim3(1), dim3(1024), 0, s, jobs, (const double *)klrow);
     BHT_LAUNCH_CHECK();
 }

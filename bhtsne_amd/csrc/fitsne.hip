@@ -41,7 +41,7 @@ void job_block_counts(JobDev &J, bool batched) {
     J.nblk[F_PT] = point_blocks(n);
     J.nblk[F_ONE] = 1;
     J.nblk[F_SPR] = spread_blocks(n);
-    J.nblk[F_ATT] = tiled_blocks(n);
+    J.nblk[F_ATT] = J.dim >= 3 ? nd_attract_blocks(n) : tiled_blocks(n);
     J.nblk[F_TRAV] = traverse_blocks(n);
     J.nblk[F_LINE1] = 2;
     // transform lines per job: 2 * nodes (rows) and 2 * L (columns), a few hundred for batch-sized
@@ -296,6 +296,10 @@ void epoch_head(const LaunchSet &L, hipStream_t s) {
 }
 
 void center_points(const LaunchSet &L, hipStream_t s) {
+    if (L.dim >= 3) {
+        nd_center_points(L, s);
+        return;
+    }
     hipLaunchKernelGGL(k_mean_finalize, dim3(L.nb[F_ONE]), dim3(FIN_THREADS), 0, s, L.jobs,
                        L.map[F_ONE]);
     if (L.dim == 2)
@@ -308,6 +312,10 @@ void center_points(const LaunchSet &L, hipStream_t s) {
 }
 
 void plan_grid(const LaunchSet &L, bool do_plan, hipStream_t s) {
+    if (L.dim >= 3) {  // Barnes-Hut only: the bounds, no grid
+        nd_bounds(L, s);
+        return;
+    }
     const int *lo = L.tw ? L.tw->len_of_boxes : nullptr, *of = L.tw ? L.tw->off_of_boxes : nullptr;
     const int plan = do_plan && L.tw != nullptr;
     if (L.dim == 2)

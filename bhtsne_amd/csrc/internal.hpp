@@ -25,13 +25,13 @@ struct GridPlan {
 // what the rest of the epoch writes, so that the two streams never write into one 128-byte line.
 struct alignas(128) EpochScalars {
     // -- written by the repulsion's kernels (grid plan, bounds, Z)
-    float amin[2], amax[2];// per-axis bounding box after centring (Barnes-Hut)
+    float amin[8], amax[8];// per-axis bounding box after centring (Barnes-Hut, D <= 7)
     GridPlan plan;
     double zsum;           // sum of the per-point Z terms (FP64)
     float z, inv_z;        // Z and 1/Z of this epoch
     int sort_passes;       // 7-bit passes the box sort needs for this epoch's grid
     // -- written by the centring, attraction and update kernels and at set-up
-    alignas(128) float mean[2]; // pending recentring shift (applied by the next centring kernel)
+    alignas(128) float mean[8]; // pending recentring shift (applied by the next centring kernel)
     float ysub[2];         // shift every reader subtracts from the epoch's embedding buffer
     int epoch;             // index of the epoch being executed
     int pad0;
@@ -58,7 +58,8 @@ struct TwiddleSet {
 
 // Barnes-Hut tree in breadth-first order, as the reference's arena (src/tsne/arena.rs:36-62).
 struct BhNodes {
-    float4 *node;     // [2n] com.x, com.y, count (u32 bits), first_child (u32 bits, ~0 = leaf)
+    float4 *node;     // [2n] com.x, com.y, count (u32 bits), first_child (u32 bits, ~0 = leaf);
+                      // D >= 3: nd_node_floats(D) floats per node, com[D], count, first_child
     uint32_t *meta;   // [2n] child_count | level << 8
     uint32_t *range;  // [4n] start, end in the sorted order
     float *hw2;       // [33] squared half-width per level
@@ -128,11 +129,14 @@ struct JobDev {
     // word the grid plan reports its FFT length to (null: none)
     float2 *fft_fb;
     int *host_len;
-    // Barnes-Hut: Morton sort input and output, tree
+    // Barnes-Hut: Morton sort input and output, tree.  Two-word codes (D >= 5) keep their low and
+    // high words per point (clo, chi) and the low words in sorted order (slo); scodes then holds
+    // the sorted high words.
     uint64_t *codes;
     uint32_t *cidx;
     const uint64_t *scodes;
     const uint32_t *scidx;
+    uint64_t *clo, *chi, *slo;
     BhNodes nodes;
 };
 
@@ -281,6 +285,19 @@ void fft_convolve(const LaunchSet &L, hipStream_t s);
 
 // Barnes-Hut (bh.hip): after center_plan (no plan), frep and Z; returns job 0's sorted indices.
 const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s);
+
+// Barnes-Hut epochs with D = 3..7 (bhnd.hip): the recentring, the bounds, the repulsion, the
+// attraction over the CSR (tval: the values in CSR order) and the update, for a launch group of
+// that dimension.  center_points, plan_grid, bh_repulsion, attract_tiled and update_points hand
+// such groups to these.
+int nd_bits(int dim);             // bits per axis of the Morton code
+int nd_node_floats(int dim);      // floats per tree node
+int nd_attract_blocks(int n);     // F_ATT blocks of a job
+void nd_center_points(const LaunchSet &L, hipStream_t s);
+void nd_bounds(const LaunchSet &L, hipStream_t s);
+const uint32_t *nd_bh_repulsion(const LaunchSet &L, hipStream_t s);
+void nd_attract(const LaunchSet &L, float *pos, bool epoch, hipStream_t s);
+void nd_update(const LaunchSet &L, hipStream_t s);
 
 // Attraction sums over the tiled layout (tiled.hip): epoch = true writes fattr (and applies the
 // stop-lying rescale to tval), epoch = false writes job 0's sums into pos.

@@ -22,16 +22,17 @@ __global__ void __launch_bounds__(256) k_sparse_kl(const float *y, int n, const 
     double ref = 0.0, tru = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const float pi = qv[perm ? perm[i] : (uint32_t)i];
-        float a[2] = {y[(size_t)i * D], D == 2 ? y[(size_t)i * D + 1] : 0.0f};
+        float a[D];
+#pragma unroll
+        for (int t = 0; t < D; t++) a[t] = y[(size_t)i * D + t];
         float cost = 0.0f;
         for (uint32_t e = rows[i]; e < rows[i + 1]; e++) {
             const uint32_t j = cols[e];
             float q = 0.0f;
-            float d0 = a[0] - y[(size_t)j * D];
-            q = q + d0 * d0;
-            if (D == 2) {
-                float d1 = a[1] - y[(size_t)j * D + 1];
-                q = q + d1 * d1;
+#pragma unroll
+            for (int t = 0; t < D; t++) {
+                const float dt = a[t] - y[(size_t)j * D + t];
+                q = q + dt * dt;
             }
             q = (1.0f / (1.0f + q)) * inv_z;
             cost += pi * logf((pi + FLT_MIN) / (q + FLT_MIN));
@@ -66,12 +67,20 @@ void sparse_kl(const float *y, int n, int dim, const uint32_t *rows, const uint3
                const float *vals, const float *qv, const uint32_t *perm, EpochScalars *scal,
                double *dpart, hipStream_t s) {
     int nb = (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), 256), 2048);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_sparse_kl<2>, dim3(nb), dim3(256), 0, s, y, n, rows, cols, vals, qv,
-                           perm, scal, dpart);
-    else
-        hipLaunchKernelGGL(k_sparse_kl<1>, dim3(nb), dim3(256), 0, s, y, n, rows, cols, vals, qv,
-                           perm, scal, dpart);
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(nb), dim3(256), 0, s, y, n, rows, cols, vals, qv, perm,
+                           scal, dpart);
+    };
+    switch (dim) {
+    case 1: launch(k_sparse_kl<1>); break;
+    case 2: launch(k_sparse_kl<2>); break;
+    case 3: launch(k_sparse_kl<3>); break;
+    case 4: launch(k_sparse_kl<4>); break;
+    case 5: launch(k_sparse_kl<5>); break;
+    case 6: launch(k_sparse_kl<6>); break;
+    case 7: launch(k_sparse_kl<7>); break;
+    default: throw Error(ERR_INVALID, "KL divergence: d_out must be in 1..7");
+    }
     BHT_LAUNCH_CHECK();
 }
 

@@ -791,6 +791,10 @@ uint64_t tiled_build(const uint32_t *rows, const uint32_t *cols, const float *va
 int attract_default_queue_blocks() { return ATT_QUEUE_BLOCKS; }
 
 void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s, int queue_blocks) {
+    if (L.dim >= 3) {  // plain CSR rows (Barnes-Hut, D = 3..7)
+        nd_attract(L, pos, epoch, s);
+        return;
+    }
     const int queued = epoch && queue_blocks > 0 && L.map[F_ATT] == nullptr ? 1 : 0;
     const dim3 g(queued ? std::min(L.nb[F_ATT], queue_blocks) : L.nb[F_ATT]);
     if (L.dim == 2) {

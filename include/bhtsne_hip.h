@@ -58,7 +58,8 @@ typedef void (*bht_epoch_cb)(uint64_t epoch, const float *embedding, uint64_t le
 
 /* Builder state of tSNE (src/lib.rs:149-176) with the defaults of tSNE::new (src/lib.rs:241-266). */
 typedef struct bht_params {
-    uint32_t d_out;                 /* embedding dimension: 2 (Barnes-Hut), 1 or 2 (FIt-SNE) */
+    uint32_t d_out;                 /* embedding dimension: 2..7 (Barnes-Hut), 1 or 2 (FIt-SNE),
+                                       1..7 (exact) */
     uint32_t bandwidth_mode;        /* BHT_BANDWIDTH_* */
     uint64_t epochs;                /* 1000 */
     uint64_t momentum_switch_epoch; /* 250 */

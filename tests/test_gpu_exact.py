@@ -65,6 +65,21 @@ def test_exact_epochs_match_oracle(bt, metric, epochs, stop_lying):
     assert _rel(y, yr) < 1e-5
 
 
+@pytest.mark.parametrize("D,epochs", [(1, 1), (3, 50), (5, 50), (7, 50)])
+def test_exact_epochs_match_oracle_in_d_dimensions(bt, D, epochs):
+    # the reference's exact fit takes any output dimension (src/lib.rs:489-646).  In one
+    # dimension the points' order along the line amplifies last-bit differences (50 epochs:
+    # 9.3e-5 against the oracle), so D = 1 is compared after one epoch.
+    x = lcg_samples(60, 4, 7)
+    y0 = lcg_samples(60, D, 99 + D)
+    t = bt.TSNE(x, d_out=D).perplexity(10.0).epochs(epochs).bandwidth_mode(SEQ)
+    t.stop_lying_epoch(0).initial_embedding(y0).exact("sqeuclidean")
+    y = t.embedding().reshape(-1, D)
+    yr, _ = _oracle(x, y0, epochs, "sqeuclidean", stop_lying_epoch=0)
+    print(f"exact D={D} {epochs} epochs: rel {_rel(y, yr):.2e}", flush=True)
+    assert _rel(y, yr) < 1e-5
+
+
 def test_exact_final_kl_within_one_percent(bt):
     # crosses stop-lying (250) and the momentum switch; KL against the reference's formula
     x = lcg_samples(1000, 10, 3)
