@@ -183,12 +183,27 @@ static_assert(KCAND + MC <= KBUF, "candidate buffer");
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// Element (row, d) of the samples, 0 outside [0, n) x [0, D).  The load address is clamped into
-// the array so that the load is unconditional (no branch per operand).
-__device__ __forceinline__ float operand(const float *x, int n, int D, int row, int d) {
-    const bool ok = row < n && d < D;
-    const float v = x[(size_t)(row < n ? row : n - 1) * D + (d < D ? d : D - 1)];
-    return ok ? v : 0.0f;
+// Packed operands.  For the MFMA's 16 x 4 operand of k-step kb, lane l holds row (l & 15) of a
+// 16-point tile and dimension 4 kb + (l >> 4) (A: query rows; B: candidate columns, the same
+// layout).  xp stores, per tile t and lane l, that lane's KSP values (KSP = the k-steps rounded up
+// to a multiple of 4) contiguously: xp[(t * 64 + l) * KSP + kb], 0 outside the data, so a lane
+// loads its operands of a tile as KSP / 4 aligned float4s.  The norms of points past n are +inf:
+// such a column's approximate distance is +inf and never passes a row's threshold.
+__global__ void __launch_bounds__(256) k_knn_pack(const float *x, int n, int D, int ksp,
+                                                  uint32_t ntiles, float *xp, float *nrm) {
+    const uint64_t total = (uint64_t)ntiles * 64 * (uint64_t)ksp;
+    for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256) {
+        const int kb = (int)(e % (uint64_t)ksp);
+        const uint64_t tl = e / (uint64_t)ksp;
+        const int l = (int)(tl & 63);
+        const uint64_t row = (tl >> 6) * 16 + (uint64_t)(l & 15);
+        const int d = 4 * kb + (l >> 4);
+        xp[e] = (row < (uint64_t)n && d < D) ? x[row * D + d] : 0.0f;
+    }
+    const uint64_t npad = (uint64_t)ntiles * 16;
+    for (uint64_t r = (uint64_t)n + blockIdx.x * 256ull + threadIdx.x; r < npad;
+         r += (uint64_t)gridDim.x * 256)
+        nrm[r] = INFINITY;
 }
 
 // Block barrier ordering LDS accesses only: global loads issued before it stay in flight.
@@ -214,77 +229,101 @@ __global__ void __launch_bounds__(256) k_sqnorm(const float *x, int n, int D, fl
     if ((threadIdx.x & 63) == 0) atomicMax(max_bits, __float_as_uint(m));
 }
 
-__global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *x, const float *nrm, int n,
-                                                          int D, uint32_t *cand, float *tau_out) {
+// The KS k-steps of a lane's operands of tile `tile` (KSP values per lane, float4 aligned).
+template <int KS>
+__device__ __forceinline__ void load_operands(const float *xp, uint32_t tile, int lane, float *v) {
+    constexpr int KSP = (KS + 3) & ~3;
+    const float4 *p = reinterpret_cast<const float4 *>(xp + ((size_t)tile * 64 + lane) * KSP);
+#pragma unroll
+    for (int g = 0; g < KSP / 4; g++) {
+        const float4 q = p[g];
+        if (4 * g + 0 < KS) v[4 * g + 0] = q.x;
+        if (4 * g + 1 < KS) v[4 * g + 1] = q.y;
+        if (4 * g + 2 < KS) v[4 * g + 2] = q.z;
+        if (4 * g + 3 < KS) v[4 * g + 3] = q.w;
+    }
+}
+
+// A block owns MQ = 32 query rows (two 16-row tiles); per step each of its four waves takes one
+// 16-candidate tile, so a step covers 64 candidates.  Per row, the candidates whose approximate
+// squared distance is below the row's threshold (the KCAND-th kept one, +inf until KCAND are
+// kept; a register copy per lane, reloaded after each compaction phase) are appended to the row's
+// LDS buffer; a row whose buffer may not take another step's appends is sorted by (distance,
+// index) and cut to KCAND.  A pair is only compared in f32 against the threshold: the append
+// path runs for the waves that have a pair below it.
+template <int KS>
+__global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *xp, const float *nrm,
+                                                          int n, uint32_t ntiles, uint32_t *cand,
+                                                          float *tau_out) {
     __shared__ uint64_t buf[MQ][KBUF];
     __shared__ uint32_t cnt[MQ];
-    __shared__ uint64_t tau[MQ];
+    __shared__ __align__(16) float tauf[MQ];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int q0 = blockIdx.x * MQ;
-    const int ksteps = (D + 3) / 4;
+    const uint32_t qt0 = (uint32_t)q0 / 16;
     if (t < MQ) {
         cnt[t] = 0;
-        tau[t] = ~0ull;
+        tauf[t] = INFINITY;
     }
-    // A operands (16 x 4 per step): lane holds query row (lane % 16) of the tile, dimension
-    // 4 * step + lane / 16
-    float a[2][MKS];
-#pragma unroll
-    for (int rt = 0; rt < 2; rt++) {
-        const int row = q0 + rt * 16 + (lane & 15);
-#pragma unroll
-        for (int kb = 0; kb < MKS; kb++) a[rt][kb] = operand(x, n, D, row, kb * 4 + (lane >> 4));
-    }
+    float a[2][KS];
+    load_operands<KS>(xp, qt0, lane, a[0]);
+    load_operands<KS>(xp, qt0 + 1, lane, a[1]);
     // output element r of a lane is row 4 * (lane / 16) + r of the tile, column lane % 16
     float qn[2][4];
 #pragma unroll
     for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = q0 + rt * 16 + 4 * (lane >> 4) + r;
-            qn[rt][r] = row < n ? nrm[row] : 0.0f;
-        }
+        for (int r = 0; r < 4; r++) qn[rt][r] = nrm[q0 + rt * 16 + 4 * (lane >> 4) + r];
+    float b[KS];
+    load_operands<KS>(xp, (uint32_t)w, lane, b);
+    float cn = nrm[w * 16 + (lane & 15)];
     __syncthreads();
-    // B operands (4 x 16 per step): lane holds dimension 4 * step + lane / 16 of candidate column
-    // lane % 16.  The next tile's operands and norms are loaded while this tile is reduced; the
-    // block barriers order LDS only, so those loads stay in flight across them.
-    float b[MKS];
-    float cn = 0.0f;
-    {
-        const int col = w * 16 + (lane & 15);
-#pragma unroll
-        for (int kb = 0; kb < MKS; kb++) b[kb] = operand(x, n, D, col, kb * 4 + (lane >> 4));
-        cn = col < n ? nrm[col] : 0.0f;
-    }
-    for (int c0 = 0; c0 < n; c0 += MC) {
-        const int col = c0 + w * 16 + (lane & 15);
-        const int ncol = col + MC;
-        float bn[MKS];
-#pragma unroll
-        for (int kb = 0; kb < MKS; kb++) bn[kb] = operand(x, n, D, ncol, kb * 4 + (lane >> 4));
-        const float cnn = nrm[ncol < n ? ncol : n - 1];
+    const uint32_t nsteps = ntiles / 4;
+    for (uint32_t st = 0; st < nsteps; st++) {
+        const uint32_t ct = st * 4 + (uint32_t)w;
+        const int col = (int)(ct * 16) + (lane & 15);
+        // the next step's operands are loaded while this step is reduced
+        float bn[KS];
+        const uint32_t nt = st + 1 < nsteps ? ct + 4 : ct;
+        load_operands<KS>(xp, nt, lane, bn);
+        const float cnn = nrm[nt * 16 + (lane & 15)];
         v4f acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int kb = 0; kb < MKS; kb++) {
-            if (kb < ksteps) {
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][kb], b[kb], acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][kb], b[kb], acc1, 0, 0, 0);
-            }
+        for (int kb = 0; kb < KS; kb++) {
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][kb], b[kb], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][kb], b[kb], acc1, 0, 0, 0);
         }
+        // the step whose columns hold the block's own rows excludes each row's own point
+        const bool diag = (int)(st * 64) <= q0 && q0 < (int)(st * 64 + 64);
+        float approx[2][4];
+        bool any = false;
 #pragma unroll
-        for (int rt = 0; rt < 2; rt++)
+        for (int rt = 0; rt < 2; rt++) {
+            const float4 tq = *reinterpret_cast<const float4 *>(&tauf[rt * 16 + 4 * (lane >> 4)]);
+            const float tqv[4] = {tq.x, tq.y, tq.z, tq.w};
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int qi = rt * 16 + 4 * (lane >> 4) + r, gq = q0 + qi;
-                if (gq >= n || col >= n || col == gq) continue;
                 const float dot = rt == 0 ? acc0[r] : acc1[r];
-                const float approx = fmaxf((qn[rt][r] + cn) - 2.0f * dot, 0.0f);
-                const uint64_t key = ((uint64_t)__float_as_uint(approx) << 32) | (uint32_t)col;
-                if (key < tau[qi]) {
+                float ap = fmaxf((qn[rt][r] + cn) - 2.0f * dot, 0.0f);
+                if (diag && col == q0 + rt * 16 + 4 * (lane >> 4) + r) ap = INFINITY;
+                approx[rt][r] = ap;
+                any |= ap < tqv[r];
+                if (!(ap < tqv[r])) approx[rt][r] = INFINITY;
+            }
+        }
+        if (__ballot(any) != 0ull) {
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    if (approx[rt][r] == INFINITY) continue;
+                    const int qi = rt * 16 + 4 * (lane >> 4) + r;
+                    const uint64_t key =
+                        ((uint64_t)__float_as_uint(approx[rt][r]) << 32) | (uint32_t)col;
                     const uint32_t pos = atomicAdd(&cnt[qi], 1u);
                     buf[qi][pos] = key;
                 }
-            }
+        }
         lds_barrier();
         for (int qi = w; qi < MQ; qi += KNN_THREADS / 64) {
             const uint32_t c = cnt[qi];
@@ -296,13 +335,14 @@ __global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *x, const 
             if (lane == 0) {
                 const uint32_t keep = c < (uint32_t)KCAND ? c : (uint32_t)KCAND;
                 cnt[qi] = keep;
-                if (keep == (uint32_t)KCAND) tau[qi] = buf[qi][KCAND - 1];
+                if (keep == (uint32_t)KCAND)
+                    tauf[qi] = __uint_as_float((uint32_t)(buf[qi][KCAND - 1] >> 32));
             }
             __builtin_amdgcn_wave_barrier();
         }
         lds_barrier();
 #pragma unroll
-        for (int kb = 0; kb < MKS; kb++) b[kb] = bn[kb];
+        for (int kb = 0; kb < KS; kb++) b[kb] = bn[kb];
         cn = cnn;
     }
     for (int qi = w; qi < MQ; qi += KNN_THREADS / 64) {
@@ -372,13 +412,20 @@ __global__ void k_list_rows(const uint32_t *flag, const uint32_t *pos, int n, ui
     if (i < n && flag[i]) list[pos[i]] = (uint32_t)i;
 }
 
-// Workspace pieces, each starting on a 256-byte boundary: norms, candidates, tau, flags,
-// positions, row list, misc words, scan temp.
-static size_t knn_layout(size_t nn, size_t *off) {
-    const size_t piece[8] = {nn * 4, nn * KCAND * 4, nn * 4, nn * 4, nn * 4, nn * 4, 256,
-                             scan_temp_bytes(nn)};
+// Candidate tiles of 16 points: a multiple of 4 (a step's four tiles), at least two per block of
+// 32 query rows.
+static uint32_t knn_tiles(size_t nn) { return 4 * (uint32_t)((nn + 63) / 64); }
+static int knn_ksteps(int d) { return (d + 3) / 4; }
+
+// Workspace pieces, each starting on a 256-byte boundary: norms (padded to the tiles), candidates,
+// tau, flags, positions, row list, misc words, scan temp, packed operands.
+static size_t knn_layout(size_t nn, int d, size_t *off) {
+    const size_t nt = knn_tiles(nn);
+    const size_t ksp = (size_t)((knn_ksteps(d) + 3) & ~3);
+    const size_t piece[9] = {nt * 16 * 4, nn * KCAND * 4, nn * 4, nn * 4, nn * 4, nn * 4, 256,
+                             scan_temp_bytes(nn), nt * 64 * ksp * 4};
     size_t at = 0;
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < 9; i++) {
         if (off) off[i] = at;
         at += (piece[i] + 255) & ~(size_t)255;
     }
@@ -387,7 +434,21 @@ static size_t knn_layout(size_t nn, size_t *off) {
 
 size_t knn_temp_bytes(int64_t n, int d, int k) {
     if (d > 4 * MKS || k > KCAND || n <= KCAND) return 0;
-    return knn_layout((size_t)n, nullptr);
+    return knn_layout((size_t)n, d, nullptr);
+}
+
+template <int KS>
+static void launch_cand(const float *xp, const float *nrm, int n, uint32_t ntiles, uint32_t *cand,
+                        float *tau, hipStream_t s) {
+    hipLaunchKernelGGL(k_knn_cand<KS>, dim3(ceil_div((uint64_t)n, MQ)), dim3(KNN_THREADS), 0, s,
+                       xp, nrm, n, ntiles, cand, tau);
+}
+
+template <int... KS>
+static void cand_by_ksteps(int ks, std::integer_sequence<int, KS...>, const float *xp,
+                           const float *nrm, int n, uint32_t ntiles, uint32_t *cand, float *tau,
+                           hipStream_t s) {
+    ((ks == KS + 1 ? launch_cand<KS + 1>(xp, nrm, n, ntiles, cand, tau, s) : void()), ...);
 }
 
 bool knn_fast(const float *x, int64_t n, int d, int k, uint32_t *idx, float *dist, void *temp,
@@ -400,8 +461,8 @@ bool knn_fast(const float *x, int64_t n, int d, int k, uint32_t *idx, float *dis
     }
     const size_t nn = (size_t)n;
     char *p = (char *)temp;
-    size_t off[8];
-    knn_layout(nn, off);
+    size_t off[9];
+    knn_layout(nn, d, off);
     float *nrm = (float *)(p + off[0]);
     uint32_t *cand = (uint32_t *)(p + off[1]);
     float *tau = (float *)(p + off[2]);
@@ -410,11 +471,17 @@ bool knn_fast(const float *x, int64_t n, int d, int k, uint32_t *idx, float *dis
     uint32_t *list = (uint32_t *)(p + off[5]);
     uint32_t *misc = (uint32_t *)(p + off[6]);  // [0] max norm bits, [1] uncertain rows
     void *stemp = p + off[7];
+    float *xp = (float *)(p + off[8]);
+    const uint32_t ntiles = knn_tiles(nn);
+    const int ks = knn_ksteps(d);
     BHT_HIP(hipMemsetAsync(misc, 0, 256, s));
     const unsigned nb = ceil_div(nn, 256);
     hipLaunchKernelGGL(k_sqnorm, dim3(nb), dim3(256), 0, s, x, (int)n, d, nrm, misc);
-    hipLaunchKernelGGL(k_knn_cand, dim3(ceil_div(nn, MQ)), dim3(KNN_THREADS), 0, s, x, nrm,
-                       (int)n, d, cand, tau);
+    hipLaunchKernelGGL(k_knn_pack, dim3(4096), dim3(256), 0, s, x, (int)n, d, (ks + 3) & ~3,
+                       ntiles, xp, nrm);
+    cand_by_ksteps(ks, std::make_integer_sequence<int, MKS>(), xp, nrm, (int)n, ntiles, cand,
+                   tau, s);
+    BHT_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_knn_rerank, dim3(ceil_div(nn, 4)), dim3(256), 0, s, x, nrm, misc,
                        (int)n, d, k, cand, tau, idx, dist, flag);
     scan_exclusive<uint32_t>(flag, pos, nn, stemp, misc + 1, s);
