@@ -287,6 +287,11 @@ __global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *xp, const
         const uint32_t nt = st + 1 < nsteps ? ct + 4 : ct;
         load_operands<KS>(xp, nt, lane, bn);
         const float cnn = nrm[nt * 16 + (lane & 15)];
+        // the rows' thresholds (settled by the last compaction phase) load beside the MFMAs
+        float4 tq[2];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+            tq[rt] = *reinterpret_cast<const float4 *>(&tauf[rt * 16 + 4 * (lane >> 4)]);
         v4f acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int kb = 0; kb < KS; kb++) {
@@ -299,8 +304,7 @@ __global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *xp, const
         bool any = false;
 #pragma unroll
         for (int rt = 0; rt < 2; rt++) {
-            const float4 tq = *reinterpret_cast<const float4 *>(&tauf[rt * 16 + 4 * (lane >> 4)]);
-            const float tqv[4] = {tq.x, tq.y, tq.z, tq.w};
+            const float tqv[4] = {tq[rt].x, tq[rt].y, tq[rt].z, tq[rt].w};
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const float dot = rt == 0 ? acc0[r] : acc1[r];
@@ -325,9 +329,15 @@ __global__ void __launch_bounds__(KNN_THREADS) k_knn_cand(const float *xp, const
                 }
         }
         lds_barrier();
-        for (int qi = w; qi < MQ; qi += KNN_THREADS / 64) {
-            const uint32_t c = cnt[qi];
-            if (c <= (uint32_t)(KBUF - MC)) continue;
+        // this wave compacts rows w, w + 4, ..., w + 28: their counts in one read (lane j holds
+        // row w + 4 j), then only the rows that may not take another step's appends
+        const uint32_t cl = lane < MQ / 4 ? cnt[w + 4 * lane] : 0u;
+        uint64_t over = __ballot(cl > (uint32_t)(KBUF - MC));
+        while (over != 0ull) {
+            const int j = __builtin_ctzll(over);
+            over &= over - 1ull;
+            const int qi = w + 4 * j;
+            const uint32_t c = __builtin_amdgcn_readlane(cl, j);
             for (int s = lane; s < KBUF; s += 64)
                 if (s >= (int)c) buf[qi][s] = ~0ull;
             __builtin_amdgcn_wave_barrier();
