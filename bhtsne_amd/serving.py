@@ -1,8 +1,9 @@
 """Request loop: continuous batching of t-SNE jobs on one device.
 
-Requests arrive over time (a Poisson stream in the benchmark).  A preparer thread builds each
-arriving job's affinities (kNN, bandwidth search, symmetrisation) and optimiser state on the job's
-own stream, admitting it only while the context's HBM budget has room for the job's footprint;
+Requests arrive over time (a Poisson stream in the benchmark).  Preparer threads (four by
+default) build arriving jobs' affinities (kNN, bandwidth search, symmetrisation) and optimiser
+state, each job on its own stream, so that the builds of several requests overlap on the device;
+a job is admitted only while the context's HBM budget has room for its footprint;
 the stepping thread joins prepared jobs to the running batch at the next step boundary, advances
 the batch a few epochs at a time (one graph replay per epoch for all members) and hands finished
 jobs back.  Affinity builds of arriving jobs therefore overlap the epochs of running ones on the
@@ -69,9 +70,11 @@ def _collect(job: TSNE, res: Result, want_kl: bool, t0: float):
 
 
 def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: int = 25,
-          hbm_budget: int = 0, want_kl: bool = True) -> List[Result]:
+          hbm_budget: int = 0, want_kl: bool = True, preparers: int = 4) -> List[Result]:
     """Serves the requests with continuous batching; returns one Result per request (times in
-    seconds from the start of the stream)."""
+    seconds from the start of the stream).  `preparers` threads build arriving jobs'
+    affinities concurrently (each job on its own stream), taking the requests in arrival
+    order."""
     ctx = ctx or Context(0, hbm_budget)
     reqs = sorted(requests, key=lambda r: r.arrival)
     results = {r.rid: Result(r.rid, r.x.shape[0], r.d_out, r.arrival) for r in reqs}
@@ -83,9 +86,16 @@ def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: i
     stats = {"step_s": 0.0, "steps": 0}
     t0 = time.perf_counter()
 
+    pending = iter(reqs)
+    take = threading.Lock()
+
     def preparer():
         try:
-            for r in reqs:
+            while True:
+                with take:
+                    r = next(pending, None)
+                if r is None:
+                    break
                 delay = r.arrival - (time.perf_counter() - t0)
                 if delay > 0:
                     time.sleep(delay)
@@ -105,11 +115,14 @@ def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: i
         finally:
             ready.put(None)
 
-    th = threading.Thread(target=preparer, daemon=True)
-    th.start()
+    nprep = max(1, int(preparers))
+    threads = [threading.Thread(target=preparer, daemon=True) for _ in range(nprep)]
+    for th in threads:
+        th.start()
     batch = Batch(ctx)
     live = {}
     feeding = True
+    running_preparers = nprep
     while feeding or len(batch):
         # join whatever has been prepared; block only when nothing is running
         while True:
@@ -118,8 +131,11 @@ def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: i
             except queue.Empty:
                 break
             if item is None:
-                feeding = False
-                break
+                running_preparers -= 1
+                feeding = running_preparers > 0
+                if not feeding:
+                    break
+                continue
             r, job, need = item
             batch.add(job, r.method, r.theta)
             results[r.rid].admitted = time.perf_counter() - t0
@@ -139,7 +155,8 @@ def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: i
                 with lock:
                     reserved["bytes"] -= need
                     lock.notify_all()
-    th.join()
+    for th in threads:
+        th.join()
     if errors:
         raise errors[0]
     batch.close()
