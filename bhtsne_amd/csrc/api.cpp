@@ -461,7 +461,7 @@ void ensure_bh_workspace(bht_job *j) {
     j->bsort_temp.ensure(c, group_sort_temp_bytes(n));
     const size_t node_bytes = j->dim() >= 3 ? (size_t)nd_node_floats((int)j->dim()) * 4 : 16;
     j->bnode.ensure(c, 2 * n * node_bytes);
-    j->bmeta.ensure(c, 2 * n * 4);
+    j->bmeta.ensure(c, 2 * n * 8);
     if (j->dim() >= 5) {
         j->bclo.ensure(c, n * 8);
         j->bchi.ensure(c, n * 8);
@@ -472,7 +472,7 @@ void ensure_bh_workspace(bht_job *j) {
     j->bcount.ensure(c, 16);
     JobDev &d = j->hd;
     d.nodes.node = j->bnode.as<float4>();
-    d.nodes.meta = j->bmeta.as<uint32_t>();
+    d.nodes.meta = j->bmeta.as<uint2>();
     d.nodes.range = j->brange.as<uint32_t>();
     d.nodes.hw2 = j->bhw2.as<float>();
     d.nodes.counts = j->bcount.as<uint32_t>();
@@ -1408,6 +1408,9 @@ void fit(bht_job *j, int method, float theta, bht_epoch_cb cb, void *user, bool 
 void check_dims(const bht_job *j, int method) {
     if (method == FIT_BH && (j->dim() < 2 || j->dim() > 7))
         throw Error(ERR_INVALID, "the Barnes-Hut path supports d_out in 2..7");
+    // the 2-D walk packs a child index (below 2n) in 29 bits
+    if (method == FIT_BH && j->dim() == 2 && j->n >= (1ull << 28))
+        throw Error(ERR_INVALID, "the Barnes-Hut path supports fewer than 2^28 points");
     if (method == FIT_FITSNE && j->dim() != 1 && j->dim() != 2)
         throw Error(ERR_INVALID, "the FIt-SNE path supports d_out in {1, 2}");
 }
@@ -2529,7 +2532,7 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
         b += n * (2 * 8 + 2 * 4) + group_sort_temp_bytes(n);  // Morton sort
         if (D >= 5) b += 3 * n * 8;                            // two-word codes
         const uint64_t node_bytes = D >= 3 && D <= 7 ? (uint64_t)nd_node_floats((int)D) * 4 : 16;
-        b += 2 * n * node_bytes + 2 * n * 4 + 4 * n * 4;       // tree
+        b += 2 * n * node_bytes + 2 * n * 8 + 4 * n * 4;       // tree
         b += pad_tile(n) * 24 + group_sort_temp_bytes(pad_tile(n));  // batch arena
     } else {
         uint32_t maxb = std::min<uint32_t>(std::max<uint32_t>(p.max_boxes, 1), MAX_BOXES);

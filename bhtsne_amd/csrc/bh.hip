@@ -8,8 +8,14 @@
 //     current frontier in node order (children located by binary search of the sorted codes, a
 //     block scan assigns their slots), which yields exactly the reference's node numbering; then
 //     centres of mass bottom-up, round by round, with the reference's f32 summation order.
-//  4. One lane per point walks the tree with an LDS stack in the reference's push/pop order;
-//     lanes take points in Morton order so a wave's walks share their paths.  Z is summed in FP64.
+//  4. One lane per point walks the tree in the reference's visit order; lanes take points in
+//     Morton order so a wave's walks share their paths.  Z is summed in FP64.  The reference pops
+//     node ids from a stack (up to 128 at D = 2); a lane here keeps a stack of child ranges (first
+//     child, children left), packed in a u32: taking the last child left of the top range is the
+//     node the reference pops next.  A range is held per internal node on the current path, whose
+//     levels increase strictly, so at most 33 ranges: 8.4 KB of LDS per wave, and the top range
+//     lives in a register.  An internal node's squared half-width is stored beside its child
+//     count, so a visit issues its two loads at once.
 #include <cfloat>
 
 #include "internal.hpp"
@@ -20,7 +26,7 @@ constexpr uint32_t SENTINEL = 0xffffffffu;
 constexpr int BITS = 32;
 constexpr int BUILD_THREADS = 1024;
 constexpr int TRAV_THREADS = 64;
-constexpr int STACK = 112;  // >= 4 + 3 * 33
+constexpr int STACK = BITS + 1;  // child ranges per lane
 constexpr int MAX_ROUNDS = 64;
 
 __device__ __forceinline__ uint64_t part1by1(uint64_t x) {
@@ -106,6 +112,7 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
                                                             const uint32_t *map) {
     __shared__ uint32_t sbuf[BUILD_THREADS / 64];
     __shared__ uint32_t round_start[MAX_ROUNDS + 2];
+    __shared__ float shw2[BITS + 1];
     const BlockCtx bc = block_ctx(map, jobs, F_ONE);
     const JobDev &J = jobs[bc.job];
     const float *y = job_y(J);
@@ -123,9 +130,10 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
         for (int level = 0; level <= BITS; level++) {
             float hw = max_extent / (float)(1ull << (level + 1));
             t.hw2[level] = hw * hw;
+            shw2[level] = hw * hw;
         }
         t.node[0] = make_float4(0.f, 0.f, __uint_as_float((uint32_t)n), __uint_as_float(SENTINEL));
-        t.meta[0] = (uint32_t)BITS << 8;
+        t.meta[0] = make_uint2((uint32_t)BITS << 8, 0u);
         t.range[0] = 0;
         t.range[1] = (uint32_t)n;
         round_start[0] = 0;
@@ -169,13 +177,13 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
                 float4 nd = t.node[i];
                 nd.w = __uint_as_float(first);
                 t.node[i] = nd;
-                t.meta[i] = nchild | ((uint32_t)level << 8);
+                t.meta[i] = make_uint2(nchild | ((uint32_t)level << 8), __float_as_uint(shw2[level]));
                 uint32_t c = first;
                 for (int g = 0; g < 4; g++) {
                     if (b[g + 1] <= b[g]) continue;
                     t.node[c] = make_float4(0.f, 0.f, __uint_as_float(b[g + 1] - b[g]),
                                             __uint_as_float(SENTINEL));
-                    t.meta[c] = (uint32_t)BITS << 8;
+                    t.meta[c] = make_uint2((uint32_t)BITS << 8, 0u);
                     t.range[2 * c] = b[g];
                     t.range[2 * c + 1] = b[g + 1];
                     c++;
@@ -203,7 +211,7 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
                     cy += p.y;
                 }
             } else {
-                const uint32_t cc = t.meta[i] & 0xffu;
+                const uint32_t cc = t.meta[i].x & 0xffu;
                 for (uint32_t ch = fc; ch < fc + cc; ch++) {
                     float4 cn = t.node[ch];
                     const float w = (float)__float_as_uint(cn.z);
@@ -238,12 +246,19 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const JobDev *jobs
         const uint32_t i = sidx[s];
         const float2 qp = reinterpret_cast<const float2 *>(y)[i];
         float fx = 0.0f, fy = 0.0f, q_sum = 0.0f;
-        int top = 0;
-        stack[top++][lane] = 0;
-        while (top > 0) {
-            top--;
-            const uint32_t id = stack[top][lane];
+        // the top range, first child << 3 | children left (at most 4): the root as a range of one
+        uint32_t cur = 1u;
+        int below = 0;  // ranges under it, in LDS
+        bool have = true;
+        while (have) {
+            const uint32_t id = (cur >> 3) + (cur & 7u) - 1u;
+            cur -= 1u;
+            if ((cur & 7u) == 0u) {
+                if (below > 0) cur = stack[--below][lane];
+                else have = false;
+            }
             const float4 nd = t.node[id];
+            const uint2 m = t.meta[id];
             const float dx = qp.x - nd.x, dy = qp.y - nd.y;
             float distance = 0.0f;
             distance += dx * dx;
@@ -251,13 +266,12 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const JobDev *jobs
             const uint32_t fc = __float_as_uint(nd.w);
             if (fc == SENTINEL) {
                 if (distance == 0.0f) continue;  // the query's own leaf
-            } else {
-                const uint32_t m = t.meta[id];
-                if (t.hw2[m >> 8] >= theta_sq * distance) {
-                    const uint32_t cc = m & 0xffu;
-                    for (uint32_t c = 0; c < cc; c++) stack[top++][lane] = fc + c;
-                    continue;
-                }
+            } else if (__uint_as_float(m.y) >= theta_sq * distance) {
+                // descend: the siblings left wait under the children
+                if (have && below < STACK) stack[below++][lane] = cur;
+                cur = (fc << 3) | (m.x & 0xffu);
+                have = true;
+                continue;
             }
             const float inverse = 1.0f / (1.0f + distance);
             float magnitude = (float)__float_as_uint(nd.z) * inverse;

@@ -311,6 +311,7 @@ __global__ void __launch_bounds__(ND_BUILD_THREADS) k_nd_build(const JobDev *job
     constexpr int BITS = Nd<D>::BITS;
     __shared__ uint32_t sbuf[ND_BUILD_THREADS / 64];
     __shared__ uint32_t round_start[ND_MAX_ROUNDS + 2];
+    __shared__ float shw2[BITS + 1];
     const BlockCtx bc = block_ctx(map, jobs, F_ONE);
     const JobDev &J = jobs[bc.job];
     const float *y = job_y(J);
@@ -326,12 +327,13 @@ __global__ void __launch_bounds__(ND_BUILD_THREADS) k_nd_build(const JobDev *job
         for (int level = 0; level <= BITS; level++) {
             const float hw = max_extent / (float)(1ull << (level + 1));
             t.hw2[level] = hw * hw;
+            shw2[level] = hw * hw;
         }
         float *r = node_ptr<D>(t.node, 0);
         for (int a = 0; a < D; a++) r[a] = 0.0f;
         r[D] = __uint_as_float((uint32_t)n);
         r[D + 1] = __uint_as_float(SENTINEL);
-        t.meta[0] = (uint32_t)BITS << 8;
+        t.meta[0] = make_uint2((uint32_t)BITS << 8, 0u);
         t.range[0] = 0;
         t.range[1] = (uint32_t)n;
         round_start[0] = 0;
@@ -377,7 +379,7 @@ __global__ void __launch_bounds__(ND_BUILD_THREADS) k_nd_build(const JobDev *job
             if (internal) {
                 const uint32_t first = nn + off;
                 node_ptr<D>(t.node, i)[D + 1] = __uint_as_float(first);
-                t.meta[i] = nchild | ((uint32_t)level << 8);
+                t.meta[i] = make_uint2(nchild | ((uint32_t)level << 8), __float_as_uint(shw2[level]));
                 uint32_t c = first;
                 for (uint32_t p = s; p < e; c++) {
                     const uint32_t q = codes.upper(p + 1, e, shift, codes.digit(p, shift));
@@ -385,7 +387,7 @@ __global__ void __launch_bounds__(ND_BUILD_THREADS) k_nd_build(const JobDev *job
                     for (int a = 0; a < D; a++) cr[a] = 0.0f;
                     cr[D] = __uint_as_float(q - p);
                     cr[D + 1] = __uint_as_float(SENTINEL);
-                    t.meta[c] = (uint32_t)BITS << 8;
+                    t.meta[c] = make_uint2((uint32_t)BITS << 8, 0u);
                     t.range[2 * c] = p;
                     t.range[2 * c + 1] = q;
                     p = q;
@@ -414,7 +416,7 @@ __global__ void __launch_bounds__(ND_BUILD_THREADS) k_nd_build(const JobDev *job
                     for (int a = 0; a < D; a++) c[a] += pt[a];
                 }
             } else {
-                const uint32_t cc = t.meta[i] & 0xffu;
+                const uint32_t cc = t.meta[i].x & 0xffu;
                 for (uint32_t ch = fc; ch < fc + cc; ch++) {
                     const float *cn = node_ptr<D>(t.node, ch);
                     const float w = (float)__float_as_uint(cn[D]);
@@ -482,9 +484,9 @@ __global__ void __launch_bounds__(ND_TRAV_THREADS) k_nd_traverse(const JobDev *j
             if (fc == SENTINEL) {
                 if (distance == 0.0f) continue;  // the query's own leaf
             } else {
-                const uint32_t m = t.meta[id];
-                if (t.hw2[m >> 8] >= theta_sq * distance) {
-                    if (top < STACK) stack[top++][lane] = make_uint2(fc, m & 0xffu);
+                const uint2 m = t.meta[id];
+                if (__uint_as_float(m.y) >= theta_sq * distance) {
+                    if (top < STACK) stack[top++][lane] = make_uint2(fc, m.x & 0xffu);
                     continue;
                 }
             }

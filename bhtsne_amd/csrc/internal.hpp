@@ -60,7 +60,8 @@ struct TwiddleSet {
 struct BhNodes {
     float4 *node;     // [2n] com.x, com.y, count (u32 bits), first_child (u32 bits, ~0 = leaf);
                       // D >= 3: nd_node_floats(D) floats per node, com[D], count, first_child
-    uint32_t *meta;   // [2n] child_count | level << 8
+    uint2 *meta;      // [2n] child_count | level << 8; the squared half-width of an internal
+                      // node's level (f32 bits), read beside the node by the walk
     uint32_t *range;  // [4n] start, end in the sorted order
     float *hw2;       // [33] squared half-width per level
     uint32_t *counts; // [1] node count
