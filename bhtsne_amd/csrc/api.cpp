@@ -1030,15 +1030,20 @@ void enqueue_epoch(bht_job *j, const LaunchSet &L0, bool fit, bool fork, int que
     hipStream_t rs = fork ? j->side : j->stream;
     // A FIt-SNE epoch after an update leaves the recentring to the readers of the embedding (one
     // head kernel: the shift and the grid plan); otherwise the embedding is centred in place.
+    // A forked one runs its head on the repulsion's stream, the repulsion being the only reader
+    // of the grid plan: the attraction starts at once and derives the shift and the epoch index
+    // from the last update's sums itself (the same values, computed the same way).
     const bool lazy = fit && j->lazy_ready;
-    if (lazy) epoch_head(L, j->stream);
+    const bool head_aside = fork && lazy;
+    if (lazy && !head_aside) epoch_head(L, j->stream);
     if (fork) {
         if (!lazy) center_points(L, j->stream);
         BHT_HIP(hipEventRecord(j->ev_fork, j->stream));
         BHT_HIP(hipStreamWaitEvent(j->side, j->ev_fork, 0));
         // the repulsion reads y, the scalars and the bounds' partials written on `stream`
         l2_sync(j->side);
-        if (!lazy) plan_grid(L, fit, j->side);
+        if (head_aside) epoch_head(L, j->side);
+        else if (!lazy) plan_grid(L, fit, j->side);
     } else if (!lazy) {
         center_plan(L, fit, j->stream);
     }
@@ -1046,7 +1051,7 @@ void enqueue_epoch(bht_job *j, const LaunchSet &L0, bool fit, bool fork, int que
     if (fork) {
         // the attraction is enqueued first: the host takes a while to enqueue the repulsion's
         // chain of kernels, and the attraction must not wait for that
-        attract_tiled(L, nullptr, true, j->stream, queue_blocks);
+        attract_tiled(L, nullptr, true, j->stream, queue_blocks, head_aside);
         j->mark(j->stream, 2);
         if (fit) fitsne_repulsion(L, rs);
         else if (j->n > 0) bh_repulsion(L, rs);

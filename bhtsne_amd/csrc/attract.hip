@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(UPD_THREADS) k_update(const JobDev *jobs, cons
         J.mpart[2 * bc.lb] = s0;
         J.mpart[2 * bc.lb + 1] = s1;
         if (bc.lb == 0) {
-            J.scal->ctr[CTR_ATT] = 0;  // the attraction's work queue, for the next epoch
+            J.scal->att_queue = 0;  // the attraction's work queue, for the next epoch
             J.scal->epoch_next = epoch + 1;
             J.scal->mean_pending = 1;
         }

@@ -113,8 +113,6 @@ constexpr int FIN_THREADS = 1024;  // up to 16384 attraction blocks' partials pe
 // embedding takes fl(y - ysub), the value k_center would have stored).
 __device__ void mean_body(const JobDev &J, int lazy, double *red) {
     EpochScalars *scal = J.scal;
-    const double *mpart = J.mpart;
-    const int nparts = J.nblk[F_PT], n = J.n;  // partials of k_update
     const int pending = scal->mean_pending;
     if (!pending) {
         if (threadIdx.x == 0) {
@@ -125,15 +123,9 @@ __device__ void mean_body(const JobDev &J, int lazy, double *red) {
         }
         return;
     }
-    double t0 = 0.0, t1 = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-        t0 += mpart[2 * b];
-        t1 += mpart[2 * b + 1];
-    }
-    t0 = block_sum_f64(t0, red);
-    t1 = block_sum_f64(t1, red);
+    float m0, m1;
+    pending_mean2(J, m0, m1);
     if (threadIdx.x == 0) {
-        const float m0 = (float)(t0 / (double)n), m1 = (float)(t1 / (double)n);
         scal->mean[0] = m0;
         scal->mean[1] = m1;
         scal->ysub[0] = lazy ? m0 : 0.0f;

@@ -40,10 +40,12 @@ struct alignas(128) EpochScalars {
     int mean_pending;      // the last update left recentring partials to be applied
     int epoch_next;        // epoch index after the last update
     unsigned ctr[8];       // arrival counters of the single-pass reductions
+    // -- the attraction's work queue (row blocks taken so far), written by the attraction alone
+    // while the repulsion's head writes the group above on the other stream; reset by the update
+    alignas(128) unsigned att_queue;
 };
 
-enum CtrSlot { CTR_CENTER = 0, CTR_GATHER = 1, CTR_UPDATE = 2, CTR_PSUM = 3, CTR_KL = 4, CTR_BH = 5,
-               CTR_ATT = 6 };
+enum CtrSlot { CTR_CENTER = 0, CTR_GATHER = 1, CTR_UPDATE = 2, CTR_PSUM = 3, CTR_KL = 4, CTR_BH = 5 };
 
 // Per-length twiddle tables exp(-2 pi i j / L), one per distinct length reachable from
 // boxes = 1..MAX_BOXES (context-wide, built once).
@@ -153,6 +155,27 @@ __device__ __forceinline__ BlockCtx block_ctx(const uint32_t *map, const JobDev 
 }
 
 __device__ __forceinline__ int job_epoch(const JobDev &J) { return J.scal->epoch; }
+
+// The recentring shift the last update left pending (zero_mean, src/tsne/mod.rs:534-568): the
+// mean of its per-block FP64 sums (mpart, two per block).  Evaluated by a whole wave: lane-strided
+// sums, then a butterfly whose every lane holds the same bits (FP64 addition commutes), so every
+// kernel that evaluates it, whatever its block size, gets the same shift.
+__device__ __forceinline__ void pending_mean2(const JobDev &J, float &m0, float &m1) {
+    const int lane = threadIdx.x & 63;
+    const int nparts = J.nblk[F_PT];
+    double t0 = 0.0, t1 = 0.0;
+    for (int b = lane; b < nparts; b += 64) {
+        t0 += J.mpart[2 * b];
+        t1 += J.mpart[2 * b + 1];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        t0 += __shfl_xor(t0, off, 64);
+        t1 += __shfl_xor(t1, off, 64);
+    }
+    m0 = (float)(t0 / (double)J.n);
+    m1 = (float)(t1 / (double)J.n);
+}
 __device__ __forceinline__ float *job_y(const JobDev &J) { return J.ybuf[J.scal->epoch & 1]; }
 
 constexpr int FFT_FB_BLOCKS = 64;  // blocks of an FFT launch that transform lines too long for LDS
@@ -305,8 +328,11 @@ void nd_update(const LaunchSet &L, hipStream_t s);
 // queue_blocks > 0 (single job with its repulsion running concurrently on another stream): that
 // many blocks take the row blocks from a work queue, leaving the rest of the device to the
 // repulsion; 0: one block per row block.  The sums are the same either way.
+// self_head (a lazily centred epoch whose head runs on the repulsion's stream beside it): the
+// kernel takes the epoch index from epoch_next and the shift from pending_mean2 itself, the values
+// the head derives from the same data.
 void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s,
-                   int queue_blocks = 0);
+                   int queue_blocks = 0, bool self_head = false);
 int attract_default_queue_blocks();
 // The gradient step from fattr, frep and Z (attract.hip).
 void update_points(const LaunchSet &L, hipStream_t s);

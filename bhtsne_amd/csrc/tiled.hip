@@ -444,7 +444,8 @@ __device__ __forceinline__ uint32_t wave_incl_add(uint32_t v) {
 template <int D, bool EPOCH>
 __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
                                                              const uint32_t *map,
-                                                             float *pos_out, int queued) {
+                                                             float *pos_out, int queued,
+                                                             int self_head) {
     // static LDS: the arrays sit at fixed addresses, which fold into the instructions' offsets
     __shared__ __align__(16) float win[WIN_FLOATS];
     __shared__ __align__(16) float yrow[TILE_ROWS_MAX * D];
@@ -454,7 +455,9 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
     const JobDev &J = jobs[bc.job];
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int epoch = __builtin_amdgcn_readfirstlane(J.scal->epoch);
+    // self_head: the epoch's head (the recentring mean, the epoch index) runs beside this kernel
+    // on the repulsion's stream, so the kernel derives both from the last update's data itself
+    const int epoch = __builtin_amdgcn_readfirstlane(self_head ? J.scal->epoch_next : J.scal->epoch);
     const uint32_t n = (uint32_t)J.n;
     const uint32_t R = J.tile_rows, W = J.tile_points;
     const uint32_t nnz = J.tile_nnz;
@@ -469,14 +472,20 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
     // read the rescaled values rescales them in place as it reads
     const bool stop_now = EPOCH && J.stop_lying_epoch != 0 && epoch == J.stop_lying_epoch + 1;
     const float inv_exag = J.inv_exag;
-    const float ys0 = J.scal->ysub[0], ys1 = J.scal->ysub[1];
+    float ys0, ys1;
+    if (self_head) {
+        pending_mean2(J, ys0, ys1);
+    } else {
+        ys0 = J.scal->ysub[0];
+        ys1 = J.scal->ysub[1];
+    }
 
     // queued: the blocks take row blocks from the job's work queue (reset by the update kernel);
     // otherwise each block has its own row block
     const uint32_t nrb = (n + R - 1) / R;
     uint32_t lb = (uint32_t)bc.lb;
     if (queued) {
-        if (threadIdx.x == 0) next_lb = atomicAdd(&J.scal->ctr[CTR_ATT], 1u);
+        if (threadIdx.x == 0) next_lb = atomicAdd(&J.scal->att_queue, 1u);
         __syncthreads();
         lb = next_lb;
     }
@@ -643,7 +652,7 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
             out[(size_t)row0 * D + k] = lds_view(acc)[D == 2 ? k : 2 * k];
         __syncthreads();  // the sums are out before the next row block's rows overwrite them
         if (queued) {
-            if (threadIdx.x == 0) next_lb = atomicAdd(&J.scal->ctr[CTR_ATT], 1u);
+            if (threadIdx.x == 0) next_lb = atomicAdd(&J.scal->att_queue, 1u);
             __syncthreads();
             lb = next_lb;
         } else {
@@ -790,27 +799,29 @@ uint64_t tiled_build(const uint32_t *rows, const uint32_t *cols, const float *va
 
 int attract_default_queue_blocks() { return ATT_QUEUE_BLOCKS; }
 
-void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s, int queue_blocks) {
+void attract_tiled(const LaunchSet &L, float *pos, bool epoch, hipStream_t s, int queue_blocks,
+                   bool self_head) {
     if (L.dim >= 3) {  // plain CSR rows (Barnes-Hut, D = 3..7)
         nd_attract(L, pos, epoch, s);
         return;
     }
     const int queued = epoch && queue_blocks > 0 && L.map[F_ATT] == nullptr ? 1 : 0;
+    const int self = epoch && self_head && L.map[F_ATT] == nullptr ? 1 : 0;
     const dim3 g(queued ? std::min(L.nb[F_ATT], queue_blocks) : L.nb[F_ATT]);
     if (L.dim == 2) {
         if (epoch)
             hipLaunchKernelGGL((k_attract_tiled<2, true>), g, dim3(T_THREADS), 0, s, L.jobs,
-                               L.map[F_ATT], nullptr, queued);
+                               L.map[F_ATT], nullptr, queued, self);
         else
             hipLaunchKernelGGL((k_attract_tiled<2, false>), g, dim3(T_THREADS), 0, s, L.jobs,
-                               L.map[F_ATT], pos, 0);
+                               L.map[F_ATT], pos, 0, 0);
     } else {
         if (epoch)
             hipLaunchKernelGGL((k_attract_tiled<1, true>), g, dim3(T_THREADS), 0, s, L.jobs,
-                               L.map[F_ATT], nullptr, queued);
+                               L.map[F_ATT], nullptr, queued, self);
         else
             hipLaunchKernelGGL((k_attract_tiled<1, false>), g, dim3(T_THREADS), 0, s, L.jobs,
-                               L.map[F_ATT], pos, 0);
+                               L.map[F_ATT], pos, 0, 0);
     }
     BHT_LAUNCH_CHECK();
 }
