@@ -12,7 +12,9 @@ not depend on any random generator.  Every array is stored in one numpy archive 
 * y0 (200 x 2): a state; fit_pos / fit_neg / fit_invz and bh_pos / bh_neg / bh_invz: forces of
   that state (FIt-SNE, 10-box floor; Barnes-Hut theta 0.5);
 * epoch1_y / epoch1_uy / epoch1_gains: one FIt-SNE epoch from (y0, 0, 1);
-* kl_fit / kl_bh: (reference formula, true KL) at y0.
+* kl_fit / kl_bh: (reference formula, true KL) at y0;
+* y0_d3 .. y0_d7 (200 x D) and bh_pos_dD / bh_neg_dD / bh_invz_dD: Barnes-Hut forces (theta 0.5)
+  of a D-dimensional state over the same P, for D = 3..7.
 """
 import os
 import sys
@@ -52,6 +54,11 @@ def main():
         g[f"{name}_pos"], g[f"{name}_neg"], g[f"{name}_invz"] = pos, neg, np.float32(inv)
         g[f"kl_{name}"] = np.array(oracle.kl_divergence(y0, rows, cols, pn, method=method),
                                    np.float64)
+    for D in range(3, 8):
+        yd = (lcg_samples(200, D, 99 + D) * np.float32(4.0)).astype(np.float32)
+        pos, neg, inv = oracle.forces(yd, rows, cols, pn, method="bh", theta=0.5)
+        g[f"y0_d{D}"] = yd
+        g[f"bh_pos_d{D}"], g[f"bh_neg_d{D}"], g[f"bh_invz_d{D}"] = pos, neg, np.float32(inv)
     y, uy, gains = y0.copy(), np.zeros_like(y0), np.ones_like(y0)
     oracle.run_epochs(y, uy, gains, rows, cols, pn.copy(),
                       learning_rate=oracle.auto_learning_rate(200, 12.0), e0=0, e1=1)

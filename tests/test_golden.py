@@ -47,6 +47,10 @@ def test_oracle_reproduces_fixtures():
         pos, neg, inv = oracle.forces(G["y0"], rows, cols, pn, method=method, theta=0.5)
         assert np.array_equal(pos, G[f"{name}_pos"]) and np.array_equal(neg, G[f"{name}_neg"])
         assert np.float32(inv) == G[f"{name}_invz"]
+    for D in range(3, 8):
+        pos, neg, inv = oracle.forces(G[f"y0_d{D}"], rows, cols, pn, method="bh", theta=0.5)
+        assert np.array_equal(pos, G[f"bh_pos_d{D}"]) and np.array_equal(neg, G[f"bh_neg_d{D}"])
+        assert np.float32(inv) == G[f"bh_invz_d{D}"]
 
 
 # ---- the device against the fixtures --------------------------------------------------------
@@ -88,6 +92,18 @@ def test_device_forces_match_fixtures(bt, name, method):
     print(f"{method}: attraction {_rel(pos, rp):.2e} repulsion {_rel(neg, rn):.2e} "
           f"gradient {e_grad:.2e}", flush=True)
     assert _rel(pos, rp) < 1e-5 and e_grad < 1e-5 and abs(inv - ri) / ri < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D", [3, 4, 5, 6, 7])
+def test_device_bh_forces_in_d_dimensions_match_fixtures(bt, D):
+    t = bt.TSNE(G["knn_x"], d_out=D).perplexity(4.0)
+    t.with_affinities(bt.SparseAffinities(G["sym_rows"], G["sym_cols"], G["sym_vals"], 4.0))
+    t.initial_embedding(G[f"y0_d{D}"]).prepare("bh", 0.5)
+    pos, neg, inv = t.forces()
+    rp, rn, ri = G[f"bh_pos_d{D}"], G[f"bh_neg_d{D}"], float(G[f"bh_invz_d{D}"])
+    print(f"D={D}: attraction {_rel(pos, rp):.2e} repulsion {_rel(neg, rn):.2e}", flush=True)
+    assert _rel(pos, rp) < 1e-6 and _rel(neg, rn) < 1e-6 and abs(inv - ri) / ri < 1e-5
 
 
 @pytest.mark.gpu
