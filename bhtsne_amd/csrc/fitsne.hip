@@ -44,9 +44,14 @@ void job_block_counts(JobDev &J, bool batched) {
     J.nblk[F_ATT] = J.dim >= 3 ? nd_attract_blocks(n) : tiled_blocks(n);
     J.nblk[F_TRAV] = traverse_blocks(n);
     J.nblk[F_LINE1] = 2;
-    // transform lines per job: 2 * nodes (rows) and 2 * L (columns), a few hundred for batch-sized
-    // jobs; a batch gives each job up to 512 blocks so that most lines get a block of their own
-    const int cap = batched ? 512 : 2048;
+    // transform lines per job: 2 * nodes (rows) and 2 * L (columns).  A job alone takes up to
+    // 2048 blocks.  In a batch the grid a member reaches grows with its size, and the launch's
+    // blocks each ask for the group's whole line buffer in LDS, so a member takes 8 blocks per
+    // 1024 points (16 to 512, a multiple of 8 for the column passes' XCD mapping) and its blocks
+    // stride over its lines: 256 jobs of 500-50,000 points launched 65,536 mostly idle blocks
+    // per pass with 512 each.
+    const int per_size = (int)std::min<int64_t>(512, std::max<int64_t>(16, 8 * (((int64_t)n + 1023) / 1024)));
+    const int cap = batched ? per_size : 2048;
     const int64_t nodes_total =
         J.dim == 2 ? (int64_t)J.nodes_cap * J.nodes_cap : (int64_t)J.nodes_cap;
     J.nblk[F_ZERO] = (int)std::max<int64_t>(1, std::min<int64_t>(batched ? 64 : 1024,
