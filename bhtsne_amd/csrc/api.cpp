@@ -1171,7 +1171,7 @@ TuneChoice autotune(bht_job *j, const LaunchSet &L, bool fit) {
     if (sl != 0 && j->epoch <= sl + 1 && sl + 1 < trials_end) return best;
     std::vector<TuneChoice> cands;
     const int nblk = L.nb[F_ATT];
-    for (int q : {192, 256, 320, 384, 448, 512})
+    for (int q : {192, 224, 256, 288, 320, 384, 448, 512})
         if (q < nblk) cands.push_back(TuneChoice{1, q});
     cands.push_back(TuneChoice{1, 0});  // forked, one block per row block
     cands.push_back(TuneChoice{0, 0});  // in order on one stream

@@ -8,6 +8,8 @@
 
 namespace bht {
 
+typedef float upd_f2 __attribute__((ext_vector_type(2)));
+
 constexpr int UPD_THREADS = 256;  // k_update runs over the point-parallel (F_PT) block partition
 
 struct StepCtx {
@@ -35,6 +37,15 @@ struct RowState {
 template <int D>
 __device__ __forceinline__ RowState<D> load_row_state(const StepMem &M, int i) {
     RowState<D> r;
+    if (D == 2) {  // one 8-byte load per array
+        const upd_f2 f = reinterpret_cast<BHT_GLOBAL const upd_f2 *>(M.frep)[i];
+        const upd_f2 u = reinterpret_cast<BHT_GLOBAL const upd_f2 *>(M.uy)[i];
+        const upd_f2 g = reinterpret_cast<BHT_GLOBAL const upd_f2 *>(M.gains)[i];
+        r.frep[0] = f.x; r.frep[D - 1] = f.y;
+        r.uy[0] = u.x; r.uy[D - 1] = u.y;
+        r.gain[0] = g.x; r.gain[D - 1] = g.y;
+        return r;
+    }
 #pragma unroll
     for (int d = 0; d < D; d++) {
         const size_t k = (size_t)i * D + d;
@@ -52,9 +63,9 @@ __device__ __forceinline__ void apply_update(const StepMem &M, const StepCtx &c,
                                              const float *acc, const float *yi,
                                              const RowState<D> &st, double &m0, double &m1,
                                              float *ynew) {
+    float ngain[D], nuy[D];
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        const size_t k = (size_t)i * D + d;
         const float gradient = acc[d] - st.frep[d] * c.inv_z;
         float uy = st.uy[d];
         float gain = st.gain[d];
@@ -62,12 +73,24 @@ __device__ __forceinline__ void apply_update(const StepMem &M, const StepCtx &c,
         if (gain < 0.01f) gain = 0.01f;
         uy = c.momentum * uy - c.learning_rate * gain * gradient;
         const float yn = yi[d] + uy;
-        M.gains[k] = gain;
-        M.uy[k] = uy;
-        M.y_next[k] = yn;
+        ngain[d] = gain;
+        nuy[d] = uy;
         ynew[d] = yn;
         if (d == 0) m0 += (double)yn;
         else m1 += (double)yn;
+    }
+    if (D == 2) {  // one 8-byte store per array
+        reinterpret_cast<BHT_GLOBAL upd_f2 *>(M.gains)[i] = upd_f2{ngain[0], ngain[D - 1]};
+        reinterpret_cast<BHT_GLOBAL upd_f2 *>(M.uy)[i] = upd_f2{nuy[0], nuy[D - 1]};
+        reinterpret_cast<BHT_GLOBAL upd_f2 *>(M.y_next)[i] = upd_f2{ynew[0], ynew[D - 1]};
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const size_t k = (size_t)i * D + d;
+            M.gains[k] = ngain[d];
+            M.uy[k] = nuy[d];
+            M.y_next[k] = ynew[d];
+        }
     }
 }
 
@@ -94,10 +117,19 @@ __global__ void __launch_bounds__(UPD_THREADS) k_update(const JobDev *jobs, cons
     const int stride = bc.nb * UPD_THREADS;
     for (int i = bc.lb * UPD_THREADS + threadIdx.x; i < J.n; i += stride) {
         float acc[2] = {0.0f, 0.0f}, yi[2] = {0.0f, 0.0f};
+        if (D == 2) {
+            const upd_f2 a = reinterpret_cast<BHT_GLOBAL const upd_f2 *>(fattr)[i];
+            const upd_f2 v = reinterpret_cast<BHT_GLOBAL const upd_f2 *>(y)[i];
+            acc[0] = a.x;
+            acc[1] = a.y;
+            yi[0] = v.x - sh[0];
+            yi[1] = v.y - sh[1];
+        } else {
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            acc[d] = fattr[(size_t)i * D + d];
-            yi[d] = y[(size_t)i * D + d] - sh[d];
+            for (int d = 0; d < D; d++) {
+                acc[d] = fattr[(size_t)i * D + d];
+                yi[d] = y[(size_t)i * D + d] - sh[d];
+            }
         }
         float yn[2];
         apply_update<D>(M, sc, i, acc, yi, load_row_state<D>(M, i), m0, m1, yn);
