@@ -297,6 +297,7 @@ struct bht_job {
     // Barnes-Hut workspace
     DevBuf codes, codes_alt, bidx, bidx_alt, bsort_temp, bnode, bmeta, brange, bhw2, bcount;
     DevBuf bclo, bchi, bslo;  // two-word Morton codes (D >= 5): low / high words, sorted lows
+    DevBuf bdefer;            // D = 2: roots of the subtrees built by several workgroups
     // the job's entry of the device job table (host copy + device copy)
     JobDev hd{};
     DevBuf jd;
@@ -459,23 +460,27 @@ void ensure_bh_workspace(bht_job *j) {
     j->bidx.ensure(c, n * 4);
     j->bidx_alt.ensure(c, n * 4);
     j->bsort_temp.ensure(c, group_sort_temp_bytes(n));
+    // D = 2: the top nodes in [0, 2n), each deferred subtree in a region of its own in [2n, 4n)
     const size_t node_bytes = j->dim() >= 3 ? (size_t)nd_node_floats((int)j->dim()) * 4 : 16;
-    j->bnode.ensure(c, 2 * n * node_bytes);
-    j->bmeta.ensure(c, 2 * n * 8);
+    const size_t nodes = (j->dim() == 2 ? 4 : 2) * n;
+    j->bnode.ensure(c, nodes * node_bytes);
+    j->bmeta.ensure(c, nodes * 8);
+    if (j->dim() == 2) j->bdefer.ensure(c, n * 4);
     if (j->dim() >= 5) {
         j->bclo.ensure(c, n * 8);
         j->bchi.ensure(c, n * 8);
         j->bslo.ensure(c, n * 8);
     }
-    j->brange.ensure(c, 4 * n * 4);
+    j->brange.ensure(c, 2 * nodes * 4);
     j->bhw2.ensure(c, 64 * 4);
-    j->bcount.ensure(c, 16);
+    j->bcount.ensure(c, 128 * 4);
     JobDev &d = j->hd;
     d.nodes.node = j->bnode.as<float4>();
     d.nodes.meta = j->bmeta.as<uint2>();
     d.nodes.range = j->brange.as<uint32_t>();
     d.nodes.hw2 = j->bhw2.as<float>();
     d.nodes.counts = j->bcount.as<uint32_t>();
+    d.nodes.defer = j->bdefer.as<uint32_t>();
 }
 
 float resolve_learning_rate(const bht_job *j) {  // lib.rs:652-659
@@ -2537,7 +2542,8 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
         b += n * (2 * 8 + 2 * 4) + group_sort_temp_bytes(n);  // Morton sort
         if (D >= 5) b += 3 * n * 8;                            // two-word codes
         const uint64_t node_bytes = D >= 3 && D <= 7 ? (uint64_t)nd_node_floats((int)D) * 4 : 16;
-        b += 2 * n * node_bytes + 2 * n * 8 + 4 * n * 4;       // tree
+        const uint64_t nodes = (D == 2 ? 4 : 2) * n;
+        b += nodes * (node_bytes + 8 + 8) + (D == 2 ? n * 4 : 0);  // tree
         b += pad_tile(n) * 24 + group_sort_temp_bytes(pad_tile(n));  // batch arena
     } else {
         uint32_t maxb = std::min<uint32_t>(std::max<uint32_t>(p.max_boxes, 1), MAX_BOXES);

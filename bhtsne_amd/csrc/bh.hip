@@ -108,20 +108,131 @@ __device__ __forceinline__ uint32_t lower_bound_digit(const uint64_t *codes, uin
     return lo;
 }
 
-__global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
-                                                            const uint32_t *map) {
+// First position in [lo, hi) whose digit at `shift` is >= g, or hi (digits nondecreasing there),
+// found by one wave: 64 probes per round narrow the range 64-fold, so a range of a million codes
+// takes four rounds of one load each instead of twenty dependent loads.  lo and hi stay
+// wave-uniform (they follow ballots).
+__device__ __forceinline__ uint32_t wave_lower_bound(const uint64_t *codes, uint32_t lo,
+                                                     uint32_t hi, int shift, uint32_t g,
+                                                     int lane) {
+    while (hi - lo > 64) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t p = lo + (uint32_t)lane * step;
+        const bool ge = p >= hi || (uint32_t)((codes[p] >> shift) & 3ull) >= g;
+        const uint64_t m = __ballot(ge);
+        if (m == 0ull) {  // every probe is below g: the answer is past the last one
+            lo = lo + 63u * step + 1u;
+            continue;
+        }
+        const int j = __builtin_ctzll(m);
+        if (j == 0) return lo;
+        const uint32_t nhi = lo + (uint32_t)j * step;
+        lo = lo + (uint32_t)(j - 1) * step + 1u;
+        hi = nhi < hi ? nhi : hi;
+    }
+    const uint32_t p = lo + (uint32_t)lane;
+    const bool ge = p >= hi || (uint32_t)((codes[p] >> shift) & 3ull) >= g;
+    const uint64_t m = __ballot(ge);
+    return m ? lo + (uint32_t)__builtin_ctzll(m) : hi;
+}
+
+// The tree is built in three launches, the same nodes and children (in digit order) as the
+// reference's breadth-first emission (arena.rs:197-309):
+//  1. k_bh_top, one workgroup per job: breadth-first rounds from the root, expanding only nodes
+//     of more than SUB_POINTS points; an internal node of at most SUB_POINTS points is deferred
+//     (listed, its children not yet emitted).  Rounds of at most 16 nodes search a node's child
+//     bounds with a whole wave (wave_lower_bound), larger rounds with a thread per node.  These
+//     nodes take indices [0, 2n) in round order.
+//  2. k_bh_sub, several workgroups per job: each takes deferred roots and emits the root's
+//     subtree breadth-first into a region of its own, [2n + 2s, 2n + 2e) for a root over sorted
+//     points [s, e) (a subtree over m points has at most 2m - 1 nodes, and the ranges of the
+//     deferred roots are disjoint), then the subtree's centres of mass bottom-up.
+//  3. k_bh_top_com, one workgroup per job: the centres of mass of the top nodes, bottom-up.
+// The walk reads each node's first child and child count, so the numbering (which differs from
+// the reference's below the top) changes no sum; the children of a node are contiguous and in
+// digit order as in the reference.
+constexpr uint32_t SUB_POINTS = 2048;
+constexpr int CNT_TOP = 0, CNT_ROUNDS = 1, CNT_DEFER = 2, CNT_RSTART = 8;  // counts[] layout
+
+int bh_sub_blocks(int n) {
+    return (int)std::min<uint64_t>(256, std::max<uint64_t>(1, ceil_div((uint64_t)(n > 0 ? n : 1),
+                                                                       SUB_POINTS)));
+}
+
+// Children of an internal node over sorted points [s, e) at `shift`: the bounds b[0..4] of the
+// digit groups 0..3 (thread-level binary searches).
+__device__ __forceinline__ uint32_t child_bounds(const uint64_t *codes, uint32_t s, uint32_t e,
+                                                 int shift, uint32_t *b) {
+    b[0] = s;
+    b[4] = e;
+    for (uint32_t g = 1; g < 4; g++) b[g] = lower_bound_digit(codes, s, e, shift, g);
+    uint32_t nchild = 0;
+    for (int g = 0; g < 4; g++) nchild += b[g + 1] > b[g] ? 1u : 0u;
+    return nchild;
+}
+
+// Emits the children of node `i` (digit groups b[0..4]) at indices first, first + 1, ... and
+// records the node's first child and meta.
+__device__ __forceinline__ void emit_children(const BhNodes &t, uint32_t i, uint32_t first,
+                                              uint32_t nchild, int level, const float *shw2,
+                                              const uint32_t *b) {
+    float4 nd = t.node[i];
+    nd.w = __uint_as_float(first);
+    t.node[i] = nd;
+    t.meta[i] = make_uint2(nchild | ((uint32_t)level << 8), __float_as_uint(shw2[level]));
+    uint32_t c = first;
+    for (int g = 0; g < 4; g++) {
+        if (b[g + 1] <= b[g]) continue;
+        t.node[c] = make_float4(0.f, 0.f, __uint_as_float(b[g + 1] - b[g]),
+                                __uint_as_float(SENTINEL));
+        t.meta[c] = make_uint2((uint32_t)BITS << 8, 0u);
+        t.range[2 * c] = b[g];
+        t.range[2 * c + 1] = b[g + 1];
+        c++;
+    }
+}
+
+// Centre of mass of node i from its members (a leaf) or its children (arena.rs:261-309).
+__device__ __forceinline__ void node_com(const BhNodes &t, uint32_t i, const float *y,
+                                         const uint32_t *sidx) {
+    float4 nd = t.node[i];
+    const uint32_t count = __float_as_uint(nd.z);
+    const uint32_t fc = __float_as_uint(nd.w);
+    float cx = 0.0f, cy = 0.0f;
+    if (fc == SENTINEL) {
+        for (uint32_t s = t.range[2 * i]; s < t.range[2 * i + 1]; s++) {
+            float2 p = reinterpret_cast<const float2 *>(y)[sidx[s]];
+            cx += p.x;
+            cy += p.y;
+        }
+    } else {
+        const uint32_t cc = t.meta[i].x & 0xffu;
+        for (uint32_t ch = fc; ch < fc + cc; ch++) {
+            float4 cn = t.node[ch];
+            const float w = (float)__float_as_uint(cn.z);
+            cx += cn.x * w;
+            cy += cn.y * w;
+        }
+    }
+    const float inv = 1.0f / (float)count;
+    nd.x = cx * inv;
+    nd.y = cy * inv;
+    t.node[i] = nd;
+}
+
+__global__ void __launch_bounds__(BUILD_THREADS) k_bh_top(const JobDev *jobs,
+                                                          const uint32_t *map) {
     __shared__ uint32_t sbuf[BUILD_THREADS / 64];
     __shared__ uint32_t round_start[MAX_ROUNDS + 2];
     __shared__ float shw2[BITS + 1];
+    __shared__ uint32_t ndef;
     const BlockCtx bc = block_ctx(map, jobs, F_ONE);
     const JobDev &J = jobs[bc.job];
-    const float *y = job_y(J);
     const int n = J.n;
     const uint64_t *codes = J.scodes;
-    const uint32_t *sidx = J.scidx;
     EpochScalars *scal = J.scal;
     const BhNodes t = J.nodes;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     if (tid == 0) {
         // level half-widths from the largest axis extent (arena.rs:167-174)
         float e0 = scal->amax[0] - scal->amin[0];
@@ -138,95 +249,172 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_build(const JobDev *jobs,
         t.range[1] = (uint32_t)n;
         round_start[0] = 0;
         round_start[1] = 1;
+        ndef = 0;
     }
     __syncthreads();
     if (n == 0) {
-        if (tid == 0) t.counts[0] = 0;
+        if (tid == 0) {
+            t.counts[CNT_TOP] = 0;
+            t.counts[CNT_ROUNDS] = 0;
+            t.counts[CNT_DEFER] = 0;
+        }
         return;
     }
-    // breadth-first emission (arena.rs:197-259); every thread keeps the node count, which each
-    // chunk's scan total advances alike in all threads
+    // every thread keeps the node count, which each chunk's scan total advances alike
     uint32_t nn = 1;
     int rounds = 1;
     for (int r = 0; r < MAX_ROUNDS; r++) {
         const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
         if (f0 == f1) break;
-        for (uint32_t base = f0; base < f1; base += BUILD_THREADS) {
-            const uint32_t i = base + tid;
+        if (f1 - f0 <= (uint32_t)(BUILD_THREADS / 64)) {
+            // a wave per node
+            const uint32_t i = f0 + (uint32_t)(tid >> 6);
             uint32_t nchild = 0, b[5] = {0, 0, 0, 0, 0};
             int level = BITS;
-            bool internal = false;
+            bool expand = false;
             if (i < f1) {
                 const uint32_t s = t.range[2 * i], e = t.range[2 * i + 1];
                 if (e - s > 1 && codes[s] != codes[e - 1]) {
-                    internal = true;
                     const uint64_t x = codes[s] ^ codes[e - 1];
                     const int highest = 63 - __clzll((long long)x);
                     level = (BITS - 1) - highest / 2;
                     const int shift = 2 * (BITS - 1 - level);
-                    b[0] = s;
-                    b[4] = e;
-                    for (uint32_t g = 1; g < 4; g++) b[g] = lower_bound_digit(codes, s, e, shift, g);
-                    for (int g = 0; g < 4; g++) nchild += b[g + 1] > b[g] ? 1u : 0u;
+                    if (e - s > SUB_POINTS) {
+                        expand = true;
+                        b[0] = s;
+                        b[4] = e;
+                        for (uint32_t g = 1; g < 4; g++)
+                            b[g] = wave_lower_bound(codes, s, e, shift, g, lane);
+                        for (int g = 0; g < 4; g++) nchild += b[g + 1] > b[g] ? 1u : 0u;
+                    } else if (lane == 0) {
+                        t.defer[atomicAdd(&ndef, 1u)] = i;
+                    }
                 }
             }
             uint32_t total;
-            const uint32_t off = block_excl_scan(nchild, sbuf, total);
-            const uint32_t first = nn + off;
-            if (i < f1 && internal) {
-                float4 nd = t.node[i];
-                nd.w = __uint_as_float(first);
-                t.node[i] = nd;
-                t.meta[i] = make_uint2(nchild | ((uint32_t)level << 8), __float_as_uint(shw2[level]));
-                uint32_t c = first;
-                for (int g = 0; g < 4; g++) {
-                    if (b[g + 1] <= b[g]) continue;
-                    t.node[c] = make_float4(0.f, 0.f, __uint_as_float(b[g + 1] - b[g]),
-                                            __uint_as_float(SENTINEL));
-                    t.meta[c] = make_uint2((uint32_t)BITS << 8, 0u);
-                    t.range[2 * c] = b[g];
-                    t.range[2 * c + 1] = b[g + 1];
-                    c++;
-                }
-            }
+            const uint32_t off = block_excl_scan(lane == 0 ? nchild : 0u, sbuf, total);
+            if (expand && lane == 0) emit_children(t, i, nn + off, nchild, level, shw2, b);
             nn += total;
+        } else {
+            for (uint32_t base = f0; base < f1; base += BUILD_THREADS) {
+                const uint32_t i = base + tid;
+                uint32_t nchild = 0, b[5] = {0, 0, 0, 0, 0};
+                int level = BITS;
+                bool expand = false;
+                if (i < f1) {
+                    const uint32_t s = t.range[2 * i], e = t.range[2 * i + 1];
+                    if (e - s > 1 && codes[s] != codes[e - 1]) {
+                        const uint64_t x = codes[s] ^ codes[e - 1];
+                        const int highest = 63 - __clzll((long long)x);
+                        level = (BITS - 1) - highest / 2;
+                        if (e - s > SUB_POINTS) {
+                            expand = true;
+                            nchild = child_bounds(codes, s, e, 2 * (BITS - 1 - level), b);
+                        } else {
+                            t.defer[atomicAdd(&ndef, 1u)] = i;
+                        }
+                    }
+                }
+                uint32_t total;
+                const uint32_t off = block_excl_scan(nchild, sbuf, total);
+                if (expand) emit_children(t, i, nn + off, nchild, level, shw2, b);
+                nn += total;
+            }
         }
         // the round's nodes and its end are visible to the whole block from here on
         if (tid == 0) round_start[r + 2] = nn;
         rounds = r + 1;
         __syncthreads();
     }
-    // centres of mass, deepest round first (arena.rs:261-309)
-    for (int r = rounds - 1; r >= 0; r--) {
-        const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
-        for (uint32_t i = f0 + tid; i < f1; i += BUILD_THREADS) {
-            float4 nd = t.node[i];
-            const uint32_t count = __float_as_uint(nd.z);
-            const uint32_t fc = __float_as_uint(nd.w);
-            float cx = 0.0f, cy = 0.0f;
-            if (fc == SENTINEL) {
-                for (uint32_t s = t.range[2 * i]; s < t.range[2 * i + 1]; s++) {
-                    float2 p = reinterpret_cast<const float2 *>(y)[sidx[s]];
-                    cx += p.x;
-                    cy += p.y;
+    if (tid == 0) {
+        t.counts[CNT_TOP] = nn;
+        t.counts[CNT_ROUNDS] = (uint32_t)rounds;
+        t.counts[CNT_DEFER] = ndef;
+    }
+    for (int r = tid; r <= rounds; r += BUILD_THREADS) t.counts[CNT_RSTART + r] = round_start[r];
+}
+
+__global__ void __launch_bounds__(BUILD_THREADS) k_bh_sub(const JobDev *jobs,
+                                                          const uint32_t *map) {
+    __shared__ uint32_t sbuf[BUILD_THREADS / 64];
+    __shared__ uint32_t round_start[MAX_ROUNDS + 2];
+    __shared__ float shw2[BITS + 1];
+    const BlockCtx bc = block_ctx(map, jobs, F_BSUB);
+    const JobDev &J = jobs[bc.job];
+    const float *y = job_y(J);
+    const uint32_t n = (uint32_t)J.n;
+    const uint64_t *codes = J.scodes;
+    const uint32_t *sidx = J.scidx;
+    const BhNodes t = J.nodes;
+    const int tid = threadIdx.x;
+    if (tid <= BITS) shw2[tid] = t.hw2[tid];
+    const uint32_t ndef = t.counts[CNT_DEFER];
+    __syncthreads();
+    for (uint32_t q = (uint32_t)bc.lb; q < ndef; q += (uint32_t)bc.nb) {
+        const uint32_t root = t.defer[q];
+        const uint32_t base = 2u * n + 2u * t.range[2 * root] - 1u;  // region of local node k >= 1
+        auto gid = [&](uint32_t k) { return k == 0 ? root : base + k; };
+        if (tid == 0) {
+            round_start[0] = 0;
+            round_start[1] = 1;
+        }
+        __syncthreads();
+        uint32_t nn = 1;
+        int rounds = 1;
+        for (int r = 0; r < MAX_ROUNDS; r++) {
+            const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
+            if (f0 == f1) break;
+            for (uint32_t kb = f0; kb < f1; kb += BUILD_THREADS) {
+                const uint32_t k = kb + tid;
+                const uint32_t i = gid(k);
+                uint32_t nchild = 0, b[5] = {0, 0, 0, 0, 0};
+                int level = BITS;
+                bool internal = false;
+                if (k < f1) {
+                    const uint32_t s = t.range[2 * i], e = t.range[2 * i + 1];
+                    if (e - s > 1 && codes[s] != codes[e - 1]) {
+                        internal = true;
+                        const uint64_t x = codes[s] ^ codes[e - 1];
+                        const int highest = 63 - __clzll((long long)x);
+                        level = (BITS - 1) - highest / 2;
+                        nchild = child_bounds(codes, s, e, 2 * (BITS - 1 - level), b);
+                    }
                 }
-            } else {
-                const uint32_t cc = t.meta[i].x & 0xffu;
-                for (uint32_t ch = fc; ch < fc + cc; ch++) {
-                    float4 cn = t.node[ch];
-                    const float w = (float)__float_as_uint(cn.z);
-                    cx += cn.x * w;
-                    cy += cn.y * w;
-                }
+                uint32_t total;
+                const uint32_t off = block_excl_scan(nchild, sbuf, total);
+                if (internal) emit_children(t, i, gid(nn + off), nchild, level, shw2, b);
+                nn += total;
             }
-            const float inv = 1.0f / (float)count;
-            nd.x = cx * inv;
-            nd.y = cy * inv;
-            t.node[i] = nd;
+            if (tid == 0) round_start[r + 2] = nn;
+            rounds = r + 1;
+            __syncthreads();
+        }
+        for (int r = rounds - 1; r >= 0; r--) {
+            const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
+            for (uint32_t k = f0 + tid; k < f1; k += BUILD_THREADS) node_com(t, gid(k), y, sidx);
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BUILD_THREADS) k_bh_top_com(const JobDev *jobs,
+                                                              const uint32_t *map) {
+    const BlockCtx bc = block_ctx(map, jobs, F_ONE);
+    const JobDev &J = jobs[bc.job];
+    const float *y = job_y(J);
+    const uint32_t *sidx = J.scidx;
+    const BhNodes t = J.nodes;
+    const int rounds = (int)t.counts[CNT_ROUNDS];
+    for (int r = rounds - 1; r >= 0; r--) {
+        const uint32_t f0 = t.counts[CNT_RSTART + r], f1 = t.counts[CNT_RSTART + r + 1];
+        for (uint32_t i = f0 + threadIdx.x; i < f1; i += BUILD_THREADS) {
+            // deferred roots (internal, at most SUB_POINTS points) have their centres already
+            const float4 nd = t.node[i];
+            if (__float_as_uint(nd.w) != SENTINEL && __float_as_uint(nd.z) <= SUB_POINTS) continue;
+            node_com(t, i, y, sidx);
         }
         __syncthreads();
     }
-    if (tid == 0) t.counts[0] = nn;
 }
 
 __global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const JobDev *jobs,
@@ -296,7 +484,11 @@ const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s) {
     hipLaunchKernelGGL(k_bh_codes, dim3(L.nb[F_PT]), dim3(256), 0, s, L.jobs, L.map[F_PT]);
     BHT_LAUNCH_CHECK();
     const bool alt = sort_group<uint64_t>(L, MORTON_KEY_BITS, s);
-    hipLaunchKernelGGL(k_bh_build, dim3(L.nb[F_ONE]), dim3(BUILD_THREADS), 0, s, L.jobs,
+    hipLaunchKernelGGL(k_bh_top, dim3(L.nb[F_ONE]), dim3(BUILD_THREADS), 0, s, L.jobs,
+                       L.map[F_ONE]);
+    hipLaunchKernelGGL(k_bh_sub, dim3(L.nb[F_BSUB]), dim3(BUILD_THREADS), 0, s, L.jobs,
+                       L.map[F_BSUB]);
+    hipLaunchKernelGGL(k_bh_top_com, dim3(L.nb[F_ONE]), dim3(BUILD_THREADS), 0, s, L.jobs,
                        L.map[F_ONE]);
     hipLaunchKernelGGL(k_bh_traverse, dim3(L.nb[F_TRAV]), dim3(TRAV_THREADS), 0, s, L.jobs,
                        L.map[F_TRAV]);

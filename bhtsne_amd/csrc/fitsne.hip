@@ -43,6 +43,7 @@ void job_block_counts(JobDev &J, bool batched) {
     J.nblk[F_SPR] = spread_blocks(n);
     J.nblk[F_ATT] = J.dim >= 3 ? nd_attract_blocks(n) : tiled_blocks(n);
     J.nblk[F_TRAV] = traverse_blocks(n);
+    J.nblk[F_BSUB] = bh_sub_blocks(n);
     J.nblk[F_LINE1] = 2;
     // transform lines per job: 2 * nodes (rows) and 2 * L (columns).  A job alone takes up to
     // 2048 blocks.  In a batch the grid a member reaches grows with its size, and the launch's

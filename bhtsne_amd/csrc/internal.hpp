@@ -64,9 +64,10 @@ struct BhNodes {
                       // D >= 3: nd_node_floats(D) floats per node, com[D], count, first_child
     uint2 *meta;      // [2n] child_count | level << 8; the squared half-width of an internal
                       // node's level (f32 bits), read beside the node by the walk
-    uint32_t *range;  // [4n] start, end in the sorted order
+    uint32_t *range;  // start, end in the sorted order, per node
     float *hw2;       // [33] squared half-width per level
-    uint32_t *counts; // [1] node count
+    uint32_t *counts; // build bookkeeping: top nodes, top rounds, deferred roots, round starts
+    uint32_t *defer;  // [n] D = 2: the roots of the subtrees built by several workgroups
 };
 
 // ---- job table -------------------------------------------------------------------------------
@@ -87,6 +88,7 @@ enum Family {
     F_LINE1,    // D = 1 line convolution (two blocks)
     F_ATT,      // attraction + update
     F_TRAV,     // Barnes-Hut traversal
+    F_BSUB,     // Barnes-Hut subtree builds (several blocks per job)
     NFAM
 };
 
@@ -204,6 +206,7 @@ uint64_t tiled_build(const uint32_t *rows, const uint32_t *cols, const float *va
                      int dim, uint64_t nnz, const TiledOut &o, void *temp, hipStream_t s);
 int spread_blocks(int n);
 int traverse_blocks(int n);
+int bh_sub_blocks(int n);
 void job_block_counts(JobDev &J, bool batched);
 
 // Segment of a segmented sort: the keys of one job at [base, base + n) of the arena, sorted in
