@@ -11,6 +11,7 @@ namespace bht {
 typedef float upd_f2 __attribute__((ext_vector_type(2)));
 
 constexpr int UPD_THREADS = 256;  // k_update runs over the point-parallel (F_PT) block partition
+constexpr int UPD_BATCH = 4;      // points per thread whose loads are in flight together
 
 struct StepCtx {
     float inv_z, momentum, learning_rate;
@@ -114,29 +115,45 @@ __global__ void __launch_bounds__(UPD_THREADS) k_update(const JobDev *jobs, cons
     const float sh[2] = {J.scal->ysub[0], J.scal->ysub[1]};
     double m0 = 0.0, m1 = 0.0;
     float mn[2] = {INFINITY, INFINITY}, mx[2] = {-INFINITY, -INFINITY};
-    const int stride = bc.nb * UPD_THREADS;
-    for (int i = bc.lb * UPD_THREADS + threadIdx.x; i < J.n; i += stride) {
-        float acc[2] = {0.0f, 0.0f}, yi[2] = {0.0f, 0.0f};
-        if (D == 2) {
-            const upd_f2 a = reinterpret_cast<BHT_GLOBAL const upd_f2 *>(fattr)[i];
-            const upd_f2 v = reinterpret_cast<BHT_GLOBAL const upd_f2 *>(y)[i];
-            acc[0] = a.x;
-            acc[1] = a.y;
-            yi[0] = v.x - sh[0];
-            yi[1] = v.y - sh[1];
-        } else {
+    const int64_t n = J.n, stride = (int64_t)bc.nb * UPD_THREADS;
+    // A thread's points i0, i0 + stride, ... are taken UPD_BATCH at a time: every load of the
+    // batch is issued before the first point is computed, so a thread waits for memory once per
+    // batch rather than once per point.  The points are still summed in the same order.
+    for (int64_t i0 = (int64_t)bc.lb * UPD_THREADS + threadIdx.x; i0 < n;
+         i0 += UPD_BATCH * stride) {
+        float acc[UPD_BATCH][2], yi[UPD_BATCH][2];
+        RowState<D> st[UPD_BATCH];
+#pragma unroll
+        for (int q = 0; q < UPD_BATCH; q++) {
+            const int64_t i = i0 + q * stride;
+            if (i >= n) break;
+            if (D == 2) {
+                const upd_f2 a = reinterpret_cast<BHT_GLOBAL const upd_f2 *>(fattr)[i];
+                const upd_f2 v = reinterpret_cast<BHT_GLOBAL const upd_f2 *>(y)[i];
+                acc[q][0] = a.x;
+                acc[q][1] = a.y;
+                yi[q][0] = v.x - sh[0];
+                yi[q][1] = v.y - sh[1];
+            } else {
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    acc[q][d] = fattr[(size_t)i * D + d];
+                    yi[q][d] = y[(size_t)i * D + d] - sh[d];
+                }
+            }
+            st[q] = load_row_state<D>(M, (int)i);
+        }
+#pragma unroll
+        for (int q = 0; q < UPD_BATCH; q++) {
+            const int64_t i = i0 + q * stride;
+            if (i >= n) break;
+            float yn[2];
+            apply_update<D>(M, sc, (int)i, acc[q], yi[q], st[q], m0, m1, yn);
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                acc[d] = fattr[(size_t)i * D + d];
-                yi[d] = y[(size_t)i * D + d] - sh[d];
+                mn[d] = fminf(mn[d], yn[d]);
+                mx[d] = fmaxf(mx[d], yn[d]);
             }
-        }
-        float yn[2];
-        apply_update<D>(M, sc, i, acc, yi, load_row_state<D>(M, i), m0, m1, yn);
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            mn[d] = fminf(mn[d], yn[d]);
-            mx[d] = fmaxf(mx[d], yn[d]);
         }
     }
     const double s0 = block_sum_f64(m0, red);

@@ -12,6 +12,10 @@
 namespace bht {
 
 constexpr int PT_THREADS = 256;
+// Point-parallel kernels stride over at most 1024 blocks; a thread's points are taken PT_BATCH
+// at a time, their loads issued before the first is computed (one wait for memory per batch).
+constexpr int PT_BATCH = 4;
+constexpr int GATHER_BATCH = 2;  // the gather's batch (nine grid values per 2-D point)
 
 int point_blocks(int n) {
     int b = (int)ceil_div((uint64_t)(n > 0 ? n : 1), PT_THREADS);
@@ -22,7 +26,7 @@ int point_blocks(int n) {
 // per chunk in rounds of 64.  Large jobs take 4 rounds, so a box of many points leaves 4x fewer
 // chunk partials (1M points: the fixup 29 -> 15 us); small jobs keep one round per wave for the
 // parallelism (4 rounds cost C4's plain fits 8%).
-__host__ __device__ inline int spread_rounds(int n) { return n >= 65536 ? 4 : 1; }
+__host__ __device__ constexpr int spread_rounds(int n) { return n >= 65536 ? 4 : 1; }
 
 // Two waves per spreading block: the block's LDS slab (20 KiB) then fits a CU beside two of the
 // attraction's blocks.
@@ -367,23 +371,34 @@ __global__ void __launch_bounds__(PT_THREADS) k_box_keys(const JobDev *jobs,
     const float *y = job_y(J);
     const float s0 = J.scal->ysub[0], s1 = J.scal->ysub[1];
     uint32_t *keys = J.keys, *vals = J.kidx;
-    const int n = J.n;
-    const int stride = bc.nb * PT_THREADS;
-    for (int i = bc.lb * PT_THREADS + threadIdx.x; i < n; i += stride) {
-        float l;
-        uint32_t key;
-        if (D == 2) {
-            float2 v = reinterpret_cast<const float2 *>(y)[i];
-            v.x = v.x - s0;
-            v.y = v.y - s1;
-            uint32_t b0 = (uint32_t)locate_box(v.x, p, l);
-            uint32_t b1 = (uint32_t)locate_box(v.y, p, l);
-            key = b0 * (uint32_t)p.boxes + b1;
-        } else {
-            key = (uint32_t)locate_box(y[i] - s0, p, l);
+    const int64_t n = J.n, stride = (int64_t)bc.nb * PT_THREADS;
+    // PT_BATCH points per thread with their loads in flight together
+    for (int64_t i0 = (int64_t)bc.lb * PT_THREADS + threadIdx.x; i0 < n;
+         i0 += PT_BATCH * stride) {
+        float2 v[PT_BATCH];
+#pragma unroll
+        for (int q = 0; q < PT_BATCH; q++) {
+            const int64_t i = i0 + q * stride;
+            if (i >= n) break;
+            if (D == 2) v[q] = reinterpret_cast<const float2 *>(y)[i];
+            else v[q].x = y[i];
         }
-        keys[i] = key;
-        vals[i] = (uint32_t)i;
+#pragma unroll
+        for (int q = 0; q < PT_BATCH; q++) {
+            const int64_t i = i0 + q * stride;
+            if (i >= n) break;
+            float l;
+            uint32_t key;
+            if (D == 2) {
+                const uint32_t b0 = (uint32_t)locate_box(v[q].x - s0, p, l);
+                const uint32_t b1 = (uint32_t)locate_box(v[q].y - s1, p, l);
+                key = b0 * (uint32_t)p.boxes + b1;
+            } else {
+                key = (uint32_t)locate_box(v[q].x - s0, p, l);
+            }
+            keys[i] = key;
+            vals[i] = (uint32_t)i;
+        }
     }
 }
 
@@ -412,10 +427,9 @@ constexpr int NV2 = 36;  // 9 nodes x 4 terms (D = 2)
 constexpr int NV1 = 9;   // 3 nodes x 3 terms (D = 1)
 
 template <int D>
-__device__ __forceinline__ void point_contrib(const float *y, uint32_t i, float s0, float s1,
-                                              const GridPlan &p, const Lagrange3 &lb, float *v) {
+__device__ __forceinline__ void point_contrib(float2 pt, float s0, float s1, const GridPlan &p,
+                                              const Lagrange3 &lb, float *v) {
     if (D == 2) {
-        float2 pt = reinterpret_cast<const float2 *>(y)[i];
         pt.x = pt.x - s0;
         pt.y = pt.y - s1;
         float l0, l1, w0[3], w1[3];
@@ -437,11 +451,11 @@ __device__ __forceinline__ void point_contrib(const float *y, uint32_t i, float 
                 for (int t = 0; t < 4; t++) v[(k0 + 3 * k1) * 4 + t] = wt * c[t];
             }
     } else {
-        float pt = y[i] - s0;
+        const float x = pt.x - s0;
         float l0, w0[3];
-        locate_box(pt, p, l0);
+        locate_box(x, p, l0);
         lb.weights(l0, w0);
-        float c[3] = {1.0f, pt, pt * pt};
+        float c[3] = {1.0f, x, x * x};
 #pragma unroll
         for (int k0 = 0; k0 < 3; k0++)
 #pragma unroll
@@ -449,27 +463,8 @@ __device__ __forceinline__ void point_contrib(const float *y, uint32_t i, float 
     }
 }
 
-template <int D>
-__device__ __forceinline__ void write_box(float4 *grid, const GridPlan &p, uint32_t key,
-                                          const float *v) {
-    if (D == 2) {
-        uint32_t b0 = key / (uint32_t)p.boxes, b1 = key % (uint32_t)p.boxes;
-#pragma unroll
-        for (int k1 = 0; k1 < 3; k1++)
-#pragma unroll
-            for (int k0 = 0; k0 < 3; k0++) {
-                int c = k0 + 3 * k1;
-                size_t node = (size_t)(3 * b0 + k0) * p.nodes + (3 * b1 + k1);
-                grid[node] = make_float4(v[c * 4 + 0], v[c * 4 + 1], v[c * 4 + 2], v[c * 4 + 3]);
-            }
-    } else {
-#pragma unroll
-        for (int k0 = 0; k0 < 3; k0++)
-            grid[3 * key + k0] = make_float4(v[k0 * 3 + 0], v[k0 * 3 + 1], v[k0 * 3 + 2], 0.0f);
-    }
-}
-
-// One term of a box's node values (write_box for a single lane's term t).
+// Term t of a box's node values: node t / 4 of the box's nine, component t % 4 (D = 2), or node
+// t / 3 of three, component t % 3 (D = 1).
 template <int D>
 __device__ __forceinline__ void write_box_term(float4 *grid, const GridPlan &p, uint32_t key,
                                                int t, float value) {
@@ -487,15 +482,35 @@ __device__ __forceinline__ void write_box_term(float4 *grid, const GridPlan &p, 
 constexpr int SLAB_ROW = 40;             // floats per point row of the spreading slab (>= NV2)
 constexpr int SLAB_ROW4 = SLAB_ROW / 4;
 
+constexpr int SPREAD_MAX_ROUNDS = 4;
+typedef float spr_f2 __attribute__((ext_vector_type(2)));
+static_assert(spread_rounds(1 << 30) <= SPREAD_MAX_ROUNDS, "a chunk's rounds are prefetched");
+
+// acc + rows [r0, r1) of column `lane` of a wave's slab, added in row order, eight reads in
+// flight at a time (-0.0 pads the last eight: x + -0.0 == x for every x).
+__device__ __forceinline__ float sum_slab_rows(const float *slab, int lane, int r0, int r1,
+                                               float acc) {
+    for (int r = r0; r < r1; r += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = r + u < r1 ? slab[(r + u) * SLAB_ROW + lane] : -0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = acc + v[u];
+    }
+    return acc;
+}
+
 template <int D>
 __global__ void __launch_bounds__(SPR_THREADS) k_spread(const JobDev *jobs, const uint32_t *map) {
     constexpr int NV = D == 2 ? NV2 : NV1;
     const BlockCtx bc = block_ctx(map, jobs, F_SPR);
     const JobDev &J = jobs[bc.job];
     const GridPlan p = J.scal->plan;
-    const float *y = job_y(J);
+    // global (not flat) loads, so that the slab's LDS waits do not also wait for them
+    BHT_GLOBAL const float *y = global_view((const float *)job_y(J));
     const float ys0 = J.scal->ysub[0], ys1 = J.scal->ysub[1];
-    const uint32_t *keys = J.skeys, *vals = J.sidx;
+    BHT_GLOBAL const uint32_t *keys = global_view((const uint32_t *)J.skeys);
+    BHT_GLOBAL const uint32_t *vals = global_view((const uint32_t *)J.sidx);
     float4 *grid = J.grid;
     float *part = J.part;
     const int n = J.n;
@@ -512,19 +527,46 @@ __global__ void __launch_bounds__(SPR_THREADS) k_spread(const JobDev *jobs, cons
     float *slab = reinterpret_cast<float *>(slab_mem[threadIdx.x >> 6]);
     for (int c = wave0; c < nchunks; c += nwaves) {
         const int c0 = c * chunk;
+        // every round's keys, point indices and positions, and the keys just before and after
+        // the chunk, loaded before the first round
+        uint32_t rkey[SPREAD_MAX_ROUNDS], ridx[SPREAD_MAX_ROUNDS];
+        float2 rpos[SPREAD_MAX_ROUNDS];
+#pragma unroll
+        for (int q = 0; q < SPREAD_MAX_ROUNDS; q++) {
+            const int s = c0 + 64 * q + lane;
+            const bool valid = q < rounds && s < n;
+            rkey[q] = valid ? keys[s] : 0xffffffffu;
+            ridx[q] = valid ? vals[s] : 0u;
+        }
+        const uint32_t key_before = c0 > 0 ? keys[c0 - 1] : 0xffffffffu;
+        const uint32_t key_after = c0 + chunk < n ? keys[c0 + chunk] : 0xffffffffu;
+#pragma unroll
+        for (int q = 0; q < SPREAD_MAX_ROUNDS; q++) {
+            const int s = c0 + 64 * q + lane;
+            rpos[q] = make_float2(0.0f, 0.0f);
+            if (q < rounds && s < n) {
+                if (D == 2) {
+                    const spr_f2 v = reinterpret_cast<BHT_GLOBAL const spr_f2 *>(y)[ridx[q]];
+                    rpos[q] = make_float2(v.x, v.y);
+                } else {
+                    rpos[q] = make_float2(y[ridx[q]], 0.0f);
+                }
+            }
+        }
         float acc = 0.0f;      // lane t < NV: term t of the run being summed
         bool open = false;     // a run continues from the previous round
         bool open_rs = false;  // and it started in this chunk
-        for (int q = 0; q < rounds; q++) {
+#pragma unroll
+        for (int q = 0; q < SPREAD_MAX_ROUNDS; q++) {
             const int s0 = c0 + 64 * q;
-            if (s0 >= n) break;
+            if (q >= rounds || s0 >= n) break;
             const int s = s0 + lane;
             const bool valid = s < n;
-            const uint32_t key = valid ? keys[s] : 0xffffffffu;
+            const uint32_t key = rkey[q];
             {
                 float v[NV];
                 if (valid) {
-                    point_contrib<D>(y, vals[s], ys0, ys1, p, lb, v);
+                    point_contrib<D>(rpos[q], ys0, ys1, p, lb, v);
                 } else {
 #pragma unroll
                     for (int t = 0; t < NV; t++) v[t] = 0.0f;
@@ -541,7 +583,10 @@ __global__ void __launch_bounds__(SPR_THREADS) k_spread(const JobDev *jobs, cons
             // continue, or at the chunk's last point.  Per point, whether its run began in this
             // chunk (rs) and whether it ends here (re).
             const uint32_t next = __shfl_down(key, 1, 64);
-            const uint32_t after = lane == 63 ? (s + 1 < n ? keys[s + 1] : 0xffffffffu) : next;
+            // lane 63's successor: the next round's first key, or the one after the chunk
+            const uint32_t succ = q + 1 < rounds ? (uint32_t)__shfl(rkey[q + 1 < SPREAD_MAX_ROUNDS ? q + 1 : q], 0, 64)
+                                                 : key_after;
+            const uint32_t after = lane == 63 ? (s + 1 < n ? succ : 0xffffffffu) : next;
             const bool cont = valid && s + 1 < n && after == key;
             const bool last_round = q == rounds - 1 || s0 + 64 >= n;
             const bool tail = valid && (!cont || (lane == 63 && last_round));
@@ -554,7 +599,7 @@ __global__ void __launch_bounds__(SPR_THREADS) k_spread(const JobDev *jobs, cons
                 const int h = 63 - __clzll(heads & upto);  // the run's first lane in this round
                 if (h > 0) rs = true;
                 else if (q > 0) rs = open ? open_rs : true;
-                else rs = c0 == 0 || keys[c0 - 1] != key;
+                else rs = c0 == 0 || key_before != key;
             }
             const bool re = tail && !cont;
             const uint64_t rs_m = __ballot(rs), re_m = __ballot(re);
@@ -565,8 +610,7 @@ __global__ void __launch_bounds__(SPR_THREADS) k_spread(const JobDev *jobs, cons
             while (rest) {
                 const int pt = __ffsll((long long)rest) - 1;  // uniform: next run end
                 rest &= rest - 1;
-                if (lane < NV)
-                    for (; row <= pt; row++) acc += slab[row * SLAB_ROW + lane];
+                if (lane < NV) acc = sum_slab_rows(slab, lane, row, pt + 1, acc);
                 row = pt + 1;
                 const uint32_t rkey = __shfl(key, pt, 64);
                 const bool r_s = (rs_m >> pt) & 1ull, r_e = (re_m >> pt) & 1ull;
@@ -583,8 +627,7 @@ __global__ void __launch_bounds__(SPR_THREADS) k_spread(const JobDev *jobs, cons
             // the round's last run continues into the next round: carry its sums
             open = s0 + 63 < n && !((tails >> 63) & 1ull);
             if (open) {
-                if (lane < NV)
-                    for (; row < 64; row++) acc += slab[row * SLAB_ROW + lane];
+                if (lane < NV) acc = sum_slab_rows(slab, lane, row, 64, acc);
                 open_rs = (rs_m >> 63) & 1ull;
             }
             __builtin_amdgcn_wave_barrier();
@@ -592,9 +635,12 @@ __global__ void __launch_bounds__(SPR_THREADS) k_spread(const JobDev *jobs, cons
     }
 }
 
-// One wave per chunk; the chunk where a spanning run starts (its owner) finds the run's last chunk
-// by a 64-way search of the sorted keys and sums the chunk partials with the lanes (strided, then a
-// fixed butterfly), so a run of any length costs O(length / 64 / 64) steps.
+// One wave per chunk; the chunk where a spanning run starts (its owner) sums the partials of the
+// run's later chunks onto its own.  It first probes the first key of the next 64 chunks at once:
+// a run that ends within them (every run once the embedding has spread out) is summed by lane t
+// adding term t of each chunk in chunk order.  A longer run (the first epochs pack most points
+// into a few boxes) is bounded by a 64-way search of the sorted keys and summed by the lanes
+// (strided, then a fixed butterfly), so a run of any length costs O(length / 64 / 64) steps.
 template <int D>
 __global__ void __launch_bounds__(SPR_THREADS) k_spread_fixup(const JobDev *jobs,
                                                              const uint32_t *map) {
@@ -613,15 +659,37 @@ __global__ void __launch_bounds__(SPR_THREADS) k_spread_fixup(const JobDev *jobs
     const int wave0 = (bc.lb * SPR_THREADS + threadIdx.x) >> 6;
     const int nwaves = (bc.nb * SPR_THREADS) >> 6;
     for (int c = wave0; c < nchunks; c += nwaves) {
-        const int last_s = min(c * chunk + chunk - 1, n - 1);
-        const uint32_t key = keys[last_s];
-        if (!(last_s + 1 < n && keys[last_s + 1] == key)) continue;  // run ends in this chunk
         const int first_s = c * chunk;
-        const bool whole = keys[first_s] == key;
-        if (whole && first_s > 0 && keys[first_s - 1] == key) continue;  // not the run's owner
+        const int last_s = min(first_s + chunk - 1, n - 1);
+        // the four keys the ownership test reads, loaded together
+        const uint32_t key = keys[last_s];
+        const uint32_t after = last_s + 1 < n ? keys[last_s + 1] : ~0u;
+        const uint32_t head = keys[first_s];
+        const uint32_t before = first_s > 0 ? keys[first_s - 1] : ~0u;
+        if (!(last_s + 1 < n && after == key)) continue;  // the run ends in this chunk
+        if (head == key && first_s > 0 && before == key) continue;  // not the run's owner
+        // chunk j > c holds a point of the run iff its first key is the run's (keys are sorted)
+        const int j = c + 1 + lane;
+        const int near = __popcll(__ballot(j < nchunks && keys[(size_t)j * chunk] == key));
+        if (near < 64) {
+            const int c_end = c + near;  // chunk holding the run's last point
+            if (lane < NV) {
+                float tot = part_last[(size_t)c * NV + lane];
+                for (int j0 = c + 1; j0 <= c_end; j0 += 8) {
+                    float v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        v[u] = j0 + u <= c_end ? part_first[(size_t)(j0 + u) * NV + lane] : -0.0f;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) tot = tot + v[u];
+                }
+                write_box_term<D>(grid, p, key, lane, tot);
+            }
+            continue;
+        }
         // first sorted position past the run, in [lo, hi]: a 64-way search, each lane probing one
         // point of the range per step (keys[lo - 1] == key; keys[hi] > key or hi == n)
-        int lo = last_s + 1, hi = n;
+        int lo = (c + 64) * chunk + 1, hi = n;
         while (lo < hi) {
             const int step = (hi - lo + 63) / 64;
             const int probe = lo + lane * step;
@@ -631,20 +699,21 @@ __global__ void __launch_bounds__(SPR_THREADS) k_spread_fixup(const JobDev *jobs
             hi = min(hi, lo + c_le * step);
             lo = lo + (c_le - 1) * step + 1;
         }
-        const int c_end = (lo - 1) / chunk;  // chunk holding the run's last point
+        const int c_end = (lo - 1) / chunk;
         float tot[NV];
 #pragma unroll
         for (int t = 0; t < NV; t++) tot[t] = 0.0f;
-        for (int j = c + 1 + lane; j <= c_end; j += 64)
+        for (int jj = c + 1 + lane; jj <= c_end; jj += 64)
 #pragma unroll
-            for (int t = 0; t < NV; t++) tot[t] += part_first[(size_t)j * NV + t];
+            for (int t = 0; t < NV; t++) tot[t] += part_first[(size_t)jj * NV + t];
 #pragma unroll
         for (int t = 0; t < NV; t++) tot[t] = wave_sum_f32(tot[t]);
-        if (lane == 0) {
+        // lane t keeps term t
+        float mine = 0.0f;
 #pragma unroll
-            for (int t = 0; t < NV; t++) tot[t] = part_last[(size_t)c * NV + t] + tot[t];
-            write_box<D>(grid, p, key, tot);
-        }
+        for (int t = 0; t < NV; t++)
+            if (lane == t) mine = tot[t];
+        if (lane < NV) write_box_term<D>(grid, p, key, lane, part_last[(size_t)c * NV + lane] + mine);
     }
 }
 
@@ -660,61 +729,91 @@ __global__ void __launch_bounds__(PT_THREADS) k_gather(const JobDev *jobs, const
     const float s0 = J.scal->ysub[0], s1 = J.scal->ysub[1];
     const float4 *grid = J.grid;
     float *frep = J.frep;
-    const int n = J.n;
+    const int64_t n = J.n, stride = (int64_t)bc.nb * PT_THREADS;
     const Lagrange3 lb;
     double zacc = 0.0;
-    const int stride = bc.nb * PT_THREADS;
-    for (int i = bc.lb * PT_THREADS + threadIdx.x; i < n; i += stride) {
-        if (D == 2) {
-            float2 pt = reinterpret_cast<const float2 *>(y)[i];
-            pt.x = pt.x - s0;
-            pt.y = pt.y - s1;
-            float l0, l1, w0[3], w1[3];
-            int b0 = locate_box(pt.x, p, l0);
-            int b1 = locate_box(pt.y, p, l1);
-            lb.weights(l0, w0);
-            lb.weights(l1, w1);
-            float phi0 = 0.f, phi1 = 0.f, phi2 = 0.f, phi3 = 0.f;
+    constexpr int NB = GATHER_BATCH;
+    constexpr int NG = D == 2 ? 9 : 3;  // grid nodes per point
+    // NB points per thread: their positions, then the grid values of all of them, are loaded
+    // before the first is reconstructed; Z is summed in the points' order.
+    for (int64_t i0 = (int64_t)bc.lb * PT_THREADS + threadIdx.x; i0 < n; i0 += NB * stride) {
+        float2 pt[NB];
 #pragma unroll
-            for (int k1 = 0; k1 < 3; k1++)
+        for (int q = 0; q < NB; q++) {
+            const int64_t i = i0 + q * stride;
+            if (i >= n) break;
+            if (D == 2) pt[q] = reinterpret_cast<const float2 *>(y)[i];
+            else pt[q] = make_float2(y[i], 0.0f);
+        }
+        float w0[NB][3], w1[NB][3];
+        float4 g[NB][NG];
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int64_t i = i0 + q * stride;
+            if (i >= n) break;
+            pt[q].x = pt[q].x - s0;
+            float l0, l1;
+            const int b0 = locate_box(pt[q].x, p, l0);
+            lb.weights(l0, w0[q]);
+            if (D == 2) {
+                pt[q].y = pt[q].y - s1;
+                const int b1 = locate_box(pt[q].y, p, l1);
+                lb.weights(l1, w1[q]);
+#pragma unroll
+                for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+                    for (int k0 = 0; k0 < 3; k0++)
+                        g[q][k0 + 3 * k1] = grid[(size_t)(3 * b0 + k0) * p.nodes + (3 * b1 + k1)];
+            } else {
+#pragma unroll
+                for (int k0 = 0; k0 < 3; k0++) g[q][k0] = grid[3 * b0 + k0];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int64_t i = i0 + q * stride;
+            if (i >= n) break;
+            if (D == 2) {
+                float phi0 = 0.f, phi1 = 0.f, phi2 = 0.f, phi3 = 0.f;
+#pragma unroll
+                for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+                    for (int k0 = 0; k0 < 3; k0++) {
+                        const float wt = w0[q][k0] * w1[q][k1];
+                        const float4 gv = g[q][k0 + 3 * k1];
+                        phi0 = phi0 + wt * gv.x;
+                        phi1 = phi1 + wt * gv.y;
+                        phi2 = phi2 + wt * gv.z;
+                        phi3 = phi3 + wt * gv.w;
+                    }
+                const float px = pt[q].x, py = pt[q].y;
+                float2 f;
+                f.x = px * phi0 - phi1;
+                f.y = py * phi0 - phi2;
+                float norm_sq = 0.0f, cross = 0.0f;
+                norm_sq = norm_sq + px * px;
+                cross = cross + px * phi1;
+                norm_sq = norm_sq + py * py;
+                cross = cross + py * phi2;
+                reinterpret_cast<float2 *>(frep)[i] = f;
+                const float zt = (1.0f + norm_sq) * phi0 - 2.0f * cross + phi3;
+                zacc += (double)zt;
+            } else {
+                const float px = pt[q].x;
+                float phi0 = 0.f, phi1 = 0.f, phi2 = 0.f;
 #pragma unroll
                 for (int k0 = 0; k0 < 3; k0++) {
-                    float wt = w0[k0] * w1[k1];
-                    float4 g = grid[(size_t)(3 * b0 + k0) * p.nodes + (3 * b1 + k1)];
-                    phi0 = phi0 + wt * g.x;
-                    phi1 = phi1 + wt * g.y;
-                    phi2 = phi2 + wt * g.z;
-                    phi3 = phi3 + wt * g.w;
+                    const float4 gv = g[q][k0];
+                    phi0 = phi0 + w0[q][k0] * gv.x;
+                    phi1 = phi1 + w0[q][k0] * gv.y;
+                    phi2 = phi2 + w0[q][k0] * gv.w;
                 }
-            float2 f;
-            f.x = pt.x * phi0 - phi1;
-            f.y = pt.y * phi0 - phi2;
-            float norm_sq = 0.0f, cross = 0.0f;
-            norm_sq = norm_sq + pt.x * pt.x;
-            cross = cross + pt.x * phi1;
-            norm_sq = norm_sq + pt.y * pt.y;
-            cross = cross + pt.y * phi2;
-            reinterpret_cast<float2 *>(frep)[i] = f;
-            float zt = (1.0f + norm_sq) * phi0 - 2.0f * cross + phi3;
-            zacc += (double)zt;
-        } else {
-            float pt = y[i] - s0;
-            float l0, w0[3];
-            int b0 = locate_box(pt, p, l0);
-            lb.weights(l0, w0);
-            float phi0 = 0.f, phi1 = 0.f, phi2 = 0.f;
-#pragma unroll
-            for (int k0 = 0; k0 < 3; k0++) {
-                float4 g = grid[3 * b0 + k0];
-                phi0 = phi0 + w0[k0] * g.x;
-                phi1 = phi1 + w0[k0] * g.y;
-                phi2 = phi2 + w0[k0] * g.w;
+                frep[i] = px * phi0 - phi1;
+                const float norm_sq = 0.0f + px * px;
+                const float cross = 0.0f + px * phi1;
+                const float zt = (1.0f + norm_sq) * phi0 - 2.0f * cross + phi2;
+                zacc += (double)zt;
             }
-            frep[i] = pt * phi0 - phi1;
-            float norm_sq = 0.0f + pt * pt;
-            float cross = 0.0f + pt * phi1;
-            float zt = (1.0f + norm_sq) * phi0 - 2.0f * cross + phi2;
-            zacc += (double)zt;
         }
     }
     double bs = block_sum_f64(zacc, red);

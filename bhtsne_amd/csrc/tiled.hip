@@ -305,7 +305,12 @@ __device__ __forceinline__ float2 lds_pos_at(const float *a, uint32_t off) {
 }
 
 // Copies points [first, first + count) of y (D floats each) into LDS, less the epoch's pending
-// recentring shift (s0, s1); 16-byte loads when the start is 16-byte aligned.
+// recentring shift (s0, s1); 16-byte loads when the start is 16-byte aligned.  A thread issues
+// STAGE_BATCH loads before its first LDS store, so that a window (at most 2,950 float4s, under
+// six per thread) costs one wait for memory instead of one per float4: with one block per CU
+// (the attraction beside the repulsion) nothing else of the block runs while it stages.
+constexpr int STAGE_BATCH = 8;
+
 template <int D>
 __device__ void stage_points(BHT_GLOBAL const float *y, uint32_t first, uint32_t count,
                              float *dst, float s0, float s1) {
@@ -315,13 +320,23 @@ __device__ void stage_points(BHT_GLOBAL const float *y, uint32_t first, uint32_t
     // coordinate t of the copy is axis t % D (the start is a point boundary)
     const f4v sub = D == 2 ? f4v{s0, s1, s0, s1} : f4v{s0, s0, s0, s0};
     if (((first * D) & 3u) == 0) {
-        for (uint32_t k = threadIdx.x; 4 * k < nf; k += T_THREADS) {
-            if (4 * k + 3 < nf) {
-                ((BHT_LDS f4v *)out)[k] = ((BHT_GLOBAL const f4v *)src)[k] - sub;
-            } else {
-                for (uint32_t t = 4 * k; t < nf; t++)
-                    out[t] = src[t] - (D == 2 && (t & 1u) ? s1 : s0);
+        const uint32_t nf4 = nf / 4;  // whole float4s; the last nf % 4 floats one by one
+        for (uint32_t k0 = threadIdx.x; k0 < nf4; k0 += STAGE_BATCH * T_THREADS) {
+            f4v v[STAGE_BATCH];
+#pragma unroll
+            for (int u = 0; u < STAGE_BATCH; u++) {
+                const uint32_t k = k0 + u * T_THREADS;
+                if (k < nf4) v[u] = ((BHT_GLOBAL const f4v *)src)[k];
             }
+#pragma unroll
+            for (int u = 0; u < STAGE_BATCH; u++) {
+                const uint32_t k = k0 + u * T_THREADS;
+                if (k < nf4) ((BHT_LDS f4v *)out)[k] = v[u] - sub;
+            }
+        }
+        if (threadIdx.x < (nf & 3u)) {
+            const uint32_t t = (nf & ~3u) + threadIdx.x;
+            out[t] = src[t] - (D == 2 && (t & 1u) ? s1 : s0);
         }
     } else {
         for (uint32_t k = threadIdx.x; k < nf; k += T_THREADS)
@@ -451,6 +466,9 @@ __global__ void __launch_bounds__(T_THREADS) k_attract_tiled(const JobDev *jobs,
     __shared__ __align__(16) float yrow[TILE_ROWS_MAX * D];
     __shared__ __align__(16) float acc[TILE_ROWS_MAX * 2 + 64 * 2];  // (x, y) pairs
     __shared__ uint32_t next_lb;
+    // The attraction is the longer of the epoch's two streams: its waves take the SIMDs' issue
+    // slots first, and the repulsion's kernels, which have slack, fill the gaps.
+    __builtin_amdgcn_s_setprio(3);
     const BlockCtx bc = block_ctx(map, jobs, F_ATT);
     const JobDev &J = jobs[bc.job];
     const int lane = threadIdx.x & 63;
