@@ -1418,9 +1418,10 @@ void fit(bht_job *j, int method, float theta, bht_epoch_cb cb, void *user, bool 
 void check_dims(const bht_job *j, int method) {
     if (method == FIT_BH && (j->dim() < 2 || j->dim() > 7))
         throw Error(ERR_INVALID, "the Barnes-Hut path supports d_out in 2..7");
-    // the 2-D walk packs a child index (below 2n) in 29 bits
-    if (method == FIT_BH && j->dim() == 2 && j->n >= (1ull << 28))
-        throw Error(ERR_INVALID, "the Barnes-Hut path supports fewer than 2^28 points");
+    // the 2-D walk packs a child index (below 4n: the subtrees' regions follow the top nodes')
+    // in 29 bits
+    if (method == FIT_BH && j->dim() == 2 && j->n >= (1ull << 27))
+        throw Error(ERR_INVALID, "the Barnes-Hut path supports fewer than 2^27 points");
     if (method == FIT_FITSNE && j->dim() != 1 && j->dim() != 2)
         throw Error(ERR_INVALID, "the FIt-SNE path supports d_out in {1, 2}");
 }

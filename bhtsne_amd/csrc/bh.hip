@@ -4,18 +4,19 @@
 //  1. Morton codes of the quantised points (32 bits per axis, u64 words) from the per-axis
 //     bounding box, with the reference's f32 quantisation.
 //  2. Stable LSD radix sort of (code, index): ties keep index order.
-//  3. Breadth-first emission of the compressed quadtree in one workgroup: every round expands the
-//     current frontier in node order (children located by binary search of the sorted codes, a
-//     block scan assigns their slots), which yields exactly the reference's node numbering; then
-//     centres of mass bottom-up, round by round, with the reference's f32 summation order.
-//  4. One lane per point walks the tree in the reference's visit order; lanes take points in
-//     Morton order so a wave's walks share their paths.  Z is summed in FP64.  The reference pops
-//     node ids from a stack (up to 128 at D = 2); a lane here keeps a stack of child ranges (first
-//     child, children left), packed in a u32: taking the last child left of the top range is the
-//     node the reference pops next.  A range is held per internal node on the current path, whose
-//     levels increase strictly, so at most 33 ranges: 8.4 KB of LDS per wave, and the top range
-//     lives in a register.  An internal node's squared half-width is stored beside its child
-//     count, so a visit issues its two loads at once.
+//  3. Breadth-first emission of the compressed quadtree: every round expands the current
+//     frontier in node order (children located by searches of the sorted codes, a block scan
+//     assigns their slots), the top rounds in one workgroup per job and the subtrees below them in
+//     several (k_bh_top, k_bh_sub); then centres of mass bottom-up, round by round, with the
+//     reference's f32 summation order.
+//  4. Each point's walk in the reference's visit order, a wave walking the union of the walks
+//     of 64 consecutive points of the Morton order (k_bh_traverse).  Z is summed in FP64.  The
+//     reference pops node ids from a stack (up to 128 at D = 2); the walk keeps a stack of child
+//     ranges (first child, children left), packed in a u32: taking the last child left of the top
+//     range is the node the reference pops next.  A range is held per internal node on the
+//     current path, whose levels increase strictly, so at most 32 are saved.  An internal node's
+//     squared half-width is stored beside its child count, so a visit issues its two loads at
+//     once.
 #include <cfloat>
 
 #include "internal.hpp"
@@ -25,8 +26,7 @@ namespace bht {
 constexpr uint32_t SENTINEL = 0xffffffffu;
 constexpr int BITS = 32;
 constexpr int BUILD_THREADS = 1024;
-constexpr int TRAV_THREADS = 64;
-constexpr int STACK = BITS + 1;  // child ranges per lane
+constexpr int TRAV_THREADS = 256;  // four independent waves per walk block
 constexpr int MAX_ROUNDS = 64;
 
 __device__ __forceinline__ uint64_t part1by1(uint64_t x) {
@@ -73,8 +73,9 @@ __global__ void __launch_bounds__(256) k_bh_codes(const JobDev *jobs, const uint
     }
 }
 
-// block-wide exclusive scan of one u32 per thread (BUILD_THREADS threads): a shuffle scan in
-// each wave, then every thread adds the totals of the waves before its own
+// block-wide exclusive scan of one u32 per thread (THREADS threads): a shuffle scan in each
+// wave, then every thread adds the totals of the waves before its own
+template <int THREADS = BUILD_THREADS>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *buf, uint32_t &total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t x = v;
@@ -87,7 +88,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *buf, u
     __syncthreads();
     uint32_t pre = 0, tot = 0;
 #pragma unroll
-    for (int k = 0; k < BUILD_THREADS / 64; k++) {
+    for (int k = 0; k < THREADS / 64; k++) {
         const uint32_t c = buf[k];
         pre += k < w ? c : 0u;
         tot += c;
@@ -154,9 +155,14 @@ __device__ __forceinline__ uint32_t wave_lower_bound(const uint64_t *codes, uint
 constexpr uint32_t SUB_POINTS = 2048;
 constexpr int CNT_TOP = 0, CNT_ROUNDS = 1, CNT_DEFER = 2, CNT_RSTART = 8;  // counts[] layout
 
+// A subtree round rarely holds more than a few hundred nodes: 256 threads per subtree block.  A
+// job takes a block per 512 points (the deferred roots hold a few hundred to SUB_POINTS points
+// each, so most blocks build one subtree), at most 1024.
+constexpr int SUB_THREADS = 256;
+
 int bh_sub_blocks(int n) {
-    return (int)std::min<uint64_t>(256, std::max<uint64_t>(1, ceil_div((uint64_t)(n > 0 ? n : 1),
-                                                                       SUB_POINTS)));
+    return (int)std::min<uint64_t>(1024, std::max<uint64_t>(1, ceil_div((uint64_t)(n > 0 ? n : 1),
+                                                                        512)));
 }
 
 // Children of an internal node over sorted points [s, e) at `shift`: the bounds b[0..4] of the
@@ -334,11 +340,18 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_top(const JobDev *jobs,
     for (int r = tid; r <= rounds; r += BUILD_THREADS) t.counts[CNT_RSTART + r] = round_start[r];
 }
 
-__global__ void __launch_bounds__(BUILD_THREADS) k_bh_sub(const JobDev *jobs,
+// The subtree of one deferred root is built out of LDS: the root's sorted codes (at most
+// SUB_POINTS) and its nodes' point ranges, relative to the root's first point (at most
+// 2 SUB_POINTS - 1 nodes), so that the child-bound searches and the range reads of every round
+// cost LDS latency rather than a global round trip per probe.  Nodes, meta and ranges still go to
+// the job's arrays for the walk and the centres of mass.
+__global__ void __launch_bounds__(SUB_THREADS) k_bh_sub(const JobDev *jobs,
                                                           const uint32_t *map) {
-    __shared__ uint32_t sbuf[BUILD_THREADS / 64];
+    __shared__ uint32_t sbuf[SUB_THREADS / 64];
     __shared__ uint32_t round_start[MAX_ROUNDS + 2];
     __shared__ float shw2[BITS + 1];
+    __shared__ uint64_t lcodes[SUB_POINTS];
+    __shared__ uint32_t lrange[2 * (2 * SUB_POINTS)];
     const BlockCtx bc = block_ctx(map, jobs, F_BSUB);
     const JobDev &J = jobs[bc.job];
     const float *y = job_y(J);
@@ -352,11 +365,16 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_sub(const JobDev *jobs,
     __syncthreads();
     for (uint32_t q = (uint32_t)bc.lb; q < ndef; q += (uint32_t)bc.nb) {
         const uint32_t root = t.defer[q];
-        const uint32_t base = 2u * n + 2u * t.range[2 * root] - 1u;  // region of local node k >= 1
+        const uint32_t s0 = t.range[2 * root];
+        const uint32_t m = t.range[2 * root + 1] - s0;  // 2 .. SUB_POINTS (a deferred root)
+        const uint32_t base = 2u * n + 2u * s0 - 1u;    // region of local node k >= 1
         auto gid = [&](uint32_t k) { return k == 0 ? root : base + k; };
+        for (uint32_t k = tid; k < m; k += SUB_THREADS) lcodes[k] = codes[s0 + k];
         if (tid == 0) {
             round_start[0] = 0;
             round_start[1] = 1;
+            lrange[0] = 0;
+            lrange[1] = m;
         }
         __syncthreads();
         uint32_t nn = 1;
@@ -364,25 +382,45 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_sub(const JobDev *jobs,
         for (int r = 0; r < MAX_ROUNDS; r++) {
             const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
             if (f0 == f1) break;
-            for (uint32_t kb = f0; kb < f1; kb += BUILD_THREADS) {
+            for (uint32_t kb = f0; kb < f1; kb += SUB_THREADS) {
                 const uint32_t k = kb + tid;
-                const uint32_t i = gid(k);
                 uint32_t nchild = 0, b[5] = {0, 0, 0, 0, 0};
                 int level = BITS;
                 bool internal = false;
                 if (k < f1) {
-                    const uint32_t s = t.range[2 * i], e = t.range[2 * i + 1];
-                    if (e - s > 1 && codes[s] != codes[e - 1]) {
+                    const uint32_t s = lrange[2 * k], e = lrange[2 * k + 1];
+                    if (e - s > 1 && lcodes[s] != lcodes[e - 1]) {
                         internal = true;
-                        const uint64_t x = codes[s] ^ codes[e - 1];
+                        const uint64_t x = lcodes[s] ^ lcodes[e - 1];
                         const int highest = 63 - __clzll((long long)x);
                         level = (BITS - 1) - highest / 2;
-                        nchild = child_bounds(codes, s, e, 2 * (BITS - 1 - level), b);
+                        nchild = child_bounds(lcodes, s, e, 2 * (BITS - 1 - level), b);
                     }
                 }
                 uint32_t total;
-                const uint32_t off = block_excl_scan(nchild, sbuf, total);
-                if (internal) emit_children(t, i, gid(nn + off), nchild, level, shw2, b);
+                const uint32_t off = block_excl_scan<SUB_THREADS>(nchild, sbuf, total);
+                if (internal) {
+                    // the node (count = its range) with its first child, then the children, in
+                    // digit order, at local indices nn + off, ...
+                    const uint32_t i = gid(k), kc0 = nn + off;
+                    t.node[i] = make_float4(0.f, 0.f, __uint_as_float(b[4] - b[0]),
+                                            __uint_as_float(gid(kc0)));
+                    t.meta[i] = make_uint2(nchild | ((uint32_t)level << 8),
+                                           __float_as_uint(shw2[level]));
+                    uint32_t kc = kc0;
+                    for (int g = 0; g < 4; g++) {
+                        if (b[g + 1] <= b[g]) continue;
+                        const uint32_t c = gid(kc);
+                        t.node[c] = make_float4(0.f, 0.f, __uint_as_float(b[g + 1] - b[g]),
+                                                __uint_as_float(SENTINEL));
+                        t.meta[c] = make_uint2((uint32_t)BITS << 8, 0u);
+                        t.range[2 * c] = s0 + b[g];
+                        t.range[2 * c + 1] = s0 + b[g + 1];
+                        lrange[2 * kc] = b[g];
+                        lrange[2 * kc + 1] = b[g + 1];
+                        kc++;
+                    }
+                }
                 nn += total;
             }
             if (tid == 0) round_start[r + 2] = nn;
@@ -391,7 +429,7 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_sub(const JobDev *jobs,
         }
         for (int r = rounds - 1; r >= 0; r--) {
             const uint32_t f0 = round_start[r], f1 = round_start[r + 1];
-            for (uint32_t k = f0 + tid; k < f1; k += BUILD_THREADS) node_com(t, gid(k), y, sidx);
+            for (uint32_t k = f0 + tid; k < f1; k += SUB_THREADS) node_com(t, gid(k), y, sidx);
             __syncthreads();
         }
     }
@@ -417,66 +455,107 @@ __global__ void __launch_bounds__(BUILD_THREADS) k_bh_top_com(const JobDev *jobs
     }
 }
 
+// The walk.  A wave takes 64 consecutive points of the Morton order and walks the union of their
+// walks in the reference's visit order, one node at a time: every lane reads the same node, and a
+// lane evaluates it only while it is active there, that is, when it opened every ancestor.  An
+// active lane sums the node (a leaf other than its own, or an internal node far enough away) or
+// opens it; the wave descends when any lane opens, and the lanes that did not are muted below
+// that node until the walk is back at its depth.  Each lane therefore sums exactly the nodes of
+// its own walk, in the same order and with the same arithmetic, while the wave issues one scalar
+// load per node instead of 64 scattered ones.  The range stack is wave-uniform: entry k (the
+// siblings still to visit and their depth) sits in lane k of two registers.  The four waves of a
+// block walk independently.
+#define BHT_CONST __attribute__((address_space(4)))
+typedef float trav_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t trav_u2 __attribute__((ext_vector_type(2)));
+
 __global__ void __launch_bounds__(TRAV_THREADS) k_bh_traverse(const JobDev *jobs,
                                                               const uint32_t *map) {
-    __shared__ uint32_t stack[STACK][TRAV_THREADS];
+    __shared__ double red[TRAV_THREADS / 64];
     const BlockCtx bc = block_ctx(map, jobs, F_TRAV);
     const JobDev &J = jobs[bc.job];
     const float *y = job_y(J);
     const int n = J.n;
     const uint32_t *sidx = J.scidx;
     const BhNodes t = J.nodes;
+    // the tree was written by earlier launches and is read-only here: scalar loads
+    BHT_CONST const trav_f4 *node = (BHT_CONST const trav_f4 *)(const void *)t.node;
+    BHT_CONST const trav_u2 *meta = (BHT_CONST const trav_u2 *)(const void *)t.meta;
     const float theta_sq = J.theta_sq;
     float *frep = J.frep;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double zacc = 0.0;
-    for (int s = bc.lb * TRAV_THREADS + lane; s < n; s += bc.nb * TRAV_THREADS) {
-        const uint32_t i = sidx[s];
-        const float2 qp = reinterpret_cast<const float2 *>(y)[i];
+    for (int s0 = bc.lb * TRAV_THREADS + wid * 64; s0 < n; s0 += bc.nb * TRAV_THREADS) {
+        const int s = s0 + lane;
+        const bool valid = s < n;
+        const uint32_t i = valid ? sidx[s] : 0u;
+        const float2 qp = valid ? reinterpret_cast<const float2 *>(y)[i] : make_float2(0.f, 0.f);
         float fx = 0.0f, fy = 0.0f, q_sum = 0.0f;
-        // the top range, first child << 3 | children left (at most 4): the root as a range of one
-        uint32_t cur = 1u;
-        int below = 0;  // ranges under it, in LDS
-        bool have = true;
-        while (have) {
-            const uint32_t id = (cur >> 3) + (cur & 7u) - 1u;
-            cur -= 1u;
-            if ((cur & 7u) == 0u) {
-                if (below > 0) cur = stack[--below][lane];
-                else have = false;
+        int mute = valid ? INT_MAX : -1;  // the lane is active at depth d iff d < mute
+        uint32_t first = 0u, left = 1u;   // the range walked (the root), visited last node first
+        int dep = 0;                      // its depth
+        int below = 0;  // saved ranges (first << 3 | left), in lanes [0, below) of stk_range
+        int stk_range = 0, stk_dep = 0;
+        while (true) {
+            if (left == 0u) {
+                if (below == 0) break;
+                below--;
+                const uint32_t c = (uint32_t)__builtin_amdgcn_readlane(stk_range, below);
+                first = c >> 3;
+                left = c & 7u;
+                dep = __builtin_amdgcn_readlane(stk_dep, below);
+                continue;
             }
-            const float4 nd = t.node[id];
-            const uint2 m = t.meta[id];
+            const uint32_t id = first + left - 1u;
+            left -= 1u;
+            const trav_f4 nd = node[id];
+            const trav_u2 mt = meta[id];
+            const bool act = dep < mute;
             const float dx = qp.x - nd.x, dy = qp.y - nd.y;
             float distance = 0.0f;
             distance += dx * dx;
             distance += dy * dy;
             const uint32_t fc = __float_as_uint(nd.w);
-            if (fc == SENTINEL) {
-                if (distance == 0.0f) continue;  // the query's own leaf
-            } else if (__uint_as_float(m.y) >= theta_sq * distance) {
-                // descend: the siblings left wait under the children
-                if (have && below < STACK) stack[below++][lane] = cur;
-                cur = (fc << 3) | (m.x & 0xffu);
-                have = true;
-                continue;
+            const bool leaf = fc == SENTINEL;
+            const bool open = act && !leaf && __uint_as_float(mt.y) >= theta_sq * distance;
+            // a leaf at distance 0 is the query's own
+            if (act && !open && !(leaf && distance == 0.0f)) {
+                const float inverse = 1.0f / (1.0f + distance);
+                float magnitude = (float)__float_as_uint(nd.z) * inverse;
+                q_sum += magnitude;
+                magnitude = magnitude * inverse;
+                fx += magnitude * dx;
+                fy += magnitude * dy;
             }
-            const float inverse = 1.0f / (1.0f + distance);
-            float magnitude = (float)__float_as_uint(nd.z) * inverse;
-            q_sum += magnitude;
-            magnitude = magnitude * inverse;
-            fx += magnitude * dx;
-            fy += magnitude * dy;
+            if (__builtin_amdgcn_ballot_w64(open) != 0ull) {
+                // descend; the siblings not yet visited wait with their depth (at most one range
+                // per internal level on the path: 32 < 64 lanes)
+                if (left > 0u) {
+                    if (lane == below) {
+                        stk_range = (int)((first << 3) | left);
+                        stk_dep = dep;
+                    }
+                    below++;
+                }
+                if (act) mute = open ? INT_MAX : dep + 1;
+                first = fc;
+                left = mt.x & 0xffu;
+                dep++;
+            }
         }
-        reinterpret_cast<float2 *>(frep)[i] = make_float2(fx, fy);
-        zacc += (double)q_sum;
+        if (valid) {
+            reinterpret_cast<float2 *>(frep)[i] = make_float2(fx, fy);
+            zacc += (double)q_sum;
+        }
     }
-    zacc = wave_sum_f64(zacc);
-    if (lane == 0) J.dpart[bc.lb] = zacc;
+    const double zb = block_sum_f64(zacc, red);
+    if (threadIdx.x == 0) J.dpart[bc.lb] = zb;
 }
 
-int traverse_blocks(int n) {
-    return (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), TRAV_THREADS), 4096);
+int traverse_blocks(int n, int dim) {
+    // D >= 3 walks one wave per block (bhnd.hip)
+    const int threads = dim == 2 ? TRAV_THREADS : 64;
+    return (int)std::min<uint64_t>(ceil_div((uint64_t)(n > 0 ? n : 1), threads), 4096);
 }
 
 const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s) {
@@ -486,7 +565,7 @@ const uint32_t *bh_repulsion(const LaunchSet &L, hipStream_t s) {
     const bool alt = sort_group<uint64_t>(L, MORTON_KEY_BITS, s);
     hipLaunchKernelGGL(k_bh_top, dim3(L.nb[F_ONE]), dim3(BUILD_THREADS), 0, s, L.jobs,
                        L.map[F_ONE]);
-    hipLaunchKernelGGL(k_bh_sub, dim3(L.nb[F_BSUB]), dim3(BUILD_THREADS), 0, s, L.jobs,
+    hipLaunchKernelGGL(k_bh_sub, dim3(L.nb[F_BSUB]), dim3(SUB_THREADS), 0, s, L.jobs,
                        L.map[F_BSUB]);
     hipLaunchKernelGGL(k_bh_top_com, dim3(L.nb[F_ONE]), dim3(BUILD_THREADS), 0, s, L.jobs,
                        L.map[F_ONE]);

@@ -46,7 +46,7 @@ void job_block_counts(JobDev &J, bool batched) {
     J.nblk[F_ONE] = 1;
     J.nblk[F_SPR] = spread_blocks(n);
     J.nblk[F_ATT] = J.dim >= 3 ? nd_attract_blocks(n) : tiled_blocks(n);
-    J.nblk[F_TRAV] = traverse_blocks(n);
+    J.nblk[F_TRAV] = traverse_blocks(n, J.dim);
     J.nblk[F_BSUB] = bh_sub_blocks(n);
     J.nblk[F_LINE1] = 2;
     // transform lines per job: 2 * nodes (rows) and 2 * L (columns).  A job alone takes up to

@@ -205,7 +205,7 @@ TiledSizes tiled_sizes(int n, int dim, uint64_t nnz);
 uint64_t tiled_build(const uint32_t *rows, const uint32_t *cols, const float *vals, int n,
                      int dim, uint64_t nnz, const TiledOut &o, void *temp, hipStream_t s);
 int spread_blocks(int n);
-int traverse_blocks(int n);
+int traverse_blocks(int n, int dim);
 int bh_sub_blocks(int n);
 void job_block_counts(JobDev &J, bool batched);
 
