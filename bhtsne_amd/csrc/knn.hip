@@ -383,19 +383,50 @@ __global__ void __launch_bounds__(256) k_knn_rerank(const float *x, const float 
     const int i = blockIdx.x * 4 + w;
     if (i >= n) return;  // whole wave: no block barrier follows
     uint64_t *s = sb[w];
-    for (int q = lane; q < KBUF; q += 64) {
-        uint64_t key = ~0ull;
-        const uint32_t j = q < KCAND ? cand[(size_t)i * KCAND + q] : 0xffffffffu;
-        if (j < (uint32_t)n) {
-            float acc = 0.0f;
-            for (int d = 0; d < D; d++) {
-                const float e = x[(size_t)j * D + d] - x[(size_t)i * D + d];
-                acc = acc + e * e;
-            }
-            key = ((uint64_t)__float_as_uint(sqrtf(acc)) << 32) | j;
-        }
-        s[q] = key;
+    // a lane's KCAND / 64 candidates at once, eight dimensions of each per batch of loads (the
+    // sums still run left to right)
+    constexpr int PER = KCAND / 64;
+    static_assert(KCAND % 64 == 0, "candidates per lane");
+    uint32_t j[PER];
+    float acc[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        j[u] = cand[(size_t)i * KCAND + lane + 64 * u];
+        acc[u] = 0.0f;
     }
+    const float *qr = x + (size_t)i * D;
+    int d = 0;
+    for (; d + 8 <= D; d += 8) {
+        float qv[8], cv[PER][8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) qv[t] = qr[d + t];
+#pragma unroll
+        for (int u = 0; u < PER; u++)
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                cv[u][t] = j[u] < (uint32_t)n ? x[(size_t)j[u] * D + d + t] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < PER; u++)
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const float e = cv[u][t] - qv[t];
+                acc[u] = acc[u] + e * e;
+            }
+    }
+    for (; d < D; d++) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const float c = j[u] < (uint32_t)n ? x[(size_t)j[u] * D + d] : 0.0f;
+            const float e = c - qr[d];
+            acc[u] = acc[u] + e * e;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++)
+        s[lane + 64 * u] = j[u] < (uint32_t)n
+                               ? ((uint64_t)__float_as_uint(sqrtf(acc[u])) << 32) | j[u]
+                               : ~0ull;
+    for (int q = KCAND + lane; q < KBUF; q += 64) s[q] = ~0ull;
     __builtin_amdgcn_wave_barrier();
     wave_bitonic_u64(s, lane);
     for (int q = lane; q < k; q += 64) {
