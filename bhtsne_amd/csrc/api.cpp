@@ -1658,6 +1658,8 @@ void batch_rebuild(bht_batch *b) {
         L.segs = g.segs.as<SortSeg>();
         L.tile_seg = g.tile_seg.as<uint32_t>();
         L.sort_tiles = (uint32_t)tile_seg.size();
+        L.max_seg_tiles = 0;
+        for (const SortSeg &sg : segs) L.max_seg_tiles = std::max(L.max_seg_tiles, sg.ntiles);
     }
     b->table.ensure(c, std::max<size_t>(table.size(), 1) * sizeof(JobDev));
     if (!table.empty())

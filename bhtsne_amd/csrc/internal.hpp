@@ -237,6 +237,7 @@ struct LaunchSet {
     const SortSeg *segs = nullptr;      // null: one unpadded segment of sort_n keys
     const uint32_t *tile_seg = nullptr; // segment of each tile
     uint32_t sort_tiles = 0;            // tiles of the segmented sort
+    uint32_t max_seg_tiles = ~0u;       // tiles of its largest segment
     // a single job's box sort with its pass count decided on the device (the grid plan's
     // sort_passes); its result is then in (keys_fin, vals_fin)
     const int *sort_npass = nullptr;

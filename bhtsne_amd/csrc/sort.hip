@@ -192,48 +192,72 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K *keys, uint64_t 
 }
 
 // In place: each digit's tile counts become their exclusive prefix within the segment (tile
-// order), and the digit's total goes to totals.  One wave per (segment, digit).
+// order), and the digit's total goes to totals.  A wave takes DPW consecutive digits of one
+// segment (DPW = 1, or SEG_DPW for segments of at most 64 tiles: a batch's many small segments
+// would otherwise launch a wave per (segment, digit)); their counts are loaded DIG_BATCH digits
+// at a time.
+constexpr int SEG_DPW = 16;
+constexpr int DIG_BATCH = 4;
+
 template <int BITS>
 __global__ void __launch_bounds__(256) k_rs_digit_scan(uint32_t *hist, uint32_t *totals,
                                                        unsigned ntiles, const SortSeg *segs,
                                                        unsigned nsegs, const int *npass,
-                                                       int pass) {
+                                                       int pass, int dpw) {
     constexpr unsigned BINS = 1u << BITS;
     if (pass_skipped(npass, pass)) return;
     const unsigned lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
-    if (wave >= (uint64_t)nsegs * BINS) return;
-    const unsigned g = (unsigned)(wave / BINS), d = (unsigned)(wave % BINS);
+    const unsigned groups = BINS / (unsigned)dpw;  // digit groups per segment
+    if (wave >= (uint64_t)nsegs * groups) return;
+    const unsigned g = (unsigned)(wave / groups), d0 = (unsigned)(wave % groups) * (unsigned)dpw;
     size_t h0, t0;
     unsigned cnt;
     if (segs) {
         const SortSeg sg = segs[g];
-        h0 = (size_t)BINS * sg.tile0 + (size_t)d * sg.ntiles;
+        h0 = (size_t)BINS * sg.tile0;
         t0 = (size_t)BINS * sg.tile0;
         cnt = sg.ntiles;
     } else {
-        h0 = (size_t)d * ntiles;
+        h0 = 0;
         t0 = 0;
         cnt = ntiles;
     }
     if (cnt == 0) return;  // an empty segment: no tile reads its totals
-    // each lane scans a contiguous run of the counts; the lanes' sums are scanned across the wave
+    // each lane scans a contiguous run of a digit's counts; the lanes' sums are scanned across
+    // the wave
     const unsigned per = (cnt + 63) / 64, b = lane * per, e = b + per < cnt ? b + per : cnt;
-    uint32_t sum = 0;
-    for (unsigned t = b; t < e; t++) sum += hist[h0 + t];
-    uint32_t incl = sum;
+    for (unsigned dq = 0; dq < (unsigned)dpw; dq += DIG_BATCH) {
+        const unsigned nd = (unsigned)dpw - dq < (unsigned)DIG_BATCH ? (unsigned)dpw - dq
+                                                                     : (unsigned)DIG_BATCH;
+        uint32_t sum[DIG_BATCH];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(incl, o, 64);
-        if ((int)lane >= o) incl += u;
+        for (int u = 0; u < DIG_BATCH; u++) {
+            sum[u] = 0;
+            if ((unsigned)u < nd) {
+                const size_t hd = h0 + (size_t)(d0 + dq + u) * cnt;
+                for (unsigned t = b; t < e; t++) sum[u] += hist[hd + t];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < DIG_BATCH; u++) {
+            if ((unsigned)u >= nd) continue;
+            uint32_t incl = sum[u];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o, 64);
+                if ((int)lane >= o) incl += v;
+            }
+            const size_t hd = h0 + (size_t)(d0 + dq + u) * cnt;
+            uint32_t run = incl - sum[u];
+            for (unsigned t = b; t < e; t++) {
+                const uint32_t v = hist[hd + t];
+                hist[hd + t] = run;
+                run += v;
+            }
+            if (lane == 63) totals[t0 + d0 + dq + u] = incl;
+        }
     }
-    uint32_t run = incl - sum;
-    for (unsigned t = b; t < e; t++) {
-        const uint32_t v = hist[h0 + t];
-        hist[h0 + t] = run;
-        run += v;
-    }
-    if (lane == 63) totals[t0 + d] = incl;
 }
 
 // Exclusive prefix of the BINS digit totals of a segment into base[] (all threads of the block).
@@ -367,11 +391,14 @@ static bool radix_passes(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_al
                          unsigned ntiles, const SortSeg *segs, unsigned nsegs,
                          const uint32_t *tile_seg, int begin_bit, int end_bit, void *temp,
                          hipStream_t s, const int *npass = nullptr, K *keys_fin = nullptr,
-                         uint32_t *vals_fin = nullptr) {
+                         uint32_t *vals_fin = nullptr, uint32_t max_seg_tiles = ~0u) {
     const uint64_t nh = (uint64_t)ntiles << BITS;
     uint32_t *hist = (uint32_t *)temp;
     uint32_t *totals = hist + nh;
-    const unsigned scan_blocks = (unsigned)ceil_div((uint64_t)nsegs << BITS, 4);
+    // digits per scan wave: SEG_DPW when every segment has at most 64 tiles
+    const int dpw = segs != nullptr && max_seg_tiles <= 64 && (1 << BITS) % SEG_DPW == 0
+                        ? SEG_DPW : 1;
+    const unsigned scan_blocks = (unsigned)ceil_div(((uint64_t)nsegs << BITS) / dpw, 4);
     K *ki = keys, *ko = keys_alt;
     uint32_t *vi = vals, *vo = vals_alt;
     bool alt = false;
@@ -380,7 +407,7 @@ static bool radix_passes(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_al
         hipLaunchKernelGGL((k_rs_hist<K, BITS>), dim3(ntiles), dim3(RS_THREADS), 0, s, ki, n,
                            shift, hist, ntiles, segs, tile_seg, npass, pass);
         hipLaunchKernelGGL((k_rs_digit_scan<BITS>), dim3(scan_blocks), dim3(256), 0, s, hist,
-                           totals, ntiles, segs, nsegs, npass, pass);
+                           totals, ntiles, segs, nsegs, npass, pass, dpw);
         hipLaunchKernelGGL((k_rs_scatter<K, BITS>), dim3(ntiles), dim3(RS_THREADS), 0, s, ki, vi,
                            ko, vo, n, shift, hist, totals, ntiles, segs, tile_seg, npass, pass,
                            keys_fin, vals_fin);
@@ -409,7 +436,8 @@ static bool sort_group_bits(const LaunchSet &L, int end_bit, hipStream_t s) {
         if (L.sort_tiles == 0) return false;
         return radix_passes<K, BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt,
                                      (uint64_t)L.sort_tiles * RS_TILE, L.sort_tiles, L.segs,
-                                     (unsigned)L.njobs, L.tile_seg, 0, end_bit, L.sort_temp, s);
+                                     (unsigned)L.njobs, L.tile_seg, 0, end_bit, L.sort_temp, s,
+                                     nullptr, nullptr, nullptr, L.max_seg_tiles);
     }
     if (L.sort_n == 0) return false;
     return radix_passes<K, BITS>((K *)L.keys, L.vals, (K *)L.keys_alt, L.vals_alt, L.sort_n,
