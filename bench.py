@@ -134,7 +134,8 @@ def bench_serving(args):
     ctx = bt.Context(0)
     # warm-up request
     serving.serve([serving.Request(-1, reqs[0].x[:2000], epochs=20)], ctx, want_kl=False)
-    res_b = serving.serve(reqs, ctx, step_epochs=args.step_epochs)
+    res_b = serving.serve(reqs, ctx, step_epochs=args.step_epochs,
+                          graph=not getattr(args, "no_graph", False))
     sb = serving.latency_summary(res_b)
     log(f"batched: {sb}; loop {serving.serve.last_stats}")
     res_p = serving.serve_plain(reqs, ctx)
@@ -240,7 +241,8 @@ def main():
     ap.add_argument("--serial-epoch", action="store_true",
                     help="run each epoch's kernels in order on one stream (per-kernel profiles)")
     ap.add_argument("--no-graph", action="store_true",
-                    help="batched workload: plain launches instead of the captured epoch graph")
+                    help="batched and serving workloads: plain launches instead of the captured "
+                         "epoch graph")
     ap.add_argument("--step-epochs", type=int, default=50,
                     help="serving workload: epochs between admissions")
     ap.add_argument("--short", action="store_true",

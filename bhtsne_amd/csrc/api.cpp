@@ -401,6 +401,20 @@ void ensure_state_buffers(bht_job *j) {
 // Jobs from this size on run their epochs forked and size their FFT launches dynamically.
 constexpr uint64_t DYN_FFT_MIN_POINTS = 32768;
 
+// The global-memory FFT lines and the grid-length report of a 2-D FIt-SNE job: with them its FFT
+// launches may hold less LDS than its grid capacity needs (lines that outgrow a launch take the
+// global lines, with the same arithmetic).
+void ensure_fft_lines(bht_job *j) {
+    if (j->dim() != 2) return;
+    j->fft_fb.ensure(j->ctx, (size_t)FFT_FB_BLOCKS * 2 * j->len_cap * sizeof(float2));
+    if (!j->host_len) {
+        BHT_HIP(hipHostMalloc((void **)&j->host_len, 64, hipHostMallocDefault));
+        *j->host_len = 0;
+    }
+    j->hd.fft_fb = j->fft_fb.as<float2>();
+    j->hd.host_len = j->host_len;
+}
+
 void ensure_fitsne_workspace(bht_job *j) {
     ensure_state_buffers(j);
     bht_ctx *c = j->ctx;
@@ -434,13 +448,7 @@ void ensure_fitsne_workspace(bht_job *j) {
     d.kspec = j->kspec.as<float>();
     d.part = j->part.as<float>();
     if (j->dim() == 2 && n >= DYN_FFT_MIN_POINTS) {
-        j->fft_fb.ensure(c, (size_t)FFT_FB_BLOCKS * 2 * j->len_cap * sizeof(float2));
-        if (!j->host_len) {
-            BHT_HIP(hipHostMalloc((void **)&j->host_len, 64, hipHostMallocDefault));
-            *j->host_len = 0;
-        }
-        d.fft_fb = j->fft_fb.as<float2>();
-        d.host_len = j->host_len;
+        ensure_fft_lines(j);
         if (n >= SORT7_MIN_KEYS) {
             j->keys_fin.ensure(c, n * 4);
             j->svals_fin.ensure(c, n * 4);
@@ -1018,12 +1026,22 @@ void epoch_tail(bht_job *j, uint64_t epoch) {
 // take the global-memory fallback, which is several times slower (with a quarter more than the
 // report, a 300-epoch run at 1M points took 7% longer).
 int fft_cap_for(const bht_job *j) {
-    if (!j->host_len || !j->hd.fft_fb) return j->len_cap;
+    if (!j->hd.host_len || !j->hd.fft_fb) return j->len_cap;
     if (j->params.flags & BHT_FLAG_FFT_GLOBAL_LINES) return 64;
     const int seen = __atomic_load_n(j->host_len, __ATOMIC_ACQUIRE);
     if (seen <= 0) return j->len_cap;
     const int want = (4 * seen + 63) / 64 * 64;
     return std::min(want, j->len_cap);
+}
+
+// LDS points per block of a batch's 2-D FFT launches for member j: four times the length its grid
+// last reported (before any report, the length of its first grid, min_boxes per axis), at most
+// its capacity.  The batch grows a group's launches when a member's report outgrows them.
+int batch_fft_cap(const bht_job *j) {
+    if (j->params.flags & BHT_FLAG_FFT_GLOBAL_LINES) return 64;
+    int seen = j->hd.host_len ? __atomic_load_n(j->hd.host_len, __ATOMIC_ACQUIRE) : 0;
+    if (seen <= 0) seen = (int)next_smooth(6 * (int64_t)std::max<int>(j->hd.min_boxes, 1) - 1);
+    return std::min((4 * seen + 63) / 64 * 64, j->len_cap);
 }
 
 // One epoch's launches (src/lib.rs:961-1014: repulsion + attraction, gradient step, zero-mean):
@@ -1259,7 +1277,7 @@ void run_epochs(bht_job *j, uint64_t count, bht_epoch_cb cb, void *user) {
     const TuneChoice ch = count > 0 ? epoch_choice(j, L, fit) : TuneChoice{0, 0};
     j->last_choice = ch.fork ? ch.queue_blocks : -1;
     if (j->timing) j->timed_fork = ch.fork != 0;
-    const bool bounded = fit && j->host_len != nullptr;
+    const bool bounded = fit && j->hd.host_len != nullptr;
     for (auto &r : j->ahead_rec) r = false;
     for (uint64_t e = 0; e < count; e++) {
         // at most 16 epochs in flight (every 4th epoch records an event)
@@ -1515,6 +1533,10 @@ struct bht_batch {
     uint64_t epochs_launched = 0;
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // a batch with 2-D FIt-SNE groups keeps at most 16 epochs queued (an event every 4th epoch),
+    // so that the members' grid-length reports are recent when their FFT launches are resized
+    hipEvent_t ahead[4] = {};
+    bool ahead_rec[4] = {};
 
     // non-blocking: synchronous copies on the null stream by another host thread (a request
     // loop preparing jobs) neither wait for the batch nor disturb a capture in progress
@@ -1524,9 +1546,12 @@ struct bht_batch {
         BHT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         BHT_HIP(hipEventCreateWithFlags(&ev_fork, HANDOVER_EVENT));
         BHT_HIP(hipEventCreateWithFlags(&ev_join, HANDOVER_EVENT));
+        for (auto &e : ahead) BHT_HIP(hipEventCreateWithFlags(&e, HANDOVER_EVENT));
     }
     ~bht_batch() {
         drop_graph();
+        for (auto &e : ahead)
+            if (e) hipEventDestroy(e);
         if (ev_fork) hipEventDestroy(ev_fork);
         if (ev_join) hipEventDestroy(ev_join);
         if (side) hipStreamDestroy(side);
@@ -1601,10 +1626,11 @@ void batch_rebuild(bht_batch *b) {
             BHT_HIP(hipMemcpyAsync(g.tile_seg.p, tile_seg.data(), tile_seg.size() * 4,
                                    hipMemcpyHostToDevice, b->stream));
         // table entries: batch grid sizes, sort fields into the arena
-        int len_cap = 1;
+        int len_cap = 1, dyn_cap = 1;
         for (size_t q = 0; q < g.jobs.size(); q++) {
             bht_job *j = g.jobs[q];
             fill_jobdev(j);
+            if (!bh && g.dim == 2) dyn_cap = std::max(dyn_cap, batch_fft_cap(j));
             JobDev d = j->hd;
             job_block_counts(d, true);
             const uint64_t o = segs[q].base;
@@ -1648,7 +1674,9 @@ void batch_rebuild(bht_batch *b) {
         L.dim = g.dim;
         L.njobs = (int)g.jobs.size();
         L.tw = &c->tw;
-        L.fft_cap = len_cap;
+        // 2-D groups: LDS for the lengths reached (every member has the global lines); the 1-D
+        // line convolution holds its capacity
+        L.fft_cap = !bh && g.dim == 2 ? dyn_cap : len_cap;
         L.keys = g.keys.p;
         L.keys_alt = g.keys_alt.p;
         L.vals = g.vals.as<uint32_t>();
@@ -1760,10 +1788,27 @@ void batch_admit(bht_batch *b, bht_job *j, int method, float theta) {
         validate_init(j);
         prepare_run(j, method, theta, cached);
     }
+    // a 2-D FIt-SNE member's FFT launches are sized by the lengths the grids reach
+    if (method == FIT_FITSNE) ensure_fft_lines(j);
     j->in_batch = true;
     j->lazy_ready = false;
     b->members.push_back(j);
     b->dirty = true;
+}
+
+// Grows a 2-D FIt-SNE group's FFT launches when a member's reported grid length has outgrown
+// them (its lines meanwhile took the global lines); the next epoch captures a new graph.
+void batch_grow_fft(bht_batch *b) {
+    for (auto &gp : b->groups) {
+        BatchGroup &g = *gp;
+        if (g.method != M_FITSNE || g.dim != 2) continue;
+        int need = 1;
+        for (bht_job *j : g.jobs) need = std::max(need, batch_fft_cap(j));
+        if (need > g.L.fft_cap) {
+            g.L.fft_cap = need;
+            b->drop_graph();
+        }
+    }
 }
 
 // Advances every member by up to max_epochs (never past the member with the fewest epochs
@@ -1781,8 +1826,24 @@ uint64_t batch_step(bht_batch *b, uint64_t max_epochs) {
             batch_rebuild(b);
             // new members were prepared on their own streams
             l2_sync(b->stream);
+        } else {
+            batch_grow_fft(b);
         }
-        for (uint64_t e = 0; e < k; e++) batch_launch_epoch(b, !rebuilt || k >= 64);
+        bool dyn = false;  // a group sizes its FFT launches by its members' reports
+        for (auto &gp : b->groups) dyn |= gp->method == M_FITSNE && gp->dim == 2;
+        for (auto &r : b->ahead_rec) r = false;
+        for (uint64_t e = 0; e < k; e++) {
+            const int slot = (int)((e / 4) % 4);
+            if (dyn && e % 4 == 0 && b->ahead_rec[slot]) {
+                BHT_HIP(hipEventSynchronize(b->ahead[slot]));
+                batch_grow_fft(b);
+            }
+            batch_launch_epoch(b, !rebuilt || k >= 64);
+            if (dyn && e % 4 == 0) {
+                BHT_HIP(hipEventRecord(b->ahead[slot], b->stream));
+                b->ahead_rec[slot] = true;
+            }
+        }
         b->epochs_launched += k;
         for (bht_job *j : b->members)
             for (uint64_t e = 0; e < k; e++) {
@@ -2558,8 +2619,8 @@ bht_status bht_job_footprint(uint64_t n, uint32_t d_in, const bht_params *params
         if (D == 2) b += 2 * nodes * len * 8;                     // packed spectra
         b += 2 * h * h * 4;                                       // kernel transforms
         b += 2 * ((n + 63) / 64) * 36 * 4;                        // spread partials
-        if (D == 2 && n >= DYN_FFT_MIN_POINTS)
-            b += (uint64_t)FFT_FB_BLOCKS * 2 * len * 8 + n * 8;   // FFT lines, sort output
+        if (D == 2) b += (uint64_t)FFT_FB_BLOCKS * 2 * len * 8;   // global FFT lines
+        if (D == 2 && n >= DYN_FFT_MIN_POINTS) b += n * 8;       // sort output
         b += pad_tile(n) * 16 + group_sort_temp_bytes(pad_tile(n));  // batch arena
     }
     *bytes = b;

@@ -70,11 +70,13 @@ def _collect(job: TSNE, res: Result, want_kl: bool, t0: float):
 
 
 def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: int = 25,
-          hbm_budget: int = 0, want_kl: bool = True, preparers: int = 4) -> List[Result]:
+          hbm_budget: int = 0, want_kl: bool = True, preparers: int = 4,
+          graph: bool = True) -> List[Result]:
     """Serves the requests with continuous batching; returns one Result per request (times in
     seconds from the start of the stream).  `preparers` threads build arriving jobs'
     affinities concurrently (each job on its own stream), taking the requests in arrival
-    order."""
+    order.  `graph=False` launches the batch's epochs plainly instead of replaying a captured
+    graph (same kernels and results; for tracing tools that cannot follow graphs)."""
     ctx = ctx or Context(0, hbm_budget)
     reqs = sorted(requests, key=lambda r: r.arrival)
     results = {r.rid: Result(r.rid, r.x.shape[0], r.d_out, r.arrival) for r in reqs}
@@ -119,7 +121,7 @@ def serve(requests: List[Request], ctx: Optional[Context] = None, step_epochs: i
     threads = [threading.Thread(target=preparer, daemon=True) for _ in range(nprep)]
     for th in threads:
         th.start()
-    batch = Batch(ctx)
+    batch = Batch(ctx, graph=graph)
     live = {}
     feeding = True
     running_preparers = nprep
