@@ -361,9 +361,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_kernel_cols(const JobDev *jobs,
             }
             __syncthreads();
             const float2 *f = eng.template fft<false>(buf, L, tw);
+            // stored by columns (kspec[v * h + u]): each column's writes here, and the column
+            // pass's reads along a column, are contiguous
             for (int u = threadIdx.x; u < h; u += FFT_THREADS) {
-                kspec[(size_t)u * h + v1] = f[u].x * inv_total;
-                if (v2 < h) kspec[(size_t)u * h + v2] = f[u].y * inv_total;
+                kspec[(size_t)v1 * h + u] = f[u].x * inv_total;
+                if (v2 < h) kspec[(size_t)v2 * h + u] = f[u].y * inv_total;
             }
             __syncthreads();
         }
@@ -435,7 +437,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const JobDev *jobs, c
             const int cc = c <= L / 2 ? c : L - c;
             for (int u = lane; u < L; u += 64) {
                 const int uu = u <= L / 2 ? u : L - u;
-                const float kv = kspec[(size_t)uu * h + cc];
+                const float kv = kspec[(size_t)cc * h + uu];
                 const float2 z = f[u];
                 f[u] = make_float2(z.x * kv, z.y * kv);
             }
@@ -461,7 +463,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const JobDev *jobs, c
             const int cc = c <= L / 2 ? c : L - c;
             for (int u = threadIdx.x; u < L; u += FFT_THREADS) {
                 const int uu = u <= L / 2 ? u : L - u;
-                const float kv = kspec[(size_t)uu * h + cc];
+                const float kv = kspec[(size_t)cc * h + uu];
                 float2 z = f[u];
                 f[u] = make_float2(z.x * kv, z.y * kv);
             }
