@@ -272,18 +272,27 @@ __device__ __forceinline__ int xcd_line(int k, int per) { return (k & 7) * per +
 
 // ---- kernel spectrum -------------------------------------------------------------------------
 
-// Runs body(line engine, first line, line stride) with the block's LDS when a line of L points
-// fits `cap`, else with the block's two lines of the job's fallback buffer (the first
-// FFT_FB_BLOCKS blocks only).
+// Runs body(line engine, first line, line stride, twiddles) with the block's LDS when a line of L
+// points fits `cap`, else with the block's two lines of the job's fallback buffer (the first
+// FFT_FB_BLOCKS blocks only).  When the LDS holds two lines, the length's twiddle table is copied
+// after the line once per block and every stage reads it there (the same values).
 template <typename Body>
 __device__ __forceinline__ void with_line_buffer(float2 *lds, int cap, int L, const JobDev &J,
-                                                 const BlockCtx &bc, Body body) {
+                                                 const BlockCtx &bc, const float2 *tw,
+                                                 Body body) {
     if (L <= cap) {
-        body(LdsLine{lds}, bc.lb, bc.nb);
+        const float2 *twp = tw;
+        if (2 * L <= cap) {
+            float2 *t = lds + L;
+            for (int i = threadIdx.x; i < L; i += FFT_THREADS) t[i] = tw[i];
+            __syncthreads();
+            twp = t;
+        }
+        body(LdsLine{lds}, bc.lb, bc.nb, twp);
     } else if (bc.lb < FFT_FB_BLOCKS) {
         float2 *a = J.fft_fb + (size_t)bc.lb * 2 * J.len_cap;
         body(GlobalLines{a, a + J.len_cap}, bc.lb,
-             bc.nb < FFT_FB_BLOCKS ? bc.nb : FFT_FB_BLOCKS);
+             bc.nb < FFT_FB_BLOCKS ? bc.nb : FFT_FB_BLOCKS, tw);
     }
 }
 
@@ -301,7 +310,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_kernel_rows(const JobDev *jobs,
     const int L = p.len, h = L / 2 + 1;
     const float2 *tw = tw_all + p.tw_off;
     const int npairs = dim == 2 ? (h + 1) / 2 : 1;
-    with_line_buffer(lds, cap, L, J, bc, [&](const auto &eng, int lb, int nb) {
+    with_line_buffer(lds, cap, L, J, bc, tw, [&](const auto &eng, int lb, int nb,
+                                                 const float2 *twp) {
         float2 *buf = eng.line();
         for (int q = lb; q < npairs; q += nb) {
             const int r1 = 2 * q, r2 = 2 * q + 1;
@@ -322,7 +332,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_kernel_rows(const JobDev *jobs,
                 buf[c] = make_float2(v[0], dim == 2 ? v[1] : 0.0f);
             }
             __syncthreads();
-            const float2 *f = eng.template fft<false>(buf, L, tw);
+            const float2 *f = eng.template fft<false>(buf, L, twp);
             if (dim == 2) {
                 for (int c = threadIdx.x; c < h; c += FFT_THREADS) {
                     rker[(size_t)r1 * h + c] = f[c].x;
@@ -353,7 +363,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_kernel_cols(const JobDev *jobs,
     const float inv_total = 1.0f / (float)(L * L);
     const int npairs = (h + 1) / 2;
     const int per = (npairs + 7) / 8;
-    with_line_buffer(lds, cap, L, J, bc, [&](const auto &eng, int lb, int nb) {
+    with_line_buffer(lds, cap, L, J, bc, tw, [&](const auto &eng, int lb, int nb,
+                                                 const float2 *twp) {
         float2 *buf = eng.line();
         for (int k = lb; k < 8 * per; k += nb) {
             const int q = xcd_line(k, per);
@@ -366,7 +377,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_kernel_cols(const JobDev *jobs,
                 buf[r] = make_float2(a, b);
             }
             __syncthreads();
-            const float2 *f = eng.template fft<false>(buf, L, tw);
+            const float2 *f = eng.template fft<false>(buf, L, twp);
             // stored by columns (kspec[v * h + u]): each column's writes here, and the column
             // pass's reads along a column, are contiguous
             for (int u = threadIdx.x; u < h; u += FFT_THREADS) {
@@ -393,7 +404,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rows_fwd(const JobDev *jobs, co
     const float2 *tw = tw_all + p.tw_off;
     const float2 *g2 = reinterpret_cast<const float2 *>(J.grid);
     const float inv_q = 1.0f / p.qscale;
-    with_line_buffer(lds, cap, L, J, bc, [&](const auto &eng, int lb, int nb) {
+    with_line_buffer(lds, cap, L, J, bc, tw, [&](const auto &eng, int lb, int nb,
+                                                 const float2 *twp) {
         float2 *buf = eng.line();
         for (int line = lb; line < 2 * nodes; line += nb) {
             const int g = line / nodes, r = line % nodes;
@@ -404,7 +416,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rows_fwd(const JobDev *jobs, co
                 buf[c] = make_float2(v.x * sa, v.y * sb);
             }
             __syncthreads();
-            const float2 *f = eng.template fft<false>(buf, L, tw);
+            const float2 *f = eng.template fft<false>(buf, L, twp);
             float2 *out = spec + g * gstride + (size_t)r * L;
             for (int c = threadIdx.x; c < L; c += FFT_THREADS) out[c] = f[c];
             __syncthreads();
@@ -464,7 +476,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const JobDev *jobs, c
         return;
     }
     const int per = (2 * L + 7) / 8;
-    with_line_buffer(lds, cap, L, J, bc, [&](const auto &eng, int lb, int nb) {
+    with_line_buffer(lds, cap, L, J, bc, tw, [&](const auto &eng, int lb, int nb,
+                                                 const float2 *twp) {
         float2 *buf = eng.line();
         for (int k = lb; k < 8 * per; k += nb) {
             const int line = xcd_line(k, per);
@@ -474,7 +487,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const JobDev *jobs, c
             for (int r = threadIdx.x; r < L; r += FFT_THREADS)
                 buf[r] = r < nodes ? col[(size_t)r * L] : make_float2(0.f, 0.f);
             __syncthreads();
-            float2 *f = eng.template fft<false>(buf, L, tw);
+            float2 *f = eng.template fft<false>(buf, L, twp);
             const int cc = c <= L / 2 ? c : L - c;
             for (int u = threadIdx.x; u < L; u += FFT_THREADS) {
                 const int uu = u <= L / 2 ? u : L - u;
@@ -483,7 +496,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols_conv(const JobDev *jobs, c
                 f[u] = make_float2(z.x * kv, z.y * kv);
             }
             __syncthreads();
-            const float2 *r2 = eng.template fft<true>(f, L, tw);
+            const float2 *r2 = eng.template fft<true>(f, L, twp);
             for (int r = threadIdx.x; r < nodes; r += FFT_THREADS) col[(size_t)r * L] = r2[r];
             __syncthreads();
         }
@@ -504,7 +517,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rows_inv(const JobDev *jobs, co
     const float2 *tw = tw_all + p.tw_off;
     float2 *g2 = reinterpret_cast<float2 *>(J.grid);
     const float q = p.qscale;
-    with_line_buffer(lds, cap, L, J, bc, [&](const auto &eng, int lb, int nb) {
+    with_line_buffer(lds, cap, L, J, bc, tw, [&](const auto &eng, int lb, int nb,
+                                                 const float2 *twp) {
         float2 *buf = eng.line();
         for (int line = lb; line < 2 * nodes; line += nb) {
             const int g = line / nodes, r = line % nodes;
@@ -512,7 +526,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_rows_inv(const JobDev *jobs, co
             const float2 *in = spec + g * gstride + (size_t)r * L;
             for (int c = threadIdx.x; c < L; c += FFT_THREADS) buf[c] = in[c];
             __syncthreads();
-            const float2 *f = eng.template fft<true>(buf, L, tw);
+            const float2 *f = eng.template fft<true>(buf, L, twp);
             for (int c = threadIdx.x; c < nodes; c += FFT_THREADS)
                 g2[2 * ((size_t)r * nodes + c) + g] = make_float2(f[c].x * sa, f[c].y * sb);
             __syncthreads();
